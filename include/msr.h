@@ -1,0 +1,164 @@
+/*
+ * msr.h -- C ABI of the MI355X (gfx950) Toeplitz subspace normal operator,
+ * subspace backprojection and device conjugate gradient.
+ *
+ * Drop-in boundary for MRISubspaceRecon.jl (repo MagneticResonanceImaging/
+ * MRFingerprintingRecon.jl): every entry point below names the reference
+ * function it replaces.  The reference binds its GPU path by multiple dispatch
+ * on CuArray inside a package extension (ext/MRISubspaceReconCUDAExt); an AMD
+ * extension binds these symbols with `ccall` instead.
+ *
+ * Conventions
+ *  - Arrays are device pointers in Julia column-major layout with the
+ *    reference's shapes: trj (d, Nsamp, Nt) Float32, sample_mask (Nsamp, Nt)
+ *    Bool (1 byte), data (Nsamp, Nt, C) ComplexF32, U (Nt, K) ComplexF32 or
+ *    Float32, images (N1, N2[, N3], K) ComplexF32, cmaps one (N1, N2[, N3])
+ *    ComplexF32 array per coil, packed kernel (P, Nkm) with P = K(K+1)/2
+ *    (ComplexF32 for complex U, Float32 for real U), kmask Int64 1-based.
+ *  - Complex numbers are interleaved float pairs (ComplexF32 == float2).
+ *  - Every call is ordered on the given HIP stream (hipStream_t passed as
+ *    void*, NULL = default stream).  No call synchronises the host unless its
+ *    comment says so.
+ *  - Return value: MSR_OK or an error code; msr_last_error() returns the
+ *    message of the calling thread's last error.  Error codes map to the Julia
+ *    exceptions the reference throws (ArgumentError, AssertionError,
+ *    ErrorException).
+ *  - Handles are not re-entrant: one in-flight call per operator handle, as
+ *    the reference's operator shares its scratch arrays
+ *    (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:36-37).
+ */
+#ifndef MSR_H
+#define MSR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSR_OK 0
+#define MSR_ERR_ARGUMENT 1    /* ArgumentError (src/BackProjection.jl:183-202) */
+#define MSR_ERR_ASSERTION 2   /* AssertionError (src/NFFTNormalOp.jl:59-61,107,128) */
+#define MSR_ERR_ERROR 3       /* ErrorException (src/BackProjection.jl:179) */
+#define MSR_ERR_UNSUPPORTED 4 /* size or option this build has no kernel for */
+#define MSR_ERR_NOMEM 5       /* device allocation failed */
+#define MSR_ERR_HIP 6         /* HIP runtime error */
+
+/* density_compensation modes (src/BackProjection.jl:169-181) */
+#define MSR_DCF_NONE 0
+#define MSR_DCF_RADIAL_2D 1
+#define MSR_DCF_RADIAL_3D 2
+#define MSR_DCF_ARRAY 3
+
+/* conjugate-gradient execution modes */
+#define MSR_CG_DEVICE 0 /* scalars stay on the device (FP64), no host sync per iteration */
+#define MSR_CG_HOST 1   /* IterativeSolvers-style: dot/norm read back to the host each iteration */
+#define MSR_CG_GRAPH 2  /* as MSR_CG_DEVICE, iterations captured once into a hipGraph and replayed */
+
+typedef struct msr_ctx msr_ctx;
+typedef struct msr_traj msr_traj;
+typedef struct msr_normalop msr_normalop;
+
+/* ------------------------------------------------------------------ misc */
+/* Copies the calling thread's last error message into buf (NUL-terminated,
+ * truncated to len); returns the full message length. */
+size_t msr_last_error(char *buf, size_t len);
+int msr_version(void);
+
+/* Library context on one HIP device (twiddle tables, launch parameters).
+ * No reference counterpart: CUDA.jl's implicit context. */
+int msr_ctx_create(int device, msr_ctx **out);
+int msr_ctx_destroy(msr_ctx *ctx);
+
+/* --------------------------------------------------------- trajectories */
+/* Sample selection shared by the kernel build and the backprojection:
+ * trj[:, sample_mask] and the per-frame sample counts / offsets
+ * (src/NFFTNormalOp.jl:126-131; ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:95-97;
+ * ext/MRISubspaceReconCUDAExt/BackProjection.jl:37-48).  trj: (d, nsamp, nt)
+ * Float32 device; mask: (nsamp, nt) uint8 device or NULL (all samples).
+ * Synchronises the host once (the masked sample count is a host value, as in
+ * the reference's sum(nsamp_t)).  AssertionError if the mask removes all
+ * samples (src/NFFTNormalOp.jl:128). */
+int msr_traj_create(msr_ctx *ctx, int d, int64_t nsamp, int64_t nt, const float *trj,
+                    const uint8_t *mask, void *stream, msr_traj **out);
+int msr_traj_info(const msr_traj *traj, int64_t *n_masked, int64_t *nt);
+int msr_traj_destroy(msr_traj *traj);
+
+/* ------------------------------------------------------ normal operator */
+/* NFFTNormalOp(img_shape, trj, U; cmaps, sample_mask)
+ * (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:1-13, which calls
+ * calculate_kernel_noncartesian :89-191 and calculate_kmask_indcs :75-86).
+ * U: (Nt, K) device, ComplexF32 (u_real = 0) or Float32 (u_real = 1; builds
+ * the real symmetric kernel).  cmaps: host array of ncoils device pointers, or
+ * NULL with ncoils = 0 for the reference default cmaps = (1,).
+ * AssertionError if a grid size 2N is not 2,3,5-smooth
+ * (src/NFFTNormalOp.jl:107). */
+int msr_normalop_create_from_traj(msr_ctx *ctx, const msr_traj *traj, int d,
+                                  const int64_t *img_shape, int K, const void *U, int u_real,
+                                  int ncoils, const void *const *cmaps, void *stream,
+                                  msr_normalop **out);
+
+/* NFFTNormalOp(img_shape, Λ, kmask_indcs; cmaps)
+ * (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:22-70): lambda is the packed
+ * (P, Nkm) kernel on the device, kmask the 1-based Int64 indices into the 2N
+ * grid (device).  K is derived from P as (isqrt(8P+1)-1)/2 (:33).
+ * AssertionError if a kmask entry is outside 1..prod(2N) (:28-30). */
+int msr_normalop_create(msr_ctx *ctx, int d, const int64_t *img_shape, int64_t P,
+                        int lambda_real, const void *lambda, const int64_t *kmask, int64_t nkm,
+                        int ncoils, const void *const *cmaps, void *stream, msr_normalop **out);
+
+/* Sizes of the operator's packed kernel (op.prod!.A.Λ and .kmask_indcs of the
+ * reference, test/reconstruct_radial_gpu.jl:104). */
+int msr_normalop_info(const msr_normalop *op, int *K, int64_t *P, int64_t *nkm, int *lambda_real);
+
+/* Writes the packed kernel (P, Nkm) and the 1-based kmask into caller device
+ * buffers of the sizes msr_normalop_info reports. */
+int msr_normalop_export_kernel(const msr_normalop *op, void *lambda_out, int64_t *kmask_out,
+                               void *stream);
+
+/* mul!(x, ::_NFFTNormalOp, b, α, β) (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:193-218):
+ * x = β x + α AᴴA b with x, b (N..., K) ComplexF32 on the device; β == 0
+ * overwrites x without reading it (NaN in x is ignored, :195-199). */
+int msr_normalop_apply(msr_normalop *op, void *x, const void *b, double alpha_re,
+                       double alpha_im, double beta_re, double beta_im, void *stream);
+
+/* The reference's GPU apply order, one coil at a time (fill, pad, forward FFT,
+ * kernel multiply re-reading Λ, inverse FFT, crop-accumulate per coil, :204-216),
+ * on this library's kernels.  Same result as msr_normalop_apply; kept as the
+ * plain one-recon-at-a-time GPU path the fused apply is measured against. */
+int msr_normalop_apply_percoil(msr_normalop *op, void *x, const void *b, double alpha_re,
+                               double alpha_im, double beta_re, double beta_im, void *stream);
+
+int msr_normalop_destroy(msr_normalop *op);
+
+/* --------------------------------------------------------- backprojection */
+/* calculate_backprojection(data, trj, cmaps; U, sample_mask, density_compensation)
+ * (ext/MRISubspaceReconCUDAExt/BackProjection.jl:36-71; per-coil variant :1-34
+ * when cmaps == NULL, which takes img_shape and writes (N..., K, C)).
+ * data: (nsamp, nt, ncoils) ComplexF32 device (the full, unmasked array);
+ * U: (Nt, K) device or NULL for the reference default I(Nt) (then K = Nt).
+ * dcf_mode: MSR_DCF_*; dcf: masked-sample weights (Float32, length n_masked)
+ * for MSR_DCF_ARRAY.  out: (N..., K) or (N..., K, C) ComplexF32 device.
+ * ArgumentError on shape mismatch (src/BackProjection.jl:183-202);
+ * ErrorException on an unknown dcf_mode (:179). */
+int msr_backprojection(msr_ctx *ctx, const msr_traj *traj, int d, const int64_t *img_shape,
+                       const void *data, int ncoils, int K, const void *U, int u_real,
+                       const void *const *cmaps, int dcf_mode, const float *dcf, void *out,
+                       void *stream);
+
+/* ------------------------------------------------------------------- CG */
+/* cg!(x, A, b; maxiter, reltol, abstol) of IterativeSolvers.jl 0.9 (caller of
+ * the apply: test/reconstruct_radial_gpu.jl:99, src/CoilMaps.jl:136) run on
+ * the device with FP64 inner products, norms and step lengths.  initially_zero
+ * = 1 reproduces cg(A, b) (x starts at 0 and is not read).  mode: MSR_CG_*.
+ * iters_out / residual_out (host, may be NULL) receive the iteration count and
+ * final residual norm; reading them synchronises the stream. */
+int msr_cg(msr_normalop *op, void *x, const void *b, int maxiter, double reltol, double abstol,
+           int initially_zero, int mode, int *iters_out, double *residual_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSR_H */

@@ -1,3 +1,398 @@
-"""MI355X-native Toeplitz subspace normal operator and subspace backprojection
-(drop-in for MRISubspaceRecon.jl's NFFTNormalOp / calculate_backprojection)."""
-from . import synth  # noqa: F401
+"""MI355X-native Toeplitz subspace normal operator, subspace backprojection and
+device CG: the host-side mirror of MRISubspaceRecon.jl's GPU API
+(repo MagneticResonanceImaging/MRFingerprintingRecon.jl,
+ext/MRISubspaceReconCUDAExt) over the C ABI of libmsr.so (include/msr.h).
+
+Arrays are torch tensors on a HIP device in the row-major view of the
+reference's Julia arrays (every shape reversed, identical memory):
+  trj         Julia (d, Nsamp, Nt)       -> (Nt, Nsamp, d) float32
+              4D    (d, Nr, Ncyc, Nt)     -> (Nt, Ncyc, Nr, d)
+  sample_mask Julia (Nsamp, Nt)           -> (Nt, Nsamp) bool
+  data        Julia (Nsamp, Nt, C)        -> (C, Nt, Nsamp) complex64
+  U           Julia (Nt, K)               -> (K, Nt) complex64 or float32
+  images      Julia (N1, N2[, N3], K)     -> (K[, N3], N2, N1) complex64
+  cmaps       Vector of Julia (N1, N2...) -> list of (..., N2, N1) tensors or a
+                                              stacked (C, ..., N2, N1) tensor
+  packed Λ    Julia (P, Nkm)              -> (Nkm, P)
+The operator's vectors are flat (Julia vec) tensors of length prod(N) * K.
+There is no CPU path: every call runs HIP kernels and raises if libmsr.so is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+from . import _lib, synth  # noqa: F401
+from ._lib import ArgumentError, ErrorException  # noqa: F401
+
+__all__ = ["NFFTNormalOp", "calculate_backprojection", "cg", "cg_", "mul_", "LinearOperator",
+           "ArgumentError", "ErrorException", "synth"]
+
+_ctxs = {}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(device):
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ctx(device_index: int):
+    if device_index not in _ctxs:
+        h = ctypes.c_void_p()
+        _lib.call("msr_ctx_create", device_index, ctypes.byref(h))
+        _ctxs[device_index] = h
+    return _ctxs[device_index]
+
+
+def _dev_tensor(t, name, dtype=None):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError(f"`{name}` must be a torch tensor on a HIP device (the GPU path "
+                        "dispatches on device arrays, like the reference's CuArray methods)")
+    if dtype is not None and t.dtype != dtype:
+        t = t.to(dtype)
+    return t.contiguous()
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _i64arr(vals):
+    return (ctypes.c_int64 * len(vals))(*[int(v) for v in vals])
+
+
+def _cmaps_list(cmaps, img_shape=None):
+    torch = _torch()
+    if cmaps is None:
+        return []
+    if isinstance(cmaps, torch.Tensor):
+        return [cmaps[i] for i in range(cmaps.shape[0])]
+    if isinstance(cmaps, (tuple, list)) and len(cmaps) == 1 and not hasattr(cmaps[0], "shape"):
+        return []  # reference default cmaps = (1,)
+    return list(cmaps)
+
+
+class _Traj:
+    """Masked, frame-major sample list on the device (msr_traj)."""
+
+    def __init__(self, trj, sample_mask=None):
+        torch = _torch()
+        trj = _dev_tensor(trj, "trj", torch.float32)
+        if trj.dim() == 4:  # (Nt, Ncyc, Nr, d): 4D wrapper (src/NFFTNormalOp.jl:45-49)
+            trj = trj.reshape(trj.shape[0], -1, trj.shape[3])
+        if trj.dim() != 3:
+            raise ArgumentError("trj must be (Nt, Nsamp, d) or (Nt, Ncyc, Nr, d)")
+        self.nt, self.nsamp, self.d = trj.shape
+        self.device = trj.device
+        mptr = None
+        if sample_mask is not None:
+            m = _dev_tensor(sample_mask, "sample_mask")
+            m = m.reshape(self.nt, -1)
+            if m.shape[1] != self.nsamp:
+                raise ArgumentError("sample_mask does not match the trajectory")
+            m = m.to(torch.uint8).contiguous()
+            self._mask = m
+            mptr = _ptr(m)
+        self._trj = trj
+        h = ctypes.c_void_p()
+        _lib.call("msr_traj_create", _ctx(self.device.index), self.d, self.nsamp, self.nt,
+                  _ptr(trj), mptr, _stream_ptr(self.device), ctypes.byref(h))
+        self.h = h
+        n = ctypes.c_int64()
+        _lib.call("msr_traj_info", h, ctypes.byref(n), None)
+        self.M = n.value
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib.load().msr_traj_destroy(self.h)
+            self.h = None
+
+
+class _NFFTNormalOp:
+    """State of the operator (the reference's _NFFTNormalOp struct,
+    src/NFFTNormalOp.jl:91-104): ``shape``, ``Ncoeff``, ``Λ`` (packed (Nkm, P)
+    device tensor) and ``kmask_indcs`` (1-based int64 device tensor)."""
+
+    def __init__(self, handle, img_shape, device, cmaps_keep):
+        self.h = handle
+        self.shape = tuple(int(s) for s in img_shape)
+        self.device = device
+        self._cmaps_keep = cmaps_keep
+        K = ctypes.c_int()
+        P = ctypes.c_int64()
+        nkm = ctypes.c_int64()
+        lr = ctypes.c_int()
+        _lib.call("msr_normalop_info", handle, ctypes.byref(K), ctypes.byref(P),
+                  ctypes.byref(nkm), ctypes.byref(lr))
+        self.Ncoeff = K.value
+        self.P = P.value
+        self.nkm = nkm.value
+        self.lambda_real = bool(lr.value)
+        self._lam = None
+        self._km = None
+
+    def _export(self):
+        torch = _torch()
+        dt = torch.float32 if self.lambda_real else torch.complex64
+        lam = torch.empty((self.nkm, self.P), dtype=dt, device=self.device)
+        km = torch.empty((self.nkm,), dtype=torch.int64, device=self.device)
+        _lib.call("msr_normalop_export_kernel", self.h, _ptr(lam), _ptr(km),
+                  _stream_ptr(self.device))
+        self._lam, self._km = lam, km
+
+    @property
+    def Λ(self):
+        if self._lam is None:
+            self._export()
+        return self._lam
+
+    @property
+    def kmask_indcs(self):
+        if self._km is None:
+            self._export()
+        return self._km
+
+    def mul_(self, x, b, alpha=1.0, beta=0.0, percoil=False):
+        """mul!(x, S, b, α, β) (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:193-218)."""
+        a = complex(alpha)
+        bt = complex(beta)
+        name = "msr_normalop_apply_percoil" if percoil else "msr_normalop_apply"
+        _lib.call(name, self.h, _ptr(x), _ptr(b), a.real, a.imag, bt.real, bt.imag,
+                  _stream_ptr(self.device))
+        return x
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib.load().msr_normalop_destroy(self.h)
+            self.h = None
+
+
+class _Prod:
+    """The ``prod!`` closure of the reference's LinearOperator; ``.A`` is the
+    captured _NFFTNormalOp (tests read ``op.prod!.A.Λ``,
+    test/reconstruct_radial_gpu.jl:104)."""
+
+    def __init__(self, A):
+        self.A = A
+
+    def __call__(self, res, x, alpha=1.0, beta=0.0):
+        return self.A.mul_(res, x, alpha, beta)
+
+
+class LinearOperator:
+    """LinearOperator(Complex{T}, n, n, true, true, prod!, nothing, prod!; S)
+    (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:59-69)."""
+
+    def __init__(self, A: _NFFTNormalOp):
+        self.A = A
+        n = math.prod(A.shape) * A.Ncoeff
+        self.nrow = self.ncol = n
+        self.symmetric = True
+        self.hermitian = True
+        self.prod = _Prod(A)
+        self.tprod = None
+        self.ctprod = self.prod
+
+    @property
+    def shape(self):
+        return (self.nrow, self.ncol)
+
+    def mul_(self, res, v, alpha=1.0, beta=0.0):
+        return self.prod(res, v, alpha, beta)
+
+    def __matmul__(self, v):
+        torch = _torch()
+        out = torch.empty_like(v)
+        return self.prod(out, v.contiguous(), 1.0, 0.0)
+
+    def __repr__(self):
+        return (f"Linear operator  nrow: {self.nrow}  ncol: {self.ncol}  eltype: ComplexF32  "
+                f"symmetric: true  hermitian: true")
+
+
+def mul_(res, A, v, alpha=1.0, beta=0.0):
+    """LinearAlgebra.mul!(res, A, v, α, β)."""
+    if isinstance(A, LinearOperator):
+        return A.mul_(res, v, alpha, beta)
+    return A.mul_(res, v, alpha, beta)
+
+
+def NFFTNormalOp(img_shape, a, b, cmaps=None, sample_mask=None, verbose=False):
+    """NFFTNormalOp(img_shape, trj, U; cmaps, sample_mask, verbose) or
+    NFFTNormalOp(img_shape, Λ, kmask_indcs; cmaps).
+
+    trj/U form: ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:1-20 (builds the
+    kernel on the device; real float32 U gives the real symmetric kernel).
+    Λ/kmask form: :22-70 (Λ packed (Nkm, P), kmask_indcs 1-based int64)."""
+    torch = _torch()
+    img_shape = tuple(int(s) for s in img_shape)
+    d = len(img_shape)
+    cl = [_dev_tensor(c, "cmaps", torch.complex64) for c in _cmaps_list(cmaps)]
+    carr = (ctypes.c_void_p * max(1, len(cl)))(*[c.data_ptr() for c in cl]) if cl else None
+    if isinstance(b, torch.Tensor) and b.dtype in (torch.int64, torch.int32) and b.dim() == 1:
+        lam = _dev_tensor(a, "Λ")
+        km = _dev_tensor(b, "kmask_indcs", torch.int64)
+        if lam.dim() != 2:
+            raise ArgumentError("Λ must be packed (Nkm, P)")
+        if km.numel() != lam.shape[0]:
+            raise AssertionError("length(kmask_indcs) must equal size(Λ, 2) (Julia)")
+        real = not lam.is_complex()
+        lam = lam.to(torch.float32 if real else torch.complex64).contiguous()
+        dev = lam.device
+        h = ctypes.c_void_p()
+        _lib.call("msr_normalop_create", _ctx(dev.index), d, _i64arr(img_shape), lam.shape[1],
+                  int(real), _ptr(lam), _ptr(km), km.numel(), len(cl), carr,
+                  _stream_ptr(dev), ctypes.byref(h))
+        return LinearOperator(_NFFTNormalOp(h, img_shape, dev, cl))
+    traj = a if isinstance(a, _Traj) else _Traj(a, sample_mask)
+    U = _dev_tensor(b, "U")
+    real = not U.is_complex()
+    U = U.to(torch.float32 if real else torch.complex64).contiguous()
+    if U.dim() == 1:
+        U = U.reshape(1, -1)
+    K = U.shape[0]
+    if U.shape[1] != traj.nt:
+        raise ArgumentError(f"U has {U.shape[1]} time points, trj {traj.nt}")
+    h = ctypes.c_void_p()
+    _lib.call("msr_normalop_create_from_traj", _ctx(traj.device.index), traj.h, d,
+              _i64arr(img_shape), K, _ptr(U), int(real), len(cl), carr,
+              _stream_ptr(traj.device), ctypes.byref(h))
+    return LinearOperator(_NFFTNormalOp(h, img_shape, traj.device, cl))
+
+
+def _test_dimension(data, trj_nt, trj_nsamp, trj_d, U_nt, img_shape, ncoil):
+    """test_dimension (src/BackProjection.jl:183-202), on reversed shapes."""
+    C, nt, ns = data.shape
+    if U_nt != nt:
+        raise ArgumentError(f"The second dimension of data ({nt}) and the first one of U "
+                            f"({U_nt}) do not match. Both should be number of time points.")
+    if trj_d != len(img_shape):
+        raise ArgumentError(f"`cmaps` contains {len(img_shape)} image plus one coil dimension, "
+                            f"yet the 1ˢᵗ dimension of each trj is of length {trj_d}.")
+    if trj_nsamp != ns:
+        raise ArgumentError(f"The 2ⁿᵈ dimension of each `trj` is {trj_nsamp} and the 1ˢᵗ "
+                            f"dimension of `data` is {ns}.")
+    if trj_nt != nt:
+        raise ArgumentError(f"`trj` contains {trj_nt} time frames, while data consists of {nt}.")
+    if ncoil is not None and ncoil != C:
+        raise ArgumentError(f"The last dimension of `cmaps` is {ncoil} and the 3ⁿᵈ dimension "
+                            f"of data is {C}.")
+
+
+def calculate_backprojection(data, trj, cmaps_or_img_shape, U=None, sample_mask=None,
+                             density_compensation="none", verbose=False):
+    """calculate_backprojection(data, trj, cmaps; U, sample_mask,
+    density_compensation) -> (K, *N[::-1]) coil-combined
+    (ext/MRISubspaceReconCUDAExt/BackProjection.jl:36-71), or with img_shape
+    -> (C, K, *N[::-1]) per coil (:1-34).  4D data/trj wrapper :74-79."""
+    torch = _torch()
+    data = _dev_tensor(data, "data", torch.complex64)
+    if data.dim() == 4:  # (C, Nt, Ncyc, Nr)
+        data = data.reshape(data.shape[0], data.shape[1], -1)
+    traj = trj if isinstance(trj, _Traj) else _Traj(trj, sample_mask)
+    per_coil = isinstance(cmaps_or_img_shape, (tuple, list)) and len(cmaps_or_img_shape) > 0 \
+        and not hasattr(cmaps_or_img_shape[0], "shape")
+    if per_coil:
+        img_shape = tuple(int(s) for s in cmaps_or_img_shape)
+        cl = []
+        ncoil = None
+    else:
+        cl = [_dev_tensor(c, "cmaps", torch.complex64) for c in _cmaps_list(cmaps_or_img_shape)]
+        img_shape = tuple(int(s) for s in cl[0].shape[::-1])
+        ncoil = len(cl)
+    if U is None:
+        K = traj.nt
+        uptr = None
+        real = 0
+        u_nt = traj.nt
+    else:
+        U = _dev_tensor(U, "U")
+        if U.dim() == 1:
+            U = U.reshape(1, -1)
+        real = int(not U.is_complex())
+        U = U.to(torch.float32 if real else torch.complex64).contiguous()
+        K, u_nt = U.shape
+        uptr = _ptr(U)
+    _test_dimension(data, traj.nt, traj.nsamp, traj.d, u_nt, img_shape, ncoil)
+    C = data.shape[0]
+    dcf_ptr = None
+    if isinstance(density_compensation, str):
+        if density_compensation not in _lib.DCF_MODES:
+            raise ErrorException("`density_compensation` can only be `:radial_3D`, "
+                                 "`:radial_2D`, `:none`, or of type  `AbstractArray`")
+        mode = _lib.DCF_MODES[density_compensation]
+    else:
+        w = _dev_tensor(torch.as_tensor(density_compensation), "density_compensation",
+                        torch.float32).reshape(-1)
+        if w.numel() != traj.M:
+            raise ArgumentError("density_compensation array must hold one weight per "
+                                "masked sample")
+        mode = _lib.DCF_ARRAY
+        dcf_ptr = _ptr(w)
+    if per_coil:
+        out = torch.empty((C, K) + img_shape[::-1], dtype=torch.complex64, device=data.device)
+        carr = None
+    else:
+        out = torch.empty((K,) + img_shape[::-1], dtype=torch.complex64, device=data.device)
+        carr = (ctypes.c_void_p * len(cl))(*[c.data_ptr() for c in cl])
+    _lib.call("msr_backprojection", _ctx(data.device.index), traj.h, len(img_shape),
+              _i64arr(img_shape), _ptr(data), C, K, uptr, real, carr, mode, dcf_ptr, _ptr(out),
+              _stream_ptr(data.device))
+    return out
+
+
+REL_TOL_F32 = math.sqrt(2.0 ** -23)
+_side = {}
+
+
+def _side_stream(device):
+    torch = _torch()
+    if device.index not in _side:
+        _side[device.index] = torch.cuda.Stream(device)  # default priority
+    return _side[device.index]
+
+
+def cg_(x, A, b, maxiter=None, reltol=REL_TOL_F32, abstol=0.0, initially_zero=False,
+        mode="device", log=False):
+    """cg!(x, A, b; maxiter, reltol, abstol) of IterativeSolvers.jl 0.9 on the
+    device (FP64 dot products, norms and step lengths).  mode: "device"
+    (device-resident scalars), "graph" (hipGraph-captured iterations) or
+    "host" (scalars read back every iteration, per-coil applies: the plain
+    reference-order GPU path)."""
+    op = A.A if isinstance(A, LinearOperator) else A
+    if maxiter is None:
+        maxiter = A.ncol if isinstance(A, LinearOperator) else math.prod(op.shape) * op.Ncoeff
+    torch = _torch()
+    it = ctypes.c_int()
+    res = ctypes.c_double()
+    cur = torch.cuda.current_stream(op.device)
+    side = None
+    if mode == "graph" and cur.cuda_stream == 0:
+        # stream capture needs a non-default stream: run on a side stream
+        # ordered after (and before) the caller's stream
+        side = _side_stream(op.device)
+        side.wait_stream(cur)
+    st = ctypes.c_void_p((side or cur).cuda_stream)
+    _lib.call("msr_cg", op.h, _ptr(x), _ptr(b), int(maxiter), float(reltol), float(abstol),
+              int(bool(initially_zero)), _lib.CG_MODES[mode], ctypes.byref(it),
+              ctypes.byref(res), st)
+    if side is not None:
+        cur.wait_stream(side)
+    if log:
+        return x, {"iters": it.value, "residual": res.value}
+    return x
+
+
+def cg(A, b, **kw):
+    """cg(A, b; maxiter, ...) = cg!(zeros, A, b; initially_zero=true)."""
+    torch = _torch()
+    b = b.contiguous()
+    x = torch.empty_like(b)
+    return cg_(x, A, b, initially_zero=True, **kw)

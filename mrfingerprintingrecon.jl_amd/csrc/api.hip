@@ -1,0 +1,263 @@
+// extern "C" entry points (include/msr.h): argument checks, exception ->
+// status-code translation, thread-local error message.
+#include <cstring>
+#include <string>
+
+#include "internal.hpp"
+
+namespace msr {
+// normalop.hip
+msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real);
+void op_set_cmaps(msr_normalop* op, int ncoils, const void* const* cmaps, hipStream_t st);
+void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, int64_t nkm,
+                    hipStream_t st);
+void op_alloc_scratch(msr_normalop* op);
+void op_apply(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
+              bool beta_zero, hipStream_t st);
+void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
+                      bool beta_zero, hipStream_t st);
+void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hipStream_t st);
+void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
+           double abstol, bool initially_zero, int mode, int* iters_out, double* res_out,
+           hipStream_t st);
+// traj.hip
+msr_traj* traj_create(msr_ctx* ctx, int d, int64_t nsamp, int64_t nt, const float* trj,
+                      const uint8_t* mask, hipStream_t st);
+// kernel_build.hip
+void op_build_from_traj(msr_normalop* op, const msr_traj* traj, const void* U, int u_real,
+                        hipStream_t st);
+// backproj.hip
+void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, const void* data,
+                    int ncoils, int K, const void* U, int u_real, const void* const* cmaps,
+                    int dcf_mode, const float* dcf, void* out, hipStream_t st);
+
+static thread_local std::string g_err;
+void set_last_error(const std::string& s) { g_err = s; }
+}  // namespace msr
+
+using namespace msr;
+
+template <class F>
+static int guarded(F&& f) {
+    try {
+        f();
+        return MSR_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return MSR_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return MSR_ERR_ERROR;
+    } catch (...) {
+        set_last_error("unknown error");
+        return MSR_ERR_ERROR;
+    }
+}
+
+#define MSR_NOTNULL(p) MSR_REQUIRE((p) != nullptr, MSR_ERR_ARGUMENT, "%s is NULL", #p)
+
+extern "C" {
+
+size_t msr_last_error(char* buf, size_t len) {
+    if (buf && len) {
+        size_t n = std::min(len - 1, g_err.size());
+        memcpy(buf, g_err.data(), n);
+        buf[n] = 0;
+    }
+    return g_err.size();
+}
+
+int msr_version(void) { return 10000; }
+
+int msr_ctx_create(int device, msr_ctx** out) {
+    return guarded([&] {
+        MSR_NOTNULL(out);
+        int n = 0;
+        MSR_HIP(hipGetDeviceCount(&n));
+        MSR_REQUIRE(device >= 0 && device < n, MSR_ERR_ARGUMENT, "device %d of %d", device, n);
+        auto* c = new msr_ctx;
+        c->device = device;
+        try {
+            ensure_device(c);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int msr_ctx_destroy(msr_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        ensure_device(ctx);
+        delete ctx;
+    });
+}
+
+int msr_traj_create(msr_ctx* ctx, int d, int64_t nsamp, int64_t nt, const float* trj,
+                    const uint8_t* mask, void* stream, msr_traj** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        *out = traj_create(ctx, d, nsamp, nt, trj, mask, (hipStream_t)stream);
+    });
+}
+
+int msr_traj_info(const msr_traj* traj, int64_t* n_masked, int64_t* nt) {
+    return guarded([&] {
+        MSR_NOTNULL(traj);
+        if (n_masked) *n_masked = traj->M;
+        if (nt) *nt = traj->nt;
+    });
+}
+
+int msr_traj_destroy(msr_traj* traj) {
+    return guarded([&] {
+        if (!traj) return;
+        ensure_device(traj->ctx);
+        delete traj;
+    });
+}
+
+int msr_normalop_create_from_traj(msr_ctx* ctx, const msr_traj* traj, int d,
+                                  const int64_t* img_shape, int K, const void* U, int u_real,
+                                  int ncoils, const void* const* cmaps, void* stream,
+                                  msr_normalop** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(traj);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(U);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        MSR_REQUIRE(traj->d == d, MSR_ERR_ARGUMENT,
+                    "trajectory has %d dimensions, img_shape %d", traj->d, d);
+        hipStream_t st = (hipStream_t)stream;
+        std::unique_ptr<msr_normalop> op(op_new(ctx, d, img_shape, K, u_real ? 1 : 0));
+        op_set_cmaps(op.get(), ncoils, cmaps, st);
+        op_build_from_traj(op.get(), traj, U, u_real, st);
+        op_alloc_scratch(op.get());
+        MSR_HIP(hipStreamSynchronize(st));
+        *out = op.release();
+    });
+}
+
+int msr_normalop_create(msr_ctx* ctx, int d, const int64_t* img_shape, int64_t P,
+                        int lambda_real, const void* lambda, const int64_t* kmask, int64_t nkm,
+                        int ncoils, const void* const* cmaps, void* stream, msr_normalop** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(out);
+        MSR_REQUIRE(nkm >= 0, MSR_ERR_ARGUMENT, "nkm = %lld", (long long)nkm);
+        MSR_REQUIRE(nkm == 0 || (lambda && kmask), MSR_ERR_ARGUMENT, "lambda/kmask is NULL");
+        ensure_device(ctx);
+        // Ncoeff = (isqrt(8P+1) - 1) ÷ 2 (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:33)
+        int64_t s = (int64_t)std::llround(std::sqrt((double)(8 * P + 1)));
+        while (s * s > 8 * P + 1) --s;
+        while ((s + 1) * (s + 1) <= 8 * P + 1) ++s;
+        const int K = (int)((s - 1) / 2);
+        MSR_REQUIRE(K >= 1 && (int64_t)K * (K + 1) / 2 == P, MSR_ERR_ARGUMENT,
+                    "packed kernel axis P = %lld is not K(K+1)/2", (long long)P);
+        hipStream_t st = (hipStream_t)stream;
+        std::unique_ptr<msr_normalop> op(op_new(ctx, d, img_shape, K, lambda_real ? 1 : 0));
+        op_set_cmaps(op.get(), ncoils, cmaps, st);
+        op_load_packed(op.get(), lambda, kmask, nkm, st);
+        op_alloc_scratch(op.get());
+        MSR_HIP(hipStreamSynchronize(st));
+        *out = op.release();
+    });
+}
+
+int msr_normalop_info(const msr_normalop* op, int* K, int64_t* P, int64_t* nkm, int* lambda_real) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        if (K) *K = op->K;
+        if (P) *P = op->P;
+        if (nkm) *nkm = op->nkm;
+        if (lambda_real) *lambda_real = op->lam_real;
+    });
+}
+
+int msr_normalop_export_kernel(const msr_normalop* op, void* lambda_out, int64_t* kmask_out,
+                               void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        ensure_device(op->ctx);
+        op_export(op, lambda_out, kmask_out, (hipStream_t)stream);
+    });
+}
+
+int msr_normalop_apply(msr_normalop* op, void* x, const void* b, double alpha_re,
+                       double alpha_im, double beta_re, double beta_im, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        MSR_NOTNULL(x);
+        MSR_NOTNULL(b);
+        MSR_REQUIRE(x != b, MSR_ERR_ARGUMENT, "x and b must not alias");
+        ensure_device(op->ctx);
+        op_apply(op, (float2*)x, (const float2*)b, make_float2((float)alpha_re, (float)alpha_im),
+                 make_float2((float)beta_re, (float)beta_im), beta_re == 0.0 && beta_im == 0.0,
+                 (hipStream_t)stream);
+    });
+}
+
+int msr_normalop_apply_percoil(msr_normalop* op, void* x, const void* b, double alpha_re,
+                               double alpha_im, double beta_re, double beta_im, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        MSR_NOTNULL(x);
+        MSR_NOTNULL(b);
+        MSR_REQUIRE(x != b, MSR_ERR_ARGUMENT, "x and b must not alias");
+        ensure_device(op->ctx);
+        op_apply_percoil(op, (float2*)x, (const float2*)b,
+                         make_float2((float)alpha_re, (float)alpha_im),
+                         make_float2((float)beta_re, (float)beta_im),
+                         beta_re == 0.0 && beta_im == 0.0, (hipStream_t)stream);
+    });
+}
+
+int msr_normalop_destroy(msr_normalop* op) {
+    return guarded([&] {
+        if (!op) return;
+        ensure_device(op->ctx);
+        delete op;
+    });
+}
+
+int msr_backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* img_shape,
+                       const void* data, int ncoils, int K, const void* U, int u_real,
+                       const void* const* cmaps, int dcf_mode, const float* dcf, void* out,
+                       void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(traj);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(data);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        backprojection(ctx, traj, d, img_shape, data, ncoils, K, U, u_real, cmaps, dcf_mode, dcf,
+                       out, (hipStream_t)stream);
+    });
+}
+
+int msr_cg(msr_normalop* op, void* x, const void* b, int maxiter, double reltol, double abstol,
+           int initially_zero, int mode, int* iters_out, double* residual_out, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        MSR_NOTNULL(x);
+        MSR_NOTNULL(b);
+        MSR_REQUIRE(x != b, MSR_ERR_ARGUMENT, "x and b must not alias");
+        MSR_REQUIRE(maxiter >= 0, MSR_ERR_ARGUMENT, "maxiter = %d", maxiter);
+        ensure_device(op->ctx);
+        op_cg(op, (float2*)x, (const float2*)b, maxiter, reltol, abstol, initially_zero != 0, mode,
+              iters_out, residual_out, (hipStream_t)stream);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,6 @@
+// Pass B instantiations for K = 2 (split by K so the build runs in parallel).
+#include "apply2d_b.hpp"
+
+namespace msr {
+MSR_DEFINE_PASSB(2)
+}  // namespace msr

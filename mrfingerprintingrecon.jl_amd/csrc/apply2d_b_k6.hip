@@ -1,0 +1,6 @@
+// Pass B instantiations for K = 6 (split by K so the build runs in parallel).
+#include "apply2d_b.hpp"
+
+namespace msr {
+MSR_DEFINE_PASSB(6)
+}  // namespace msr

@@ -1,0 +1,220 @@
+// LDS-staged mixed-radix Stockham FFT for gfx950.
+//
+// A group of T threads transforms one line of N complex values held in LDS.
+// Each stage of radix R reads R values per butterfly (stride N/R), applies
+// the twiddles W_{NS*R}^{k r} (k = j mod NS, NS = product of earlier radices),
+// runs an R-point DFT in registers and writes the R results to positions
+// (j - k) R + k + r NS, so the output is in natural order without a
+// bit-reversal pass (Stockham autosort).  Stages are separated by workgroup
+// barriers; every thread of the workgroup must call the transform.
+//
+// Sign: DIR = -1 forward (exp(-2 pi i ...)), DIR = +1 inverse, unnormalised.
+// Twiddles come from a table tw[m] = exp(-2 pi i m / N) computed on the host in
+// double precision and staged into LDS by the caller.
+#pragma once
+
+#include "common.hpp"
+
+namespace msr {
+
+// ----------------------------------------------------------- radix plans
+// R[]: radices in stage order; T: threads per line (divides N/R for the
+// stages where that matters; other stages use a guarded loop).
+template <int N> struct FFTPlan;
+#define MSR_PLAN(NN, TT, ...)                                   \
+    template <> struct FFTPlan<NN> {                            \
+        static constexpr int T = TT;                            \
+        static constexpr int R[] = {__VA_ARGS__};               \
+        static constexpr int NST = sizeof(R) / sizeof(int);     \
+    };
+MSR_PLAN(16, 4, 4, 4)
+MSR_PLAN(32, 4, 8, 4)
+MSR_PLAN(64, 8, 8, 8)
+MSR_PLAN(96, 4, 8, 4, 3)
+MSR_PLAN(128, 16, 8, 4, 4)
+MSR_PLAN(160, 8, 8, 4, 5)
+MSR_PLAN(192, 8, 8, 8, 3)
+MSR_PLAN(256, 32, 8, 8, 4)
+MSR_PLAN(320, 16, 8, 8, 5)
+MSR_PLAN(384, 16, 8, 8, 3, 2)
+MSR_PLAN(512, 64, 8, 8, 8)
+MSR_PLAN(640, 16, 8, 8, 5, 2)
+MSR_PLAN(768, 32, 8, 8, 4, 3)
+MSR_PLAN(1024, 64, 8, 8, 4, 4)
+#undef MSR_PLAN
+
+// Invoke F.template operator()<N>() for a runtime length n; returns false if
+// n has no plan.
+template <class F>
+inline bool dispatch_fft_len(int n, F&& f) {
+    switch (n) {
+        case 16: f.template operator()<16>(); return true;
+        case 32: f.template operator()<32>(); return true;
+        case 64: f.template operator()<64>(); return true;
+        case 96: f.template operator()<96>(); return true;
+        case 128: f.template operator()<128>(); return true;
+        case 160: f.template operator()<160>(); return true;
+        case 192: f.template operator()<192>(); return true;
+        case 256: f.template operator()<256>(); return true;
+        case 320: f.template operator()<320>(); return true;
+        case 384: f.template operator()<384>(); return true;
+        case 512: f.template operator()<512>(); return true;
+        case 640: f.template operator()<640>(); return true;
+        case 768: f.template operator()<768>(); return true;
+        case 1024: f.template operator()<1024>(); return true;
+        default: return false;
+    }
+}
+
+inline bool fft_len_supported(int64_t n) {
+    switch (n) {
+        case 16: case 32: case 64: case 96: case 128: case 160: case 192: case 256:
+        case 320: case 384: case 512: case 640: case 768: case 1024: return true;
+        default: return false;
+    }
+}
+
+// ----------------------------------------------------------- butterflies
+// multiply by DIR * i
+template <int DIR>
+__device__ __forceinline__ float2 mul_i(float2 a) {
+    return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <int R, int DIR> struct Butterfly;
+
+template <int DIR> struct Butterfly<2, DIR> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <int DIR> struct Butterfly<4, DIR> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        float2 t2 = cadd(v[1], v[3]), t3 = mul_i<DIR>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+template <int DIR> struct Butterfly<8, DIR> {
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float s = 0.70710678118654752440f;
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        Butterfly<4, DIR>::run(e);
+        Butterfly<4, DIR>::run(o);
+        // w8^q for q = 0..3 with w8 = (1 + DIR i)/sqrt2
+        float2 o1 = cscale(cadd(o[1], mul_i<DIR>(o[1])), s);
+        float2 o2 = mul_i<DIR>(o[2]);
+        float2 o3 = cscale(csub(mul_i<DIR>(o[3]), o[3]), s);
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+template <int DIR> struct Butterfly<3, DIR> {
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float h = 0.86602540378443864676f;  // sin(2 pi / 3)
+        float2 s = cadd(v[1], v[2]);
+        float2 d = mul_i<DIR>(cscale(csub(v[1], v[2]), h));
+        float2 m = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
+        v[0] = cadd(v[0], s);
+        v[1] = cadd(m, d);
+        v[2] = csub(m, d);
+    }
+};
+
+template <int DIR> struct Butterfly<5, DIR> {
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float c1 = 0.30901699437494742410f;   // cos(2 pi/5)
+        constexpr float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
+        constexpr float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
+        constexpr float s2 = 0.58778525229247312917f;   // sin(4 pi/5)
+        float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        float2 p1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+        float2 p2 = make_float2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+        float2 q1 = mul_i<DIR>(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+        float2 q2 = mul_i<DIR>(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+        v[0] = cadd(v[0], cadd(a1, a2));
+        v[1] = cadd(p1, q1);
+        v[4] = csub(p1, q1);
+        v[2] = cadd(p2, q2);
+        v[3] = csub(p2, q2);
+    }
+};
+
+// ----------------------------------------------------------- stages
+template <int N, int R, int NS, int T, int DIR>
+__device__ __forceinline__ void fft_stage(float2* buf, const float2* tw, int t) {
+    constexpr int NB = N / R;
+    constexpr int PER = (NB + T - 1) / T;
+    constexpr bool EXACT = (NB % T) == 0;
+    float2 v[PER][R];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int j = t + p * T;
+        if (EXACT || j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[p][r] = buf[j + r * NB];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int j = t + p * T;
+        if (EXACT || j < NB) {
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[(k * r) * (N / (NS * R))];
+                    if constexpr (DIR > 0) w.y = -w.y;
+                    v[p][r] = cmul(v[p][r], w);
+                }
+            }
+            Butterfly<R, DIR>::run(v[p]);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base + r * NS] = v[p][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int T, int DIR, int S, int NS>
+__device__ __forceinline__ void fft_stages(float2* buf, const float2* tw, int t) {
+    if constexpr (S < FFTPlan<N>::NST) {
+        constexpr int R = FFTPlan<N>::R[S];
+        fft_stage<N, R, NS, T, DIR>(buf, tw, t);
+        fft_stages<N, T, DIR, S + 1, NS * R>(buf, tw, t);
+    }
+}
+
+// In-place N-point FFT of buf[0..N) by the T = FFTPlan<N>::T threads with
+// local index t.  Requires a barrier before the call (data in LDS); ends with
+// a barrier.  All threads of the workgroup must call it.
+template <int N, int DIR>
+__device__ __forceinline__ void fft_lds(float2* buf, const float2* tw, int t) {
+    fft_stages<N, FFTPlan<N>::T, DIR, 0, 1>(buf, tw, t);
+}
+
+// Copy an N-entry twiddle table from global memory into LDS (no barrier).
+template <int N>
+__device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* g_tw) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = g_tw[i];
+}
+
+}  // namespace msr

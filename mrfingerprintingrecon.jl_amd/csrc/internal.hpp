@@ -1,0 +1,89 @@
+// Internal object definitions behind the opaque C handles.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+
+struct msr_ctx {
+    int device = 0;
+    std::map<int, float2*> twiddles;  // N -> device table exp(-2 pi i m/N)
+    ~msr_ctx();
+};
+
+namespace msr {
+
+// Device buffer with RAII free.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        if (n == bytes && p) return;
+        release();
+        if (n == 0) return;
+        MSR_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+const float2* twiddle_table(msr_ctx* ctx, int n);
+void ensure_device(const msr_ctx* ctx);
+
+// CG scalar state kept on the device (FP64).
+struct CgScalars {
+    double res, prev, tol;
+    double alpha_re, alpha_im;
+    double dot_re, dot_im;
+    int iter, done, maxiter, pad;
+};
+
+}  // namespace msr
+
+struct msr_traj {
+    msr_ctx* ctx = nullptr;
+    int d = 0;
+    int64_t nsamp = 0, nt = 0;
+    int64_t M = 0;                    // masked samples
+    msr::DevBuf kc;                   // (M, d) float32, masked, frame-major
+    msr::DevBuf frame;                // (M) int32 frame index
+    msr::DevBuf sel;                  // (M) int64 index of the masked sample in nsamp*nt, or empty if unmasked
+    std::vector<int64_t> nsamp_t;     // host copy of the per-frame counts
+};
+
+struct msr_normalop {
+    msr_ctx* ctx = nullptr;
+    int d = 2;
+    int64_t N[3] = {1, 1, 1};         // image size (x first)
+    int64_t V = 1;                    // voxels
+    int K = 1, P = 1, C = 1;
+    int lam_real = 0;
+    bool identity_cmaps = true;
+    int64_t nkm = 0;
+    msr::DevBuf cmaps;                // (C, V) complex
+    msr::DevBuf lam;                  // dense kernel, layout per dimension (see normalop.hip)
+    msr::DevBuf kmask;                // (nkm) int64, 1-based
+    msr::DevBuf T;                    // FFT intermediate
+    msr::DevBuf T2;                   // second intermediate (3D)
+    msr::DevBuf partial;              // (G, K, V)
+    int G = 1, Gb = 1;
+    // CG state (allocated on first use)
+    msr::DevBuf cg_r, cg_u, cg_c, cg_scal, cg_part;
+    hipGraphExec_t graph = nullptr;
+    const void* graph_key[3] = {nullptr, nullptr, nullptr};
+    int graph_maxiter = -1;
+    double graph_tol[2] = {0.0, 0.0};
+    hipStream_t graph_stream = nullptr;
+    ~msr_normalop();
+};

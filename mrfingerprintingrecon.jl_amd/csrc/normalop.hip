@@ -1,0 +1,388 @@
+// Normal-operator object: construction from a packed kernel (the reference's
+// NFFTNormalOp(img_shape, Λ, kmask_indcs; cmaps),
+// ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:22-70), dense device layout of
+// the kernel, fused and per-coil applies, kernel export, device CG.
+//
+// Dense kernel layout (2D): lam[(qx * P + p) * 2Ny + qy] for every cell of the
+// 2Nx x 2Ny grid, zero off the kmask, so that pass B reads one contiguous
+// (P, 2Ny) slab per grid column.  For 3D see apply3d.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "apply2d.hpp"
+#include "apply3d.hpp"
+#include "fft.hpp"
+#include "vecops.hpp"
+
+namespace msr {
+
+// ------------------------------------------------------------------ kernels
+__global__ void fill_kernel(float* __restrict__ p, int64_t n, float v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void fill_floats(void* p, int64_t n, float v, hipStream_t st) {
+    if (n <= 0) return;
+    int grid = (int)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(fill_kernel, dim3(grid), dim3(256), 0, st, (float*)p, n, v);
+    MSR_CHECK_LAUNCH();
+}
+
+// linear 0-based grid index q (x fastest) -> dense kernel offset of entry p
+__device__ __forceinline__ int64_t dense_offset(int d, const int64_t* n2, int P, int64_t q, int p) {
+    if (d == 2) {
+        const int64_t qx = q % n2[0], qy = q / n2[0];
+        return ((qx * P) + p) * n2[1] + qy;
+    }
+    // 3D: lam[((qy * n2x + qx) * P + p) * n2z + qz]
+    const int64_t qx = q % n2[0];
+    const int64_t r = q / n2[0];
+    const int64_t qy = r % n2[1], qz = r / n2[1];
+    return (((qy * n2[0] + qx) * P) + p) * n2[2] + qz;
+}
+
+struct GridDims {
+    int64_t n[3];
+};
+
+template <class LT>
+__global__ void scatter_kernel(const LT* __restrict__ packed, const int64_t* __restrict__ kmask,
+                               int64_t nkm, int P, int d, GridDims g, LT* __restrict__ dense,
+                               int* __restrict__ bad) {
+    const int64_t G = g.n[0] * g.n[1] * (d == 3 ? g.n[2] : 1);
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nkm * P;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / P;
+        const int p = (int)(e - i * P);
+        const int64_t q1 = kmask[i];
+        if (q1 < 1 || q1 > G) {
+            *bad = 1;
+            continue;
+        }
+        dense[dense_offset(d, g.n, P, q1 - 1, p)] = packed[e];
+    }
+}
+
+template <class LT>
+__global__ void gather_kernel(const LT* __restrict__ dense, const int64_t* __restrict__ kmask,
+                              int64_t nkm, int P, int d, GridDims g, LT* __restrict__ packed) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nkm * P;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / P;
+        const int p = (int)(e - i * P);
+        packed[e] = dense[dense_offset(d, g.n, P, kmask[i] - 1, p)];
+    }
+}
+
+// ------------------------------------------------------------------ helpers
+static int env_int(const char* name, int dflt) {
+    const char* s = getenv(name);
+    return s ? atoi(s) : dflt;
+}
+
+void op_alloc_scratch(msr_normalop* op) {
+    const int64_t nx = op->N[0], ny = op->N[1];
+    if (op->d == 2) {
+        const int tiles = (int)((ny + 15) / 16);
+        int G = (int)ceil_div(512, (int64_t)tiles * op->K);
+        G = std::max(1, std::min(G, op->C));
+        op->G = env_int("MSR_G", G);
+        op->G = std::max(1, std::min(op->G, op->C));
+        const int64_t colblocks = 2 * nx;
+        int Gb = colblocks >= 1024 ? 1 : 2;
+        op->Gb = std::max(1, std::min(env_int("MSR_GB", Gb), op->C));
+        op->T.alloc(sizeof(float2) * (size_t)op->C * op->K * (2 * nx) * ny);
+        op->partial.alloc(sizeof(float2) * (size_t)op->G * op->K * op->V);
+    } else {
+        alloc_scratch_3d(op);
+    }
+}
+
+static void check_shape(int d, const int64_t* N) {
+    MSR_REQUIRE(d == 2 || d == 3, MSR_ERR_ARGUMENT, "image dimension d = %d, expected 2 or 3", d);
+    for (int i = 0; i < d; ++i) {
+        MSR_REQUIRE(N[i] >= 1, MSR_ERR_ARGUMENT, "img_shape[%d] = %lld", i, (long long)N[i]);
+        int64_t m = 2 * N[i];
+        for (int p : {2, 3, 5})
+            while (m % p == 0) m /= p;
+        MSR_REQUIRE(m == 1, MSR_ERR_ASSERTION,
+                    "img_shape_os has to be composed of the prime factors 2, 3, and 5 "
+                    "(2N[%d] = %lld)", i, (long long)(2 * N[i]));
+        MSR_REQUIRE(fft_len_supported(2 * N[i]) && 2 * N[i] <= 512, MSR_ERR_UNSUPPORTED,
+                    "Toeplitz grid size 2N[%d] = %lld has no FFT kernel in this build",
+                    i, (long long)(2 * N[i]));
+    }
+}
+
+void op_set_cmaps(msr_normalop* op, int ncoils, const void* const* cmaps, hipStream_t st) {
+    MSR_REQUIRE(ncoils >= 0, MSR_ERR_ARGUMENT, "ncoils = %d", ncoils);
+    if (ncoils == 0 || cmaps == nullptr) {
+        op->C = 1;
+        op->identity_cmaps = true;
+        return;
+    }
+    op->C = ncoils;
+    op->identity_cmaps = false;
+    op->cmaps.alloc(sizeof(float2) * (size_t)ncoils * op->V);
+    for (int c = 0; c < ncoils; ++c) {
+        MSR_REQUIRE(cmaps[c] != nullptr, MSR_ERR_ARGUMENT, "cmaps[%d] is NULL", c);
+        MSR_HIP(hipMemcpyAsync(op->cmaps.as<float2>() + (size_t)c * op->V, cmaps[c],
+                               sizeof(float2) * op->V, hipMemcpyDeviceToDevice, st));
+    }
+}
+
+GridDims op_grid(const msr_normalop* op) {
+    GridDims g{{2 * op->N[0], 2 * op->N[1], op->d == 3 ? 2 * op->N[2] : 1}};
+    return g;
+}
+
+int64_t op_grid_cells(const msr_normalop* op) {
+    GridDims g = op_grid(op);
+    return g.n[0] * g.n[1] * g.n[2];
+}
+
+// Fill the dense kernel from a packed (P, Nkm) kernel and 1-based kmask.
+void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, int64_t nkm,
+                    hipStream_t st) {
+    const int64_t G = op_grid_cells(op);
+    const size_t es = op->lam_real ? sizeof(float) : sizeof(float2);
+    op->lam.alloc(es * (size_t)op->P * G);
+    fill_floats(op->lam.p, (int64_t)(op->lam.bytes / sizeof(float)), 0.f, st);
+    op->nkm = nkm;
+    op->kmask.alloc(sizeof(int64_t) * std::max<int64_t>(nkm, 1));
+    if (nkm > 0)
+        MSR_HIP(hipMemcpyAsync(op->kmask.p, kmask, sizeof(int64_t) * nkm, hipMemcpyDeviceToDevice, st));
+    DevBuf bad;
+    bad.alloc(sizeof(int));
+    fill_floats(bad.p, 1, 0.f, st);
+    if (nkm > 0) {
+        int grid = (int)std::min<int64_t>((nkm * op->P + 255) / 256, 16384);
+        GridDims g = op_grid(op);
+        if (op->lam_real)
+            hipLaunchKernelGGL(scatter_kernel<float>, dim3(grid), dim3(256), 0, st,
+                               (const float*)lambda, op->kmask.as<int64_t>(), nkm, op->P, op->d, g,
+                               op->lam.as<float>(), bad.as<int>());
+        else
+            hipLaunchKernelGGL(scatter_kernel<float2>, dim3(grid), dim3(256), 0, st,
+                               (const float2*)lambda, op->kmask.as<int64_t>(), nkm, op->P, op->d, g,
+                               op->lam.as<float2>(), bad.as<int>());
+        MSR_CHECK_LAUNCH();
+    }
+    int hbad = 0;
+    MSR_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    MSR_HIP(hipStreamSynchronize(st));
+    MSR_REQUIRE(hbad == 0, MSR_ERR_ASSERTION,
+                "kmask_indcs out of bounds: entries must lie in 1..prod(2 .* img_shape) = %lld",
+                (long long)G);
+}
+
+msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real) {
+    check_shape(d, N);
+    MSR_REQUIRE(K >= 1 && K <= 8, MSR_ERR_UNSUPPORTED, "K = %d coefficients, supported 1..8", K);
+    auto op = std::make_unique<msr_normalop>();
+    op->ctx = ctx;
+    op->d = d;
+    op->V = 1;
+    for (int i = 0; i < 3; ++i) op->N[i] = i < d ? N[i] : 1;
+    for (int i = 0; i < d; ++i) op->V *= N[i];
+    op->K = K;
+    op->P = K * (K + 1) / 2;
+    op->lam_real = lam_real;
+    return op.release();
+}
+
+// ------------------------------------------------------------------ applies
+static Apply2DArgs args2d(msr_normalop* op) {
+    Apply2DArgs a;
+    a.nx = (int)op->N[0];
+    a.ny = (int)op->N[1];
+    a.K = op->K;
+    a.C = op->C;
+    a.cmaps = op->identity_cmaps ? nullptr : op->cmaps.as<float2>();
+    a.lam = op->lam.p;
+    a.lam_real = op->lam_real;
+    a.T = op->T.as<float2>();
+    a.partial = op->partial.as<float2>();
+    a.G = op->G;
+    a.Gb = op->Gb;
+    a.tw_x = twiddle_table(op->ctx, 2 * a.nx);
+    a.tw_y = twiddle_table(op->ctx, 2 * a.ny);
+    return a;
+}
+
+float op_scale(const msr_normalop* op) { return (float)(1.0 / (double)op_grid_cells(op)); }
+
+// partial[g] <- unscaled group sums of A^H A b
+void op_passes(msr_normalop* op, const float2* b, hipStream_t st, const int* skip) {
+    if (op->d == 2) launch_apply2d_passes(args2d(op), b, st, skip);
+    else launch_apply3d_passes(op, b, st, skip);
+}
+
+void op_apply(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
+              bool beta_zero, hipStream_t st) {
+    op_passes(op, b, st, nullptr);
+    launch_epilogue(op->partial.as<float2>(), op->G, op->V * op->K, x, alpha, beta, beta_zero,
+                    op_scale(op), st, nullptr);
+}
+
+// The reference GPU order: one coil at a time, kernel re-read per coil.
+void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
+                      bool beta_zero, hipStream_t st) {
+    launch_scale(x, op->V * op->K, beta, beta_zero, st);
+    for (int c = 0; c < op->C; ++c) {
+        if (op->d == 2) {
+            Apply2DArgs a = args2d(op);
+            a.C = 1;
+            a.G = 1;
+            a.Gb = 1;
+            if (a.cmaps) a.cmaps += (size_t)c * op->V;
+            launch_apply2d_passes(a, b, st, nullptr);
+        } else {
+            launch_apply3d_passes_coil(op, b, c, st);
+        }
+        launch_epilogue(op->partial.as<float2>(), 1, op->V * op->K, x, alpha,
+                        make_float2(1.f, 0.f), false, op_scale(op), st, nullptr);
+    }
+}
+
+void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hipStream_t st) {
+    if (op->nkm == 0) return;
+    if (kmask_out)
+        MSR_HIP(hipMemcpyAsync(kmask_out, op->kmask.p, sizeof(int64_t) * op->nkm,
+                               hipMemcpyDeviceToDevice, st));
+    if (lambda_out) {
+        int grid = (int)std::min<int64_t>((op->nkm * op->P + 255) / 256, 16384);
+        GridDims g = op_grid(op);
+        if (op->lam_real)
+            hipLaunchKernelGGL(gather_kernel<float>, dim3(grid), dim3(256), 0, st,
+                               op->lam.as<float>(), op->kmask.as<int64_t>(), op->nkm, op->P, op->d,
+                               g, (float*)lambda_out);
+        else
+            hipLaunchKernelGGL(gather_kernel<float2>, dim3(grid), dim3(256), 0, st,
+                               op->lam.as<float2>(), op->kmask.as<int64_t>(), op->nkm, op->P,
+                               op->d, g, (float2*)lambda_out);
+        MSR_CHECK_LAUNCH();
+    }
+}
+
+// ------------------------------------------------------------------ CG
+static void cg_buffers(msr_normalop* op) {
+    const size_t n = (size_t)op->V * op->K;
+    op->cg_r.alloc(sizeof(float2) * n);
+    op->cg_u.alloc(sizeof(float2) * n);
+    op->cg_c.alloc(sizeof(float2) * n);
+    op->cg_scal.alloc(sizeof(CgScalars));
+    op->cg_part.alloc(sizeof(double) * 2 * RED_BLOCKS);
+}
+
+// one CG iteration with device scalars (all kernels skip once done is set)
+static void cg_iteration_device(msr_normalop* op, float2* x, hipStream_t st) {
+    const int64_t n = op->V * op->K;
+    auto* S = op->cg_scal.as<CgScalars>();
+    float2 *r = op->cg_r.as<float2>(), *u = op->cg_u.as<float2>(), *c = op->cg_c.as<float2>();
+    double* part = op->cg_part.as<double>();
+    launch_cg_u(r, u, n, S, st);
+    op_passes(op, u, st, &S->done);
+    launch_cg_reduce_dot(op->partial.as<float2>(), op->G, n, c, u, op_scale(op), part, S, st);
+    launch_cg_xr(x, r, u, c, n, part, S, st);
+}
+
+static void cg_prologue(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
+                        double abstol, bool initially_zero, hipStream_t st) {
+    const int64_t n = op->V * op->K;
+    auto* S = op->cg_scal.as<CgScalars>();
+    float2 *r = op->cg_r.as<float2>(), *u = op->cg_u.as<float2>(), *c = op->cg_c.as<float2>();
+    if (initially_zero) {
+        launch_cg_start(b, x, r, u, n, true, op->cg_part.as<double>(), S, reltol, abstol, maxiter,
+                        false, nullptr, st);
+    } else {
+        op_apply(op, c, x, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true, st);
+        launch_cg_start(b, x, r, u, n, false, op->cg_part.as<double>(), S, reltol, abstol,
+                        maxiter, true, c, st);
+    }
+}
+
+void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
+           double abstol, bool initially_zero, int mode, int* iters_out, double* res_out,
+           hipStream_t st) {
+    cg_buffers(op);
+    const int64_t n = op->V * op->K;
+    auto* S = op->cg_scal.as<CgScalars>();
+    if (mode == MSR_CG_DEVICE) {
+        cg_prologue(op, x, b, maxiter, reltol, abstol, initially_zero, st);
+        for (int it = 0; it < maxiter; ++it) cg_iteration_device(op, x, st);
+    } else if (mode == MSR_CG_GRAPH) {
+        const bool hit = op->graph && op->graph_key[0] == x && op->graph_key[1] == b &&
+                         op->graph_key[2] == (const void*)(intptr_t)(initially_zero ? 1 : 2) &&
+                         op->graph_maxiter == maxiter && op->graph_stream == st &&
+                         op->graph_tol[0] == reltol && op->graph_tol[1] == abstol;
+        if (!hit) {
+            if (op->graph) {
+                (void)hipGraphExecDestroy(op->graph);
+                op->graph = nullptr;
+            }
+            MSR_REQUIRE(st != nullptr, MSR_ERR_ARGUMENT,
+                        "MSR_CG_GRAPH needs an explicit (non-default) stream");
+            hipGraph_t g = nullptr;
+            MSR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                cg_prologue(op, x, b, maxiter, reltol, abstol, initially_zero, st);
+                for (int it = 0; it < maxiter; ++it) cg_iteration_device(op, x, st);
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            MSR_HIP(hipStreamEndCapture(st, &g));
+            hipGraphExec_t ge = nullptr;
+            hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            MSR_HIP(e);
+            op->graph = ge;
+            op->graph_key[0] = x;
+            op->graph_key[1] = b;
+            op->graph_key[2] = (const void*)(intptr_t)(initially_zero ? 1 : 2);
+            op->graph_maxiter = maxiter;
+            op->graph_stream = st;
+            op->graph_tol[0] = reltol;
+            op->graph_tol[1] = abstol;
+        }
+        MSR_HIP(hipGraphLaunch(op->graph, st));
+    } else if (mode == MSR_CG_HOST) {
+        // IterativeSolvers-style: scalars are read back to the host every
+        // iteration; the apply is the per-coil reference order.
+        float2 *r = op->cg_r.as<float2>(), *u = op->cg_u.as<float2>(), *c = op->cg_c.as<float2>();
+        double* part = op->cg_part.as<double>();
+        cg_prologue(op, x, b, maxiter, reltol, abstol, initially_zero, st);
+        CgScalars hs;
+        for (int it = 0; it < maxiter; ++it) {
+            MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+            MSR_HIP(hipStreamSynchronize(st));
+            if (hs.done) break;
+            launch_cg_u(r, u, n, S, st);
+            op_apply_percoil(op, c, u, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true, st);
+            launch_cg_reduce_dot(c, 1, n, c, u, 1.0f, part, S, st);
+            MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+            MSR_HIP(hipStreamSynchronize(st));
+            launch_cg_xr(x, r, u, c, n, part, S, st);
+        }
+    } else {
+        fail(MSR_ERR_ARGUMENT, "unknown CG mode %d", mode);
+    }
+    if (iters_out || res_out) {
+        CgScalars hs;
+        MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+        MSR_HIP(hipStreamSynchronize(st));
+        if (iters_out) *iters_out = hs.iter;
+        if (res_out) *res_out = hs.res;
+    }
+}
+
+}  // namespace msr
+
+msr_normalop::~msr_normalop() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+}
