@@ -152,10 +152,14 @@ int msr_backprojection(msr_ctx *ctx, const msr_traj *traj, int d, const int64_t 
  * the apply: test/reconstruct_radial_gpu.jl:99, src/CoilMaps.jl:136) run on
  * the device with FP64 inner products, norms and step lengths.  initially_zero
  * = 1 reproduces cg(A, b) (x starts at 0 and is not read).  mode: MSR_CG_*.
+ * vec_fp64 = 1 keeps the solution, residual and search vectors in FP64 (the
+ * applies still run in FP32 on an FP32 copy of the search vector); 0 keeps
+ * them ComplexF32 like the reference.  x and b are ComplexF32 either way.
  * iters_out / residual_out (host, may be NULL) receive the iteration count and
  * final residual norm; reading them synchronises the stream. */
 int msr_cg(msr_normalop *op, void *x, const void *b, int maxiter, double reltol, double abstol,
-           int initially_zero, int mode, int *iters_out, double *residual_out, void *stream);
+           int initially_zero, int mode, int vec_fp64, int *iters_out, double *residual_out,
+           void *stream);
 
 #ifdef __cplusplus
 }

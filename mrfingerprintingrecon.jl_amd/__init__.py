@@ -360,12 +360,13 @@ def _side_stream(device):
 
 
 def cg_(x, A, b, maxiter=None, reltol=REL_TOL_F32, abstol=0.0, initially_zero=False,
-        mode="device", log=False):
+        mode="device", vec_fp64=True, log=False):
     """cg!(x, A, b; maxiter, reltol, abstol) of IterativeSolvers.jl 0.9 on the
     device (FP64 dot products, norms and step lengths).  mode: "device"
     (device-resident scalars), "graph" (hipGraph-captured iterations) or
     "host" (scalars read back every iteration, per-coil applies: the plain
-    reference-order GPU path)."""
+    reference-order GPU path).  vec_fp64 keeps x, r and u in FP64 on the device
+    (the applies stay FP32); False keeps them ComplexF32 like the reference."""
     op = A.A if isinstance(A, LinearOperator) else A
     if maxiter is None:
         maxiter = A.ncol if isinstance(A, LinearOperator) else math.prod(op.shape) * op.Ncoeff
@@ -381,8 +382,8 @@ def cg_(x, A, b, maxiter=None, reltol=REL_TOL_F32, abstol=0.0, initially_zero=Fa
         side.wait_stream(cur)
     st = ctypes.c_void_p((side or cur).cuda_stream)
     _lib.call("msr_cg", op.h, _ptr(x), _ptr(b), int(maxiter), float(reltol), float(abstol),
-              int(bool(initially_zero)), _lib.CG_MODES[mode], ctypes.byref(it),
-              ctypes.byref(res), st)
+              int(bool(initially_zero)), _lib.CG_MODES[mode], int(bool(vec_fp64)),
+              ctypes.byref(it), ctypes.byref(res), st)
     if side is not None:
         cur.wait_stream(side)
     if log:

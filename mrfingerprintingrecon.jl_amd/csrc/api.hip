@@ -18,8 +18,8 @@ void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha
                       bool beta_zero, hipStream_t st);
 void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hipStream_t st);
 void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
-           double abstol, bool initially_zero, int mode, int* iters_out, double* res_out,
-           hipStream_t st);
+           double abstol, bool initially_zero, int mode, bool fp64, int* iters_out,
+           double* res_out, hipStream_t st);
 // traj.hip
 msr_traj* traj_create(msr_ctx* ctx, int d, int64_t nsamp, int64_t nt, const float* trj,
                       const uint8_t* mask, hipStream_t st);
@@ -247,7 +247,8 @@ int msr_backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t*
 }
 
 int msr_cg(msr_normalop* op, void* x, const void* b, int maxiter, double reltol, double abstol,
-           int initially_zero, int mode, int* iters_out, double* residual_out, void* stream) {
+           int initially_zero, int mode, int vec_fp64, int* iters_out, double* residual_out,
+           void* stream) {
     return guarded([&] {
         MSR_NOTNULL(op);
         MSR_NOTNULL(x);
@@ -256,7 +257,7 @@ int msr_cg(msr_normalop* op, void* x, const void* b, int maxiter, double reltol,
         MSR_REQUIRE(maxiter >= 0, MSR_ERR_ARGUMENT, "maxiter = %d", maxiter);
         ensure_device(op->ctx);
         op_cg(op, (float2*)x, (const float2*)b, maxiter, reltol, abstol, initially_zero != 0, mode,
-              iters_out, residual_out, (hipStream_t)stream);
+              vec_fp64 != 0, iters_out, residual_out, (hipStream_t)stream);
     });
 }
 

@@ -15,7 +15,8 @@ __host__ __device__ constexpr int packed_index(int j, int k) { return j + k * (k
 template <int NY2, int K>
 struct PassBShape {
     static constexpr int TP = FFTPlan<NY2>::T;
-    static constexpr int WC = (256 / (K * TP)) > 1 ? 256 / (K * TP) : 1;  // columns per WG
+    static constexpr int WC0 = (256 / (K * TP)) > 1 ? 256 / (K * TP) : 1;
+    static constexpr int WC = WC0 < 16 ? WC0 : 16;  // columns per WG
     static constexpr int BLK = WC * K * TP;
     static constexpr int NV = WC * NY2;                // voxels per WG
     static constexpr int EV = (NV + BLK - 1) / BLK;    // voxels per thread

@@ -28,8 +28,12 @@ template <int N> struct FFTPlan;
         static constexpr int NST = sizeof(R) / sizeof(int);     \
     };
 MSR_PLAN(16, 4, 4, 4)
+MSR_PLAN(24, 1, 8, 3)
 MSR_PLAN(32, 4, 8, 4)
+MSR_PLAN(40, 1, 8, 5)
+MSR_PLAN(48, 2, 8, 3, 2)
 MSR_PLAN(64, 8, 8, 8)
+MSR_PLAN(80, 2, 8, 5, 2)
 MSR_PLAN(96, 4, 8, 4, 3)
 MSR_PLAN(128, 16, 8, 4, 4)
 MSR_PLAN(160, 8, 8, 4, 5)
@@ -43,35 +47,34 @@ MSR_PLAN(768, 32, 8, 8, 4, 3)
 MSR_PLAN(1024, 64, 8, 8, 4, 4)
 #undef MSR_PLAN
 
+// Supported line lengths (each has a plan above).
+#define MSR_FFT_LENGTHS(X) \
+    X(16) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) \
+    X(512) X(640) X(768) X(1024)
+
 // Invoke F.template operator()<N>() for a runtime length n; returns false if
 // n has no plan.
 template <class F>
 inline bool dispatch_fft_len(int n, F&& f) {
+#define MSR_CASE(NN)                     \
+    case NN:                             \
+        f.template operator()<NN>();     \
+        return true;
     switch (n) {
-        case 16: f.template operator()<16>(); return true;
-        case 32: f.template operator()<32>(); return true;
-        case 64: f.template operator()<64>(); return true;
-        case 96: f.template operator()<96>(); return true;
-        case 128: f.template operator()<128>(); return true;
-        case 160: f.template operator()<160>(); return true;
-        case 192: f.template operator()<192>(); return true;
-        case 256: f.template operator()<256>(); return true;
-        case 320: f.template operator()<320>(); return true;
-        case 384: f.template operator()<384>(); return true;
-        case 512: f.template operator()<512>(); return true;
-        case 640: f.template operator()<640>(); return true;
-        case 768: f.template operator()<768>(); return true;
-        case 1024: f.template operator()<1024>(); return true;
+        MSR_FFT_LENGTHS(MSR_CASE)
         default: return false;
     }
+#undef MSR_CASE
 }
 
 inline bool fft_len_supported(int64_t n) {
+#define MSR_CASE(NN) \
+    case NN: return true;
     switch (n) {
-        case 16: case 32: case 64: case 96: case 128: case 160: case 192: case 256:
-        case 320: case 384: case 512: case 640: case 768: case 1024: return true;
+        MSR_FFT_LENGTHS(MSR_CASE)
         default: return false;
     }
+#undef MSR_CASE
 }
 
 // ----------------------------------------------------------- butterflies

@@ -1,0 +1,45 @@
+// Generic batched 1D FFT passes over lines of a grid, used by the
+// backprojection and the Toeplitz-kernel construction:
+//  - rows: lines contiguous in memory (x axis)
+//  - cols: lines strided, adjacent lines adjacent in memory (y or z axis),
+//          processed 8-16 lines per workgroup so loads and stores coalesce
+// Each line has Lin valid inputs (zero-padded to the FFT length NF) and Lout
+// selected outputs: SEL_FIRST bins 0..Lout-1, SEL_CENTER frequency
+// r = j - Lout/2 (fftshift order), SEL_FFTORDER r = j or j - Lout (FFT order),
+// each scaled by an optional real weight w[j] (deapodisation).
+#pragma once
+
+#include "internal.hpp"
+
+namespace msr {
+
+enum { SEL_FIRST = 0, SEL_CENTER = 1, SEL_FFTORDER = 2 };
+
+struct LinePass {
+    const float2* in = nullptr;
+    float2* out = nullptr;
+    int nf = 0;          // FFT length
+    int dir = -1;        // -1 forward, +1 inverse
+    int lin = 0, lout = 0, sel = SEL_FIRST;
+    const float* w = nullptr;
+    // rows: line l at in + l*in_pitch (contiguous), out + l*out_pitch
+    int64_t nlines = 0, in_pitch = 0, out_pitch = 0;
+    // cols: lines (b, col) with col < ncols adjacent in memory
+    int64_t nb = 0, ncols = 0;
+    int64_t in_bstride = 0, in_stride = 0, out_bstride = 0, out_stride = 0;
+    // cols epilogue writing the dense Toeplitz kernel: b = packed pair p,
+    // j = qy, col = qx:  lam[(qx*P + p)*ny2 + qy] = mask[qy*nx2 + qx] ? v : 0
+    void* lam = nullptr;
+    const uint8_t* mask = nullptr;
+    int P = 0, lam_real = 0;
+    int64_t nx2 = 0;
+    // 3D variant of the dense store: col = qy*nx2 + qx, j = qz,
+    // lam[((qy*nx2 + qx)*P + p)*nz2 + qz], mask[(qz*ny2 + qy)*nx2 + qx]
+    int lam3d = 0;
+    int64_t ny2 = 0;
+};
+
+void run_rows(const LinePass& p, msr_ctx* ctx, hipStream_t st);
+void run_cols(const LinePass& p, msr_ctx* ctx, hipStream_t st);
+
+}  // namespace msr

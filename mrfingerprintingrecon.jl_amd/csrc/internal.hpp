@@ -7,9 +7,13 @@
 
 #include "common.hpp"
 
+#include <array>
+#include <tuple>
+
 struct msr_ctx {
     int device = 0;
     std::map<int, float2*> twiddles;  // N -> device table exp(-2 pi i m/N)
+    std::map<std::tuple<int, int, int>, float*> deapod;  // (n_os, n_out, sel) -> 1/phihat
     ~msr_ctx();
 };
 
@@ -41,6 +45,17 @@ struct DevBuf {
 const float2* twiddle_table(msr_ctx* ctx, int n);
 void ensure_device(const msr_ctx* ctx);
 
+// Tile sort of a trajectory's points for one oversampled grid (spread.hip).
+struct SpreadPlan {
+    int d = 2;
+    int64_t n[3] = {1, 1, 1};   // grid size
+    int tb[3] = {1, 1, 1};      // tile size
+    int nb[3] = {1, 1, 1};      // tiles per dim
+    int64_t nbins = 0;
+    DevBuf perm;                // (M) int32: points sorted by tile
+    DevBuf offs;                // (nbins + 1) int32
+};
+
 // CG scalar state kept on the device (FP64).
 struct CgScalars {
     double res, prev, tol;
@@ -60,6 +75,7 @@ struct msr_traj {
     msr::DevBuf frame;                // (M) int32 frame index
     msr::DevBuf sel;                  // (M) int64 index of the masked sample in nsamp*nt, or empty if unmasked
     std::vector<int64_t> nsamp_t;     // host copy of the per-frame counts
+    mutable std::map<std::array<int64_t, 3>, std::unique_ptr<msr::SpreadPlan>> plans;
 };
 
 struct msr_normalop {
@@ -79,11 +95,12 @@ struct msr_normalop {
     msr::DevBuf partial;              // (G, K, V)
     int G = 1, Gb = 1;
     // CG state (allocated on first use)
-    msr::DevBuf cg_r, cg_u, cg_c, cg_scal, cg_part;
+    msr::DevBuf cg_r, cg_u, cg_c, cg_scal, cg_part, cg_x64, cg_u32;
     hipGraphExec_t graph = nullptr;
     const void* graph_key[3] = {nullptr, nullptr, nullptr};
     int graph_maxiter = -1;
     double graph_tol[2] = {0.0, 0.0};
+    int graph_fp64 = -1;
     hipStream_t graph_stream = nullptr;
     ~msr_normalop();
 };

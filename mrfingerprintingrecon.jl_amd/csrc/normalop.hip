@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "apply2d.hpp"
 #include "apply3d.hpp"
@@ -269,56 +270,91 @@ void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hip
 }
 
 // ------------------------------------------------------------------ CG
-static void cg_buffers(msr_normalop* op) {
+static void cg_buffers(msr_normalop* op, bool fp64) {
     const size_t n = (size_t)op->V * op->K;
-    op->cg_r.alloc(sizeof(float2) * n);
-    op->cg_u.alloc(sizeof(float2) * n);
+    const size_t es = fp64 ? sizeof(double2) : sizeof(float2);
+    op->cg_r.alloc(es * n);
+    op->cg_u.alloc(es * n);
     op->cg_c.alloc(sizeof(float2) * n);
+    if (fp64) {
+        op->cg_x64.alloc(sizeof(double2) * n);
+        op->cg_u32.alloc(sizeof(float2) * n);
+    }
     op->cg_scal.alloc(sizeof(CgScalars));
     op->cg_part.alloc(sizeof(double) * 2 * RED_BLOCKS);
 }
 
-// one CG iteration with device scalars (all kernels skip once done is set)
-static void cg_iteration_device(msr_normalop* op, float2* x, hipStream_t st) {
-    const int64_t n = op->V * op->K;
-    auto* S = op->cg_scal.as<CgScalars>();
-    float2 *r = op->cg_r.as<float2>(), *u = op->cg_u.as<float2>(), *c = op->cg_c.as<float2>();
-    double* part = op->cg_part.as<double>();
-    launch_cg_u(r, u, n, S, st);
-    op_passes(op, u, st, &S->done);
-    launch_cg_reduce_dot(op->partial.as<float2>(), op->G, n, c, u, op_scale(op), part, S, st);
-    launch_cg_xr(x, r, u, c, n, part, S, st);
+template <class VT>
+static CgVecs<VT> cg_vecs(msr_normalop* op, float2* x) {
+    CgVecs<VT> v;
+    v.r = op->cg_r.as<VT>();
+    v.u = op->cg_u.as<VT>();
+    v.c = op->cg_c.as<float2>();
+    if constexpr (std::is_same<VT, double2>::value) {
+        v.x = op->cg_x64.as<double2>();
+        v.u32 = op->cg_u32.as<float2>();
+    } else {
+        v.x = x;
+        v.u32 = op->cg_u.as<float2>();
+    }
+    return v;
 }
 
-static void cg_prologue(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
-                        double abstol, bool initially_zero, hipStream_t st) {
+// one CG iteration with device scalars (all kernels skip once done is set)
+template <class VT>
+static void cg_iteration(msr_normalop* op, const CgVecs<VT>& v, bool percoil, hipStream_t st) {
     const int64_t n = op->V * op->K;
     auto* S = op->cg_scal.as<CgScalars>();
-    float2 *r = op->cg_r.as<float2>(), *u = op->cg_u.as<float2>(), *c = op->cg_c.as<float2>();
-    if (initially_zero) {
-        launch_cg_start(b, x, r, u, n, true, op->cg_part.as<double>(), S, reltol, abstol, maxiter,
-                        false, nullptr, st);
+    double* part = op->cg_part.as<double>();
+    launch_cg_u(v, n, S, st);
+    if (percoil) {
+        op_apply_percoil(op, v.c, v.u32, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true, st);
+        launch_cg_reduce_dot(v.c, 1, n, v, 1.0f, part, S, st);
     } else {
-        op_apply(op, c, x, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true, st);
-        launch_cg_start(b, x, r, u, n, false, op->cg_part.as<double>(), S, reltol, abstol,
-                        maxiter, true, c, st);
+        op_passes(op, v.u32, st, &S->done);
+        launch_cg_reduce_dot(op->partial.as<float2>(), op->G, n, v, op_scale(op), part, S, st);
+    }
+    launch_cg_xr(v, n, part, S, st);
+}
+
+template <class VT>
+static void cg_prologue(msr_normalop* op, const CgVecs<VT>& v, float2* x, const float2* b,
+                        int maxiter, double reltol, double abstol, bool initially_zero,
+                        hipStream_t st) {
+    const int64_t n = op->V * op->K;
+    auto* S = op->cg_scal.as<CgScalars>();
+    if (initially_zero) {
+        launch_cg_start(b, x, v, n, true, op->cg_part.as<double>(), S, reltol, abstol, maxiter,
+                        false, st);
+    } else {
+        op_apply(op, v.c, x, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true, st);
+        launch_cg_start(b, x, v, n, false, op->cg_part.as<double>(), S, reltol, abstol, maxiter,
+                        true, st);
     }
 }
 
-void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
-           double abstol, bool initially_zero, int mode, int* iters_out, double* res_out,
-           hipStream_t st) {
-    cg_buffers(op);
-    const int64_t n = op->V * op->K;
+template <class VT>
+static void cg_epilogue(msr_normalop* op, const CgVecs<VT>& v, float2* x, hipStream_t st) {
+    if constexpr (std::is_same<VT, double2>::value)
+        launch_to_f32(v.x, x, op->V * op->K, st);
+}
+
+template <class VT>
+static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
+                   double abstol, bool initially_zero, int mode, hipStream_t st) {
+    const CgVecs<VT> v = cg_vecs<VT>(op, x);
     auto* S = op->cg_scal.as<CgScalars>();
     if (mode == MSR_CG_DEVICE) {
-        cg_prologue(op, x, b, maxiter, reltol, abstol, initially_zero, st);
-        for (int it = 0; it < maxiter; ++it) cg_iteration_device(op, x, st);
+        cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
+        for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, false, st);
+        cg_epilogue(op, v, x, st);
     } else if (mode == MSR_CG_GRAPH) {
+        const int fp = std::is_same<VT, double2>::value ? 1 : 0;
         const bool hit = op->graph && op->graph_key[0] == x && op->graph_key[1] == b &&
                          op->graph_key[2] == (const void*)(intptr_t)(initially_zero ? 1 : 2) &&
                          op->graph_maxiter == maxiter && op->graph_stream == st &&
-                         op->graph_tol[0] == reltol && op->graph_tol[1] == abstol;
+                         op->graph_tol[0] == reltol && op->graph_tol[1] == abstol &&
+                         op->graph_fp64 == fp;
         if (!hit) {
             if (op->graph) {
                 (void)hipGraphExecDestroy(op->graph);
@@ -329,8 +365,9 @@ void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double rel
             hipGraph_t g = nullptr;
             MSR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                cg_prologue(op, x, b, maxiter, reltol, abstol, initially_zero, st);
-                for (int it = 0; it < maxiter; ++it) cg_iteration_device(op, x, st);
+                cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
+                for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, false, st);
+                cg_epilogue(op, v, x, st);
             } catch (...) {
                 (void)hipStreamEndCapture(st, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -349,32 +386,43 @@ void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double rel
             op->graph_stream = st;
             op->graph_tol[0] = reltol;
             op->graph_tol[1] = abstol;
+            op->graph_fp64 = fp;
         }
         MSR_HIP(hipGraphLaunch(op->graph, st));
     } else if (mode == MSR_CG_HOST) {
-        // IterativeSolvers-style: scalars are read back to the host every
-        // iteration; the apply is the per-coil reference order.
-        float2 *r = op->cg_r.as<float2>(), *u = op->cg_u.as<float2>(), *c = op->cg_c.as<float2>();
+        // IterativeSolvers-style: the step length and the residual norm are
+        // read back to the host every iteration; applies run coil by coil.
+        const int64_t n = op->V * op->K;
         double* part = op->cg_part.as<double>();
-        cg_prologue(op, x, b, maxiter, reltol, abstol, initially_zero, st);
+        cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
         CgScalars hs;
         for (int it = 0; it < maxiter; ++it) {
             MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
             MSR_HIP(hipStreamSynchronize(st));
             if (hs.done) break;
-            launch_cg_u(r, u, n, S, st);
-            op_apply_percoil(op, c, u, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true, st);
-            launch_cg_reduce_dot(c, 1, n, c, u, 1.0f, part, S, st);
+            launch_cg_u(v, n, S, st);
+            op_apply_percoil(op, v.c, v.u32, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true,
+                             st);
+            launch_cg_reduce_dot(v.c, 1, n, v, 1.0f, part, S, st);
             MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
             MSR_HIP(hipStreamSynchronize(st));
-            launch_cg_xr(x, r, u, c, n, part, S, st);
+            launch_cg_xr(v, n, part, S, st);
         }
+        cg_epilogue(op, v, x, st);
     } else {
         fail(MSR_ERR_ARGUMENT, "unknown CG mode %d", mode);
     }
+}
+
+void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
+           double abstol, bool initially_zero, int mode, bool fp64, int* iters_out,
+           double* res_out, hipStream_t st) {
+    cg_buffers(op, fp64);
+    if (fp64) cg_run<double2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, st);
+    else cg_run<float2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, st);
     if (iters_out || res_out) {
         CgScalars hs;
-        MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+        MSR_HIP(hipMemcpyAsync(&hs, op->cg_scal.p, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
         MSR_HIP(hipStreamSynchronize(st));
         if (iters_out) *iters_out = hs.iter;
         if (res_out) *res_out = hs.res;

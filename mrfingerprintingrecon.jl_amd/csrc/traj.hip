@@ -11,6 +11,7 @@
 
 msr_ctx::~msr_ctx() {
     for (auto& kv : twiddles) (void)hipFree(kv.second);
+    for (auto& kv : deapod) (void)hipFree(kv.second);
 }
 
 namespace msr {

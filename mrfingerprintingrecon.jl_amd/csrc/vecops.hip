@@ -5,9 +5,9 @@
 //   u = r + (res^2/prev^2) u;  c = A u;  alpha = res^2 / dot(u, c);
 //   x += alpha u;  r -= alpha c;  prev = res;  res = ||r||
 // stopping when iter >= maxiter or res <= max(reltol*||r0||, abstol).
-// Vectors stay FP32; dot products, norms and step lengths are accumulated and
-// kept in FP64 on the device, reduced in a fixed order (block partials, then
-// one block) so results are run-to-run deterministic.
+// Dot products, norms and step lengths are accumulated and kept in FP64 on the
+// device, reduced in a fixed order (block partials, then one block) so results
+// are run-to-run deterministic.  The vectors are ComplexF32 or FP64 (VT).
 #include "vecops.hpp"
 
 namespace msr {
@@ -76,20 +76,41 @@ void launch_scale(float2* x, int64_t n, float2 beta, bool beta_zero, hipStream_t
 }
 
 // ------------------------------------------------------------------ CG
+template <class VT> struct VecT;
+template <> struct VecT<float2> {
+    using S = float;
+    __device__ static float2 make(double a, double b) { return make_float2((float)a, (float)b); }
+};
+template <> struct VecT<double2> {
+    using S = double;
+    __device__ static double2 make(double a, double b) { return make_double2(a, b); }
+};
+
+template <class VT>
+__device__ __forceinline__ VT to_vt(float2 a) { return VecT<VT>::make(a.x, a.y); }
+
+template <class VT>
 __global__ void __launch_bounds__(RED_THREADS)
-cg_start_kernel(const float2* __restrict__ b, float2* __restrict__ x, float2* __restrict__ r,
-                float2* __restrict__ u, int64_t n, int zero_x, int r_minus_c,
-                const float2* __restrict__ c, double* __restrict__ part) {
+cg_start_kernel(const float2* __restrict__ b, const float2* __restrict__ x_in, CgVecs<VT> v,
+                int64_t n, int zero_x, int r_minus_c, double* __restrict__ part) {
     __shared__ double s[RED_THREADS / 64];
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        float2 v = b[i];
-        if (r_minus_c) v = csub(v, c[i]);
-        r[i] = v;
-        u[i] = make_float2(0.f, 0.f);
-        if (zero_x) x[i] = make_float2(0.f, 0.f);
-        acc += (double)v.x * v.x + (double)v.y * v.y;
+        const float2 bb = b[i];
+        double rx = bb.x, ry = bb.y;
+        if (r_minus_c) {
+            const float2 cc = v.c[i];
+            rx -= cc.x;
+            ry -= cc.y;
+        }
+        v.r[i] = VecT<VT>::make(rx, ry);
+        v.u[i] = VecT<VT>::make(0.0, 0.0);
+        if (zero_x) v.x[i] = VecT<VT>::make(0.0, 0.0);
+        else if ((const void*)x_in != (const void*)v.x) v.x[i] = to_vt<VT>(x_in[i]);
+        // norm of the stored residual
+        const VT rr = v.r[i];
+        acc += (double)rr.x * rr.x + (double)rr.y * rr.y;
     }
     double t = block_sum(acc, s);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -110,52 +131,59 @@ __global__ void cg_start_finish(const double* __restrict__ part, int nb, CgScala
     S->done = (maxiter <= 0 || res <= S->tol) ? 1 : 0;
 }
 
-void launch_cg_start(const float2* b, float2* x, float2* r, float2* u, int64_t n, bool zero_x,
-                     double* part, CgScalars* S, double reltol, double abstol, int maxiter,
-                     bool r_minus_c, const float2* c, hipStream_t st) {
+template <class VT>
+void launch_cg_start(const float2* b, const float2* x_in, const CgVecs<VT>& v, int64_t n,
+                     bool zero_x, double* part, CgScalars* S, double reltol, double abstol,
+                     int maxiter, bool r_minus_c, hipStream_t st) {
     const int g = red_grid(n);
-    hipLaunchKernelGGL(cg_start_kernel, dim3(g), dim3(RED_THREADS), 0, st, b, x, r, u, n,
-                       (int)zero_x, (int)r_minus_c, c, part);
+    hipLaunchKernelGGL(cg_start_kernel<VT>, dim3(g), dim3(RED_THREADS), 0, st, b, x_in, v, n,
+                       (int)zero_x, (int)r_minus_c, part);
     MSR_CHECK_LAUNCH();
     hipLaunchKernelGGL(cg_start_finish, dim3(1), dim3(64), 0, st, part, g, S, reltol, abstol,
                        maxiter);
     MSR_CHECK_LAUNCH();
 }
 
-__global__ void cg_u_kernel(const float2* __restrict__ r, float2* __restrict__ u, int64_t n,
-                            const CgScalars* __restrict__ S) {
+template <class VT>
+__global__ void cg_u_kernel(CgVecs<VT> v, int64_t n, const CgScalars* __restrict__ S) {
     if (S->done) return;
-    const float beta = (float)((S->res * S->res) / (S->prev * S->prev));
+    using T = typename VecT<VT>::S;
+    const T beta = (T)((S->res * S->res) / (S->prev * S->prev));
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        float2 a = r[i], v = u[i];
-        u[i] = make_float2(fmaf(beta, v.x, a.x), fmaf(beta, v.y, a.y));
+        const VT a = v.r[i], w = v.u[i];
+        VT nu;
+        nu.x = a.x + beta * w.x;
+        nu.y = a.y + beta * w.y;
+        v.u[i] = nu;
+        if ((void*)v.u32 != (void*)v.u) v.u32[i] = make_float2((float)nu.x, (float)nu.y);
     }
 }
 
-void launch_cg_u(const float2* r, float2* u, int64_t n, CgScalars* S, hipStream_t st) {
+template <class VT>
+void launch_cg_u(const CgVecs<VT>& v, int64_t n, CgScalars* S, hipStream_t st) {
     int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(cg_u_kernel, dim3(grid), dim3(256), 0, st, r, u, n, S);
+    hipLaunchKernelGGL(cg_u_kernel<VT>, dim3(grid), dim3(256), 0, st, v, n, S);
     MSR_CHECK_LAUNCH();
 }
 
+template <class VT>
 __global__ void __launch_bounds__(RED_THREADS)
-cg_reduce_dot_kernel(const float2* __restrict__ partial, int G, int64_t n, float2* __restrict__ c,
-                     const float2* __restrict__ u, float scale, double* __restrict__ part,
-                     const CgScalars* __restrict__ S) {
+cg_reduce_dot_kernel(const float2* __restrict__ partial, int G, int64_t n, CgVecs<VT> v,
+                     float scale, double* __restrict__ part, const CgScalars* __restrict__ S) {
     if (S->done) return;
     __shared__ double s[RED_THREADS / 64];
     double ar = 0.0, ai = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        float2 v = partial[i];
-        for (int g = 1; g < G; ++g) v = cadd(v, partial[(int64_t)g * n + i]);
-        v = cscale(v, scale);
-        c[i] = v;
-        const float2 w = u[i];
+        float2 c = partial[i];
+        for (int g = 1; g < G; ++g) c = cadd(c, partial[(int64_t)g * n + i]);
+        c = cscale(c, scale);
+        v.c[i] = c;
+        const VT w = v.u[i];
         // conj(u) * c
-        ar += (double)w.x * v.x + (double)w.y * v.y;
-        ai += (double)w.x * v.y - (double)w.y * v.x;
+        ar += (double)w.x * c.x + (double)w.y * c.y;
+        ai += (double)w.x * c.y - (double)w.y * c.x;
     }
     double tr = block_sum(ar, s);
     double ti = block_sum(ai, s);
@@ -180,32 +208,38 @@ __global__ void cg_alpha_kernel(const double* __restrict__ part, int nb, CgScala
     S->alpha_im = -r2 * di / den;
 }
 
-void launch_cg_reduce_dot(const float2* partial, int G, int64_t n, float2* c, const float2* u,
+template <class VT>
+void launch_cg_reduce_dot(const float2* partial, int G, int64_t n, const CgVecs<VT>& v,
                           float scale, double* part, CgScalars* S, hipStream_t st) {
     const int g = red_grid(n);
-    hipLaunchKernelGGL(cg_reduce_dot_kernel, dim3(g), dim3(RED_THREADS), 0, st, partial, G, n, c,
-                       u, scale, part, S);
+    hipLaunchKernelGGL(cg_reduce_dot_kernel<VT>, dim3(g), dim3(RED_THREADS), 0, st, partial, G,
+                       n, v, scale, part, S);
     MSR_CHECK_LAUNCH();
     hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(64), 0, st, part, g, S);
     MSR_CHECK_LAUNCH();
 }
 
+template <class VT>
 __global__ void __launch_bounds__(RED_THREADS)
-cg_xr_kernel(float2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ u,
-             const float2* __restrict__ c, int64_t n, double* __restrict__ part,
+cg_xr_kernel(CgVecs<VT> v, int64_t n, double* __restrict__ part,
              const CgScalars* __restrict__ S) {
     if (S->done) return;
+    using T = typename VecT<VT>::S;
     __shared__ double s[RED_THREADS / 64];
-    const float2 al = make_float2((float)S->alpha_re, (float)S->alpha_im);
+    const T ar = (T)S->alpha_re, ai = (T)S->alpha_im;
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        float2 xv = x[i], rv = r[i];
-        cfma(xv, al, u[i]);
-        const float2 ac = cmul(al, c[i]);
-        rv = csub(rv, ac);
-        x[i] = xv;
-        r[i] = rv;
+        VT xv = v.x[i], rv = v.r[i];
+        const VT u = v.u[i];
+        const float2 c32 = v.c[i];
+        const T cx = (T)c32.x, cy = (T)c32.y;
+        xv.x += ar * u.x - ai * u.y;
+        xv.y += ar * u.y + ai * u.x;
+        rv.x -= ar * cx - ai * cy;
+        rv.y -= ar * cy + ai * cx;
+        v.x[i] = xv;
+        v.r[i] = rv;
         acc += (double)rv.x * rv.x + (double)rv.y * rv.y;
     }
     double t = block_sum(acc, s);
@@ -222,13 +256,37 @@ __global__ void cg_iter_finish(const double* __restrict__ part, int nb, CgScalar
     S->done = (S->iter >= S->maxiter || S->res <= S->tol) ? 1 : 0;
 }
 
-void launch_cg_xr(float2* x, float2* r, const float2* u, const float2* c, int64_t n, double* part,
-                  CgScalars* S, hipStream_t st) {
+template <class VT>
+void launch_cg_xr(const CgVecs<VT>& v, int64_t n, double* part, CgScalars* S, hipStream_t st) {
     const int g = red_grid(n);
-    hipLaunchKernelGGL(cg_xr_kernel, dim3(g), dim3(RED_THREADS), 0, st, x, r, u, c, n, part, S);
+    hipLaunchKernelGGL(cg_xr_kernel<VT>, dim3(g), dim3(RED_THREADS), 0, st, v, n, part, S);
     MSR_CHECK_LAUNCH();
     hipLaunchKernelGGL(cg_iter_finish, dim3(1), dim3(64), 0, st, part, g, S);
     MSR_CHECK_LAUNCH();
 }
+
+__global__ void to_f32_kernel(const double2* __restrict__ a, float2* __restrict__ b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+void launch_to_f32(const double2* x64, float2* x32, int64_t n, hipStream_t st) {
+    int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(to_f32_kernel, dim3(grid), dim3(256), 0, st, x64, x32, n);
+    MSR_CHECK_LAUNCH();
+}
+
+#define MSR_INST_CG(VT)                                                                          \
+    template void launch_cg_start<VT>(const float2*, const float2*, const CgVecs<VT>&, int64_t,  \
+                                      bool, double*, CgScalars*, double, double, int, bool,      \
+                                      hipStream_t);                                              \
+    template void launch_cg_u<VT>(const CgVecs<VT>&, int64_t, CgScalars*, hipStream_t);          \
+    template void launch_cg_reduce_dot<VT>(const float2*, int, int64_t, const CgVecs<VT>&, float, \
+                                           double*, CgScalars*, hipStream_t);                    \
+    template void launch_cg_xr<VT>(const CgVecs<VT>&, int64_t, double*, CgScalars*, hipStream_t);
+MSR_INST_CG(float2)
+MSR_INST_CG(double2)
+#undef MSR_INST_CG
 
 }  // namespace msr

@@ -16,15 +16,31 @@ void launch_epilogue(const float2* partial, int G, int64_t n, float2* x, float2 
 // x = beta*x (or 0 when beta_zero)
 void launch_scale(float2* x, int64_t n, float2 beta, bool beta_zero, hipStream_t st);
 
-// CG building blocks (scalars in CgScalars on the device)
-void launch_cg_start(const float2* b, float2* x, float2* r, float2* u, int64_t n, bool zero_x,
-                     double* part, CgScalars* S, double reltol, double abstol, int maxiter,
-                     bool r_minus_c, const float2* c, hipStream_t st);
-void launch_cg_u(const float2* r, float2* u, int64_t n, CgScalars* S, hipStream_t st);
-// c = scale*sum_g partial[g]; dot(u, c) partials; alpha = res^2 / dot
-void launch_cg_reduce_dot(const float2* partial, int G, int64_t n, float2* c, const float2* u,
+// CG building blocks (scalars in CgScalars on the device).  VT is the type of
+// the solution / residual / search vectors: float2 (ComplexF32 throughout) or
+// double2 (FP64 vectors; the apply still reads an FP32 copy of u).
+template <class VT>
+struct CgVecs {
+    VT* x;          // solution
+    VT* r;          // residual
+    VT* u;          // search direction
+    float2* u32;    // FP32 copy of u fed to the apply (== u for float2)
+    float2* c;      // A u (FP32, from the apply)
+};
+
+template <class VT>
+void launch_cg_start(const float2* b, const float2* x_in, const CgVecs<VT>& v, int64_t n,
+                     bool zero_x, double* part, CgScalars* S, double reltol, double abstol,
+                     int maxiter, bool r_minus_c, hipStream_t st);
+template <class VT>
+void launch_cg_u(const CgVecs<VT>& v, int64_t n, CgScalars* S, hipStream_t st);
+// c = scale*sum_g partial[g]; dot(u, c) in FP64; alpha = res^2 / dot
+template <class VT>
+void launch_cg_reduce_dot(const float2* partial, int G, int64_t n, const CgVecs<VT>& v,
                           float scale, double* part, CgScalars* S, hipStream_t st);
-void launch_cg_xr(float2* x, float2* r, const float2* u, const float2* c, int64_t n, double* part,
-                  CgScalars* S, hipStream_t st);
+template <class VT>
+void launch_cg_xr(const CgVecs<VT>& v, int64_t n, double* part, CgScalars* S, hipStream_t st);
+// x32 = (float) x64
+void launch_to_f32(const double2* x64, float2* x32, int64_t n, hipStream_t st);
 
 }  // namespace msr

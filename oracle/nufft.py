@@ -159,7 +159,7 @@ def nufft64_type2(kc, x, shift, w=W64, beta=BETA64):
     return native.es_interp_f64(s, g, nos, w, beta)
 
 
-def type1(kc, v, n, shift, exact_limit=2e10):
+def type1(kc, v, n, shift, exact_limit=2e9):
     """Exact NUDFT when affordable (M * prod(n) * nch below exact_limit), else
     the FP64 NUFFT."""
     v = np.atleast_2d(v)
@@ -169,7 +169,7 @@ def type1(kc, v, n, shift, exact_limit=2e10):
     return nufft64_type1(kc, v, n, shift)
 
 
-def type2(kc, x, shift, exact_limit=2e10):
+def type2(kc, x, shift, exact_limit=2e9):
     x = np.asarray(x)
     cost = float(np.asarray(kc).shape[0]) * float(np.prod(x.shape[1:])) * x.shape[0]
     if cost <= exact_limit:
