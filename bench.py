@@ -1,0 +1,294 @@
+#!/usr/bin/env python3
+"""CG-recon solves/s of the MI355X Toeplitz subspace reconstruction.
+
+Workload (one step = one solve): single 2D subspace reconstruction, image
+256 x 256 (Toeplitz grid 512 x 512), 20 coils, golden-angle radial trajectory
+with 1000 frames x 1 spoke x 512 samples, K = 5 temporal basis (SVD of a T2
+dictionary), ComplexF32 data.  A solve is the backprojection (right-hand side)
+plus 25 CG iterations with the Toeplitz normal operator; the kernel, coil maps
+and sorted trajectory stay resident on the device between solves.
+
+Prints one JSON line with solves/s, p50/p99 latency, the plain
+one-recon-at-a-time GPU path (reference GPU order: per-coil applies that re-read
+the kernel, IterativeSolvers-style host-synchronised CG with ComplexF32
+vectors) timed beside it, the HBM roofline of the Toeplitz apply, the CPU
+baseline (FP32 port of the reference CPU path) and the relative error of the
+solve against an FP64 oracle.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "mrfingerprintingrecon.jl_amd")
+
+N_IMG = (256, 256)
+COILS = 20
+K = 5
+NT = 1000
+NR = 512
+CG_ITERS = 25
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    name = "mrisubspacerecon_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(PKG, "__init__.py"),
+                                                  submodule_search_locations=[PKG])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_inputs(msr, seed=1002):
+    import numpy as np
+    synth = msr.synth
+    trj = synth.traj_2d_radial_goldenratio(NR, 1, NT)              # (Nt, Nsamp, 2)
+    U = synth.t2_basis(NT, K, seed=seed)                           # (K, Nt) complex64
+    cm = synth.gaussian_coil_maps(N_IMG, COILS, seed=seed)         # (C, Ny, Nx)
+    data = synth.gaussian_blob_kspace(trj, None, N_IMG, U, COILS, seed=seed, n_blobs=2)
+    return trj, U, cm, np.ascontiguousarray(data)
+
+
+def apply_bytes(nkm, lam_real):
+    """Algorithmic bytes of one fused apply: 8*(2VK + 2CV + 8CKV) + G*P*s_lam
+    (b read and x written once, coil maps read twice, the x-FFT and y-FFT
+    intermediates written and read once each, the kernel read once)."""
+    V = N_IMG[0] * N_IMG[1]
+    P = K * (K + 1) // 2
+    s_lam = 4 if lam_real else 8
+    return 8 * (2 * V * K + 2 * COILS * V + 8 * COILS * K * V) + nkm * P * s_lam
+
+
+def read_traffic():
+    """HBM bytes per apply from the committed rocprofv3 counter summary, if any
+    (line 'HBM_BYTES_PER_APPLY = <int>')."""
+    p = os.path.join(ROOT, "profiles", "apply2d_256_c20_k5_counters.md")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.strip().startswith("HBM_BYTES_PER_APPLY"):
+            try:
+                return int(line.split("=")[1].strip().split()[0])
+            except ValueError:
+                return None
+    return None
+
+
+def percentile(v, q):
+    s = sorted(v)
+    if not s:
+        return float("nan")
+    i = min(len(s) - 1, max(0, int(math.ceil(q / 100.0 * len(s))) - 1))
+    return s[i]
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end FP64 pipeline check")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        sys.exit("bench.py measures one GPU: this build has no multi-GPU path "
+                 "(a reconstruction shards by coil or by job; see DESIGN.md)")
+
+    import numpy as np
+    import torch
+    msr = load_pkg()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    log("generating inputs")
+    trj, U, cm, data = make_inputs(msr)
+    trj_d = torch.from_numpy(trj).to(dev)
+    U_d = torch.from_numpy(U).to(dev)
+    cml = [torch.from_numpy(c).to(dev) for c in cm]
+    data_d = torch.from_numpy(data).to(dev)
+
+    log("trajectory plan + kernel build")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    traj = msr.plan_trajectory(trj_d)
+    A = msr.NFFTNormalOp(N_IMG, traj, U_d, cmaps=cml)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    nvec = K * N_IMG[0] * N_IMG[1]
+    b = torch.empty((K,) + N_IMG[::-1], dtype=torch.complex64, device=dev)
+    x = torch.empty(nvec, dtype=torch.complex64, device=dev)
+    bflat = b.reshape(-1)
+
+    def solve():
+        msr.calculate_backprojection(data_d, traj, cml, U=U_d, out=b)
+        msr.cg_(x, A, bflat, maxiter=CG_ITERS, reltol=0.0, initially_zero=True, mode="graph")
+
+    def solve_plain():
+        msr.calculate_backprojection(data_d, traj, cml, U=U_d, out=b)
+        msr.cg_(x, A, bflat, maxiter=CG_ITERS, reltol=0.0, initially_zero=True, mode="host",
+                vec_fp64=False)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)]
+        t0 = time.perf_counter()
+        for s, e in evs:
+            s.record()
+            fn()
+            e.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = [s.elapsed_time(e) for s, e in evs]
+        return wall, lat
+
+    log(f"timing {args.steps} solves (warmup {args.warmup})")
+    wall, lat = timed(solve, args.steps, args.warmup)
+    solves_per_s = args.steps / wall
+    log(f"fused/graph: {1e3 * wall / args.steps:.3f} ms/solve")
+    x_gpu = x.cpu().numpy()
+    b_gpu = b.cpu().numpy()
+
+    log(f"timing {args.plain_steps} plain solves")
+    pwall, plat = timed(solve_plain, args.plain_steps, 1)
+    plain_sps = args.plain_steps / pwall
+    log(f"plain: {1e3 * pwall / args.plain_steps:.3f} ms/solve")
+
+    # ---- roofline of the Toeplitz apply, timed with events on its stream
+    y = torch.empty_like(x)
+    xin = torch.randn(nvec, dtype=torch.complex64, device=dev)
+    for _ in range(5):
+        A.A.mul_(y, xin)
+    torch.cuda.synchronize()
+    napp = 50
+    s_ev, e_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s_ev.record()
+    for _ in range(napp):
+        A.A.mul_(y, xin)
+    e_ev.record()
+    torch.cuda.synchronize()
+    t_apply_ms = s_ev.elapsed_time(e_ev) / napp
+    B = apply_bytes(A.A.nkm, A.A.lambda_real)  # kernel bytes counted on the kmask only
+    achieved = B / (t_apply_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "Toeplitz apply (3 fused passes + epilogue)",
+                "bytes_per_launch": B, "ms_per_launch": round(t_apply_ms, 4)}
+    # plain (per-coil) apply for comparison
+    s_ev.record()
+    for _ in range(5):
+        A.A.mul_(y, xin, percoil=True)
+    e_ev.record()
+    torch.cuda.synchronize()
+    t_apply_plain = s_ev.elapsed_time(e_ev) / 5
+
+    result = {
+        "metric": "CG recon solves/s + p50/p99 latency vs plain GPU path @<=1e-4 rel err vs FP64",
+        "value": round(solves_per_s, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (complex64 FFT/kernel/gridding; fp64 CG reductions and vectors)",
+        "data": "synthetic",
+        "config": {"workload": "single 2D recon 256x256 (grid 512x512), 20 coils, 1000 frames x "
+                               "1 golden-angle spoke x 512 samples, K=5, BP + 25-iteration CG",
+                   "img": list(N_IMG), "coils": COILS, "K": K, "frames": NT,
+                   "samples_per_frame": NR, "cg_iters": CG_ITERS},
+        "latency_ms": {"p50": round(percentile(lat, 50), 4), "p99": round(percentile(lat, 99), 4)},
+        "plain_gpu": {"value": round(plain_sps, 3), "unit": "solves/s",
+                      "ms_per_solve": round(1e3 * pwall / args.plain_steps, 3),
+                      "p50_ms": round(percentile(plat, 50), 3),
+                      "speedup": round(solves_per_s / plain_sps, 3),
+                      "apply_ms": round(t_apply_plain, 4)},
+        "kernel_build_ms": round(build_ms, 1),
+        "roofline": roofline,
+        "cpu_baseline": None,
+        "rel_err_vs_fp64": None,
+    }
+
+    if not args.no_cpu:
+        result["cpu_baseline"], result["rel_err_vs_fp64"] = cpu_leg(
+            msr, trj, U, cm, data, A, x_gpu, b_gpu, e2e=not args.no_e2e)
+    print(json.dumps(result), flush=True)
+
+
+def cpu_leg(msr, trj, U, cm, data, A, x_gpu, b_gpu, e2e=True):
+    """CPU baseline (FP32 port of the reference CPU path, bounded sample) and
+    the FP64 oracle check of the GPU solve (test infrastructure from oracle/,
+    used here only as the baseline and the checker)."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import cg as ocg
+    from oracle import cpu_port, native, toeplitz
+    native.build()
+    cores = os.cpu_count()
+    lam = A.A.Λ.cpu().numpy()
+    km = A.A.kmask_indcs.cpu().numpy() - 1
+    # --- CPU baseline: one backprojection and two applies, solve = BP + 25 applies
+    log("cpu baseline: port apply")
+    op = cpu_port.CpuNormalOp(N_IMG, lam, km, cm, workers=cores)
+    v = np.asarray(b_gpu, dtype=np.complex64).reshape(-1)
+    op.apply(v)
+    t0 = time.perf_counter()
+    op.apply(v)
+    t_apply = time.perf_counter() - t0
+    log(f"cpu apply {t_apply:.2f} s; port backprojection")
+    t0 = time.perf_counter()
+    cpu_port.backprojection_port(data, trj, cm, U, workers=cores)
+    t_bp = time.perf_counter() - t0
+    t_solve = t_bp + CG_ITERS * t_apply
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                 if l.startswith("model name")][0]
+    except Exception:
+        model = "unknown"
+    cpu = {"value": round(1.0 / t_solve, 5), "unit": "solves/s", "cores": cores, "kind": "port",
+           "sample": f"1 backprojection ({t_bp:.2f} s) + 1 timed apply ({t_apply:.2f} s) of the "
+                     f"same 256x256/20-coil/K=5 workload; solve time = t_bp + 25 t_apply "
+                     f"= {t_solve:.1f} s; {model}"}
+    # --- FP64 checks
+    log("fp64 oracle: solve on the device-built operator")
+    cm2 = cm.astype(np.complex128)
+    Ad = toeplitz.NormalOp(N_IMG, lam.astype(np.complex128), km, cm2)
+    ref_solve = ocg.cg(Ad, np.asarray(b_gpu, dtype=np.complex128).reshape(-1),
+                       maxiter=CG_ITERS, reltol=0.0)
+    err = {"solve_given_operator": float(np.linalg.norm(x_gpu - ref_solve) / np.linalg.norm(ref_solve))}
+    if e2e:
+        log("fp64 oracle: end-to-end pipeline (backprojection, kernel, CG)")
+        b64 = toeplitz.backprojection(data, trj, cm, U=U)
+        err["backprojection"] = float(np.linalg.norm(b_gpu - b64) / np.linalg.norm(b64))
+        lam64, km64 = toeplitz.toeplitz_kernel(N_IMG, trj, U)
+        err["kmask_identical"] = bool(np.array_equal(km64, km))
+        err["kernel"] = float(np.linalg.norm(lam - lam64) / np.linalg.norm(lam64))
+        A64 = toeplitz.NormalOp(N_IMG, lam64, km64, cm2)
+        ref = ocg.cg(A64, b64.reshape(-1), maxiter=CG_ITERS, reltol=0.0)
+        err["end_to_end"] = float(np.linalg.norm(x_gpu - ref) / np.linalg.norm(ref))
+    log(f"errors {err}")
+    return cpu, err
+
+
+if __name__ == "__main__":
+    main()

@@ -27,7 +27,7 @@ from . import _lib, synth  # noqa: F401
 from ._lib import ArgumentError, ErrorException  # noqa: F401
 
 __all__ = ["NFFTNormalOp", "calculate_backprojection", "cg", "cg_", "mul_", "LinearOperator",
-           "ArgumentError", "ErrorException", "synth"]
+           "plan_trajectory", "ArgumentError", "ErrorException", "synth"]
 
 _ctxs = {}
 
@@ -77,6 +77,13 @@ def _cmaps_list(cmaps, img_shape=None):
     if isinstance(cmaps, (tuple, list)) and len(cmaps) == 1 and not hasattr(cmaps[0], "shape"):
         return []  # reference default cmaps = (1,)
     return list(cmaps)
+
+
+def plan_trajectory(trj, sample_mask=None):
+    """Device-resident sample selection of a trajectory (masked, frame-major)
+    that NFFTNormalOp and calculate_backprojection accept in place of trj, so
+    repeated calls skip the selection and the tile sort."""
+    return _Traj(trj, sample_mask)
 
 
 class _Traj:
@@ -287,11 +294,13 @@ def _test_dimension(data, trj_nt, trj_nsamp, trj_d, U_nt, img_shape, ncoil):
 
 
 def calculate_backprojection(data, trj, cmaps_or_img_shape, U=None, sample_mask=None,
-                             density_compensation="none", verbose=False):
+                             density_compensation="none", verbose=False, out=None):
     """calculate_backprojection(data, trj, cmaps; U, sample_mask,
     density_compensation) -> (K, *N[::-1]) coil-combined
     (ext/MRISubspaceReconCUDAExt/BackProjection.jl:36-71), or with img_shape
-    -> (C, K, *N[::-1]) per coil (:1-34).  4D data/trj wrapper :74-79."""
+    -> (C, K, *N[::-1]) per coil (:1-34).  4D data/trj wrapper :74-79.
+    ``trj`` may also be a trajectory handle from ``plan_trajectory`` (reused
+    sample selection and tile sort); ``out`` an output tensor to write into."""
     torch = _torch()
     data = _dev_tensor(data, "data", torch.complex64)
     if data.dim() == 4:  # (C, Nt, Ncyc, Nr)
@@ -336,12 +345,13 @@ def calculate_backprojection(data, trj, cmaps_or_img_shape, U=None, sample_mask=
                                 "masked sample")
         mode = _lib.DCF_ARRAY
         dcf_ptr = _ptr(w)
-    if per_coil:
-        out = torch.empty((C, K) + img_shape[::-1], dtype=torch.complex64, device=data.device)
-        carr = None
-    else:
-        out = torch.empty((K,) + img_shape[::-1], dtype=torch.complex64, device=data.device)
-        carr = (ctypes.c_void_p * len(cl))(*[c.data_ptr() for c in cl])
+    shape = ((C, K) if per_coil else (K,)) + img_shape[::-1]
+    if out is None:
+        out = torch.empty(shape, dtype=torch.complex64, device=data.device)
+    elif (out.numel() != math.prod(shape) or out.dtype != torch.complex64
+          or not out.is_contiguous() or out.device != data.device):
+        raise ArgumentError(f"out must be a contiguous complex64 tensor of {shape}")
+    carr = None if per_coil else (ctypes.c_void_p * len(cl))(*[c.data_ptr() for c in cl])
     _lib.call("msr_backprojection", _ctx(data.device.index), traj.h, len(img_shape),
               _i64arr(img_shape), _ptr(data), C, K, uptr, real, carr, mode, dcf_ptr, _ptr(out),
               _stream_ptr(data.device))

@@ -1,16 +1,225 @@
-// Fused 3D Toeplitz subspace normal operator.
-#include "apply3d.hpp"
+// Fused 3D Toeplitz subspace normal operator: five streaming passes over the
+// 2N grid, each intermediate written once and read once, the kernel read once
+// per apply for all coils (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:193-218
+// runs per coil: fill, pad, batched cuFFT, kernel_mul!, cuFFT, crop).
+//   A  rows (y<Ny, z<Nz) of S_c b_k, zero-padded to 2Nx, FFT along x  -> T1[c][k][z][y][qx]
+//   B  columns along y, zero-padded to 2Ny, FFT                       -> T2[c][k][z][qy][qx]
+//   C  lines along z: pad to 2Nz, FFT, K x K kernel multiply, inverse FFT, crop to Nz (in place)
+//   D  columns along y: inverse FFT, crop to Ny                       -> T1
+//   E  rows: inverse FFT along x, crop to Nx, conj(S_c) multiply, coil sum -> partial
+#include "apply3d_c.hpp"
+#include "fftlines.hpp"
 
 namespace msr {
 
-void alloc_scratch_3d(msr_normalop*) {
-    fail(MSR_ERR_UNSUPPORTED, "3D Toeplitz apply is not built yet");
+template <int NX2>
+constexpr int rows3_block() {
+    return (16 * FFTPlan<NX2>::T) < 256 ? 16 * FFTPlan<NX2>::T : 256;
 }
-void launch_apply3d_passes(msr_normalop*, const float2*, hipStream_t, const int*) {
-    fail(MSR_ERR_UNSUPPORTED, "3D Toeplitz apply is not built yet");
+
+template <int NX2>
+__global__ void __launch_bounds__(256)
+pass_a_3d(const float2* __restrict__ b, const float2* __restrict__ cmaps, float2* __restrict__ T1,
+          const float2* __restrict__ tw, int nx, int64_t nrows, int K, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    constexpr int TP = FFTPlan<NX2>::T;
+    constexpr int BLK = rows3_block<NX2>();
+    constexpr int F = BLK / TP;
+    constexpr int LD = NX2 + 1;
+    __shared__ float2 s_tw[NX2];
+    __shared__ float2 s[F * LD];
+    load_twiddles<NX2>(s_tw, tw);
+    const int64_t r0 = (int64_t)blockIdx.x * F;
+    const int k = blockIdx.y, c = blockIdx.z;
+    const int64_t V = nrows * nx;
+    const float2* bk = b + (int64_t)k * V;
+    const float2* cm = cmaps ? cmaps + (int64_t)c * V : nullptr;
+    for (int e = threadIdx.x; e < F * NX2; e += BLK) {
+        const int l = e / NX2, xx = e - l * NX2;
+        const int64_t row = r0 + l;
+        float2 v = make_float2(0.f, 0.f);
+        if (row < nrows && xx < nx) {
+            v = bk[row * nx + xx];
+            if (cm) v = cmul(cm[row * nx + xx], v);
+        }
+        s[l * LD + xx] = v;
+    }
+    __syncthreads();
+    const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
+    fft_lds<NX2, -1>(s + g * LD, s_tw, t);
+    float2* out = T1 + ((int64_t)c * K + k) * nrows * NX2;
+    for (int e = threadIdx.x; e < F * NX2; e += BLK) {
+        const int l = e / NX2, q = e - l * NX2;
+        const int64_t row = r0 + l;
+        if (row < nrows) out[row * NX2 + q] = s[l * LD + q];
+    }
 }
-void launch_apply3d_passes_coil(msr_normalop*, const float2*, int, hipStream_t) {
-    fail(MSR_ERR_UNSUPPORTED, "3D Toeplitz apply is not built yet");
+
+template <int NX2>
+__global__ void __launch_bounds__(256)
+pass_e_3d(const float2* __restrict__ T1, const float2* __restrict__ cmaps,
+          float2* __restrict__ partial, const float2* __restrict__ tw, int nx, int64_t nrows,
+          int K, int C, int G, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    constexpr int TP = FFTPlan<NX2>::T;
+    constexpr int BLK = rows3_block<NX2>();
+    constexpr int F = BLK / TP;
+    constexpr int LD = NX2 + 1;
+    constexpr int NXC = NX2 / 2;
+    constexpr int EA = (F * NXC + BLK - 1) / BLK;
+    __shared__ float2 s_tw[NX2];
+    __shared__ float2 s[F * LD];
+    load_twiddles<NX2>(s_tw, tw);
+    const int64_t r0 = (int64_t)blockIdx.x * F;
+    const int k = blockIdx.y, grp = blockIdx.z;
+    const int c0 = (int)((int64_t)C * grp / G), c1 = (int)((int64_t)C * (grp + 1) / G);
+    const int64_t V = nrows * nx;
+    float2 acc[EA];
+#pragma unroll
+    for (int i = 0; i < EA; ++i) acc[i] = make_float2(0.f, 0.f);
+    const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
+    for (int c = c0; c < c1; ++c) {
+        const float2* in = T1 + ((int64_t)c * K + k) * nrows * NX2;
+        for (int e = threadIdx.x; e < F * NX2; e += BLK) {
+            const int l = e / NX2, q = e - l * NX2;
+            const int64_t row = r0 + l;
+            s[l * LD + q] = row < nrows ? in[row * NX2 + q] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        fft_lds<NX2, +1>(s + g * LD, s_tw, t);
+        const float2* cm = cmaps ? cmaps + (int64_t)c * V : nullptr;
+#pragma unroll
+        for (int i = 0; i < EA; ++i) {
+            const int e = threadIdx.x + i * BLK;
+            if (e < F * NXC) {
+                const int l = e / NXC, xx = e - l * NXC;
+                const int64_t row = r0 + l;
+                if (row < nrows) {
+                    const float2 v = s[l * LD + xx];
+                    if (cm) cfmac(acc[i], cm[row * nx + xx], v);
+                    else acc[i] = cadd(acc[i], v);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float2* pk = partial + ((int64_t)grp * K + k) * V;
+#pragma unroll
+    for (int i = 0; i < EA; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        if (e < F * NXC) {
+            const int l = e / NXC, xx = e - l * NXC;
+            const int64_t row = r0 + l;
+            if (row < nrows) pk[row * nx + xx] = acc[i];
+        }
+    }
+}
+
+static void launch_pass_c_3d(const Apply3DArgs& a, hipStream_t st, const int* skip) {
+    switch (a.K) {
+        case 1: launch_pass_c_3d_k1(a, st, skip); break;
+        case 2: launch_pass_c_3d_k2(a, st, skip); break;
+        case 3: launch_pass_c_3d_k3(a, st, skip); break;
+        case 4: launch_pass_c_3d_k4(a, st, skip); break;
+        case 5: launch_pass_c_3d_k5(a, st, skip); break;
+        case 6: launch_pass_c_3d_k6(a, st, skip); break;
+        case 7: launch_pass_c_3d_k7(a, st, skip); break;
+        case 8: launch_pass_c_3d_k8(a, st, skip); break;
+        default: fail(MSR_ERR_UNSUPPORTED, "number of coefficients K = %d not in 1..8", a.K);
+    }
+    MSR_CHECK_LAUNCH();
+}
+
+static void run_passes(const Apply3DArgs& a, msr_ctx* ctx, const float2* b, hipStream_t st,
+                       const int* skip) {
+    const int NX2 = 2 * a.nx;
+    const int64_t nrows = (int64_t)a.ny * a.nz;
+    bool ok = dispatch_fft_len(NX2, [&]<int N>() {
+        constexpr int F = rows3_block<N>() / FFTPlan<N>::T;
+        dim3 grid((unsigned)ceil_div(nrows, F), a.K, a.C);
+        hipLaunchKernelGGL(pass_a_3d<N>, grid, dim3(rows3_block<N>()), 0, st, b, a.cmaps, a.T1,
+                           a.tw_x, a.nx, nrows, a.K, skip);
+    });
+    MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", NX2);
+    MSR_CHECK_LAUNCH();
+    // B: y columns of T1 [(c,k,z)][y][qx] -> T2 [(c,k,z)][qy][qx]
+    LinePass pb;
+    pb.in = a.T1;
+    pb.out = a.T2;
+    pb.nf = 2 * a.ny;
+    pb.dir = -1;
+    pb.lin = a.ny;
+    pb.lout = 2 * a.ny;
+    pb.sel = SEL_FIRST;
+    pb.nb = (int64_t)a.C * a.K * a.nz;
+    pb.ncols = NX2;
+    pb.in_bstride = (int64_t)a.ny * NX2;
+    pb.in_stride = NX2;
+    pb.out_bstride = (int64_t)2 * a.ny * NX2;
+    pb.out_stride = NX2;
+    pb.skip = skip;
+    run_cols(pb, ctx, st);
+    launch_pass_c_3d(a, st, skip);
+    // D: inverse along y, crop to Ny: T2 -> T1
+    LinePass pd = pb;
+    pd.in = a.T2;
+    pd.out = a.T1;
+    pd.dir = +1;
+    pd.lin = 2 * a.ny;
+    pd.lout = a.ny;
+    pd.in_bstride = (int64_t)2 * a.ny * NX2;
+    pd.out_bstride = (int64_t)a.ny * NX2;
+    run_cols(pd, ctx, st);
+    ok = dispatch_fft_len(NX2, [&]<int N>() {
+        constexpr int F = rows3_block<N>() / FFTPlan<N>::T;
+        dim3 grid((unsigned)ceil_div(nrows, F), a.K, a.G);
+        hipLaunchKernelGGL(pass_e_3d<N>, grid, dim3(rows3_block<N>()), 0, st, a.T1, a.cmaps,
+                           a.partial, a.tw_x, a.nx, nrows, a.K, a.C, a.G, skip);
+    });
+    MSR_CHECK_LAUNCH();
+}
+
+static Apply3DArgs args3d(msr_normalop* op) {
+    Apply3DArgs a;
+    a.nx = (int)op->N[0];
+    a.ny = (int)op->N[1];
+    a.nz = (int)op->N[2];
+    a.K = op->K;
+    a.C = op->C;
+    a.cmaps = op->identity_cmaps ? nullptr : op->cmaps.as<float2>();
+    a.lam = op->lam.p;
+    a.lam_real = op->lam_real;
+    a.T1 = op->T.as<float2>();
+    a.T2 = op->T2.as<float2>();
+    a.partial = op->partial.as<float2>();
+    a.G = op->G;
+    a.Gb = op->Gb;
+    a.tw_x = twiddle_table(op->ctx, 2 * a.nx);
+    a.tw_y = twiddle_table(op->ctx, 2 * a.ny);
+    a.tw_z = twiddle_table(op->ctx, 2 * a.nz);
+    return a;
+}
+
+void alloc_scratch_3d(msr_normalop* op) {
+    const int64_t nx = op->N[0], ny = op->N[1], nz = op->N[2];
+    op->G = 1;
+    op->Gb = 1;
+    op->T.alloc(sizeof(float2) * (size_t)op->C * op->K * nz * ny * 2 * nx);
+    op->T2.alloc(sizeof(float2) * (size_t)op->C * op->K * nz * 2 * ny * 2 * nx);
+    op->partial.alloc(sizeof(float2) * (size_t)op->G * op->K * op->V);
+}
+
+void launch_apply3d_passes(msr_normalop* op, const float2* b, hipStream_t st, const int* skip) {
+    run_passes(args3d(op), op->ctx, b, st, skip);
+}
+
+void launch_apply3d_passes_coil(msr_normalop* op, const float2* b, int coil, hipStream_t st) {
+    Apply3DArgs a = args3d(op);
+    a.C = 1;
+    a.G = 1;
+    a.Gb = 1;
+    if (a.cmaps) a.cmaps += (size_t)coil * op->V;
+    run_passes(a, op->ctx, b, st, nullptr);
 }
 
 }  // namespace msr

@@ -1,0 +1,167 @@
+// Pass C of the fused 3D apply: for WC adjacent grid columns (qx) at one qy,
+// the K lines along z of every coil are zero-padded from Nz to 2Nz, FFT'd,
+// multiplied voxel by voxel by the packed K x K kernel (held in registers for
+// the whole coil loop), inverse FFT'd and cropped back to Nz, in place in
+//   T2[c][k][z][qy][qx]   (lines strided by 2Nx*2Ny; adjacent qx coalesce).
+// Dense kernel layout: lam[((qy * 2Nx + qx) * P + p) * 2Nz + qz].
+#pragma once
+
+#include "apply3d.hpp"
+#include "apply2d_b.hpp"
+#include "fft.hpp"
+
+namespace msr {
+
+struct Apply3DArgs {
+    int nx, ny, nz, K, C;
+    const float2* cmaps;  // (C, nz, ny, nx) or nullptr
+    const void* lam;
+    int lam_real;
+    float2* T1;           // (C, K, nz, ny, 2nx)
+    float2* T2;           // (C, K, nz, 2ny, 2nx)
+    float2* partial;      // (G, K, V)
+    int G, Gb;
+    const float2 *tw_x, *tw_y, *tw_z;
+};
+
+template <int NZ2, int K>
+struct PassC3Shape {
+    static constexpr int TP = FFTPlan<NZ2>::T;
+    static constexpr int WCT = (1024 / (K * TP)) > 1 ? 1024 / (K * TP) : 1;  // thread limit
+    static constexpr int WCL = (12288 / (K * NZ2)) > 1 ? 12288 / (K * NZ2) : 1;  // <= 96 KB LDS
+    static constexpr int WC0 = WCT < WCL ? WCT : WCL;
+    static constexpr int WC = WC0 < 8 ? WC0 : 8;     // adjacent qx lines per WG
+    static constexpr int BLK = WC * K * TP;
+    static constexpr int NV = WC * NZ2;
+    static constexpr int EV = (NV + BLK - 1) / BLK;
+    static constexpr int P = K * (K + 1) / 2;
+};
+
+template <int NZ2, int K, bool LREAL>
+__global__ void __launch_bounds__(1024)
+pass_c_3d(float2* __restrict__ T2, const void* __restrict__ lam_v, const float2* __restrict__ tw,
+          int nx2, int ny2, int nz, int C, int Gb, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    using S = PassC3Shape<NZ2, K>;
+    using LT = typename std::conditional<LREAL, float, float2>::type;
+    constexpr int TP = S::TP, WC = S::WC, BLK = S::BLK, NV = S::NV, EV = S::EV, P = S::P;
+    __shared__ float2 s_tw[NZ2];
+    __shared__ float2 s_buf[WC * K * NZ2];
+    const LT* lam = reinterpret_cast<const LT*>(lam_v);
+    const int tid = threadIdx.x;
+    // XCD-aware order: workgroups dispatched to one XCD (every 8th) take
+    // neighbouring column blocks, so partially used cache lines of the strided
+    // z lines are shared in that XCD's L2
+    const int nbx = gridDim.x, nxy = gridDim.x * gridDim.y;
+    const int lin = blockIdx.y * nbx + blockIdx.x;
+    const int per = (nxy + 7) / 8;
+    int logical = (lin % 8) * per + lin / 8;
+    if ((nxy % 8) != 0) logical = lin;  // keep the plain order when it does not divide
+    const int qx0 = (logical % nbx) * WC;
+    const int qy = logical / nbx;
+    const int grp = blockIdx.z;
+    const int c0 = (int)((int64_t)C * grp / Gb), c1 = (int)((int64_t)C * (grp + 1) / Gb);
+    const int64_t plane = (int64_t)nx2 * ny2;  // stride between z lines
+    load_twiddles<NZ2>(s_tw, tw);
+
+    LT lr[EV][P];
+#pragma unroll
+    for (int e = 0; e < EV; ++e) {
+        const int v = tid + e * BLK;
+        const int wc = v / NZ2, qz = v - wc * NZ2;
+        const int qx = qx0 + wc;
+        const bool ok = (v < NV) && (qx < nx2);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            if (ok) lr[e][p] = lam[(((int64_t)qy * nx2 + qx) * P + p) * NZ2 + qz];
+            else if constexpr (LREAL) lr[e][p] = 0.f;
+            else lr[e][p] = make_float2(0.f, 0.f);
+        }
+    }
+
+    const int g = tid / TP, t = tid - g * TP;
+    for (int c = c0; c < c1; ++c) {
+        // element (line = k*WC + wc, z): lanes run over wc fastest for coalescing
+        for (int el = tid; el < WC * K * NZ2; el += BLK) {
+            const int wc = el % WC;
+            const int r = el / WC;
+            const int k = r % K, z = r / K;
+            const int qx = qx0 + wc;
+            float2 v = make_float2(0.f, 0.f);
+            if (z < nz && qx < nx2)
+                v = T2[(((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx];
+            s_buf[(wc * K + k) * NZ2 + z] = v;
+        }
+        __syncthreads();
+        fft_lds<NZ2, -1>(s_buf + g * NZ2, s_tw, t);
+#pragma unroll
+        for (int e = 0; e < EV; ++e) {
+            const int v = tid + e * BLK;
+            if (v < NV) {
+                const int wc = v / NZ2, qz = v - wc * NZ2;
+                float2* col = s_buf + (size_t)wc * K * NZ2 + qz;
+                float2 in[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) in[k] = col[k * NZ2];
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        if constexpr (LREAL) {
+                            const float l = lr[e][j <= k ? packed_index(j, k) : packed_index(k, j)];
+                            acc.x = fmaf(l, in[k].x, acc.x);
+                            acc.y = fmaf(l, in[k].y, acc.y);
+                        } else {
+                            if (j <= k) cfma(acc, lr[e][packed_index(j, k)], in[k]);
+                            else cfmac(acc, lr[e][packed_index(k, j)], in[k]);
+                        }
+                    }
+                    col[j * NZ2] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        fft_lds<NZ2, +1>(s_buf + g * NZ2, s_tw, t);
+        for (int el = tid; el < WC * K * NZ2; el += BLK) {
+            const int wc = el % WC;
+            const int r = el / WC;
+            const int k = r % K, z = r / K;
+            const int qx = qx0 + wc;
+            if (z < nz && qx < nx2)
+                T2[(((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx] =
+                    s_buf[(wc * K + k) * NZ2 + z];
+        }
+        __syncthreads();
+    }
+}
+
+template <int NZ2, int K, bool LREAL>
+void launch_pass_c_3d_t(const Apply3DArgs& a, hipStream_t st, const int* skip) {
+    using S = PassC3Shape<NZ2, K>;
+    const int nx2 = 2 * a.nx, ny2 = 2 * a.ny;
+    dim3 grid((nx2 + S::WC - 1) / S::WC, ny2, a.Gb);
+    hipLaunchKernelGGL((pass_c_3d<NZ2, K, LREAL>), grid, dim3(S::BLK), 0, st, a.T2, a.lam, a.tw_z,
+                       nx2, ny2, a.nz, a.C, a.Gb, skip);
+}
+
+#define MSR_DECL_PASSC3(KK) void launch_pass_c_3d_k##KK(const Apply3DArgs& a, hipStream_t st, const int* skip);
+MSR_DECL_PASSC3(1) MSR_DECL_PASSC3(2) MSR_DECL_PASSC3(3) MSR_DECL_PASSC3(4)
+MSR_DECL_PASSC3(5) MSR_DECL_PASSC3(6) MSR_DECL_PASSC3(7) MSR_DECL_PASSC3(8)
+#undef MSR_DECL_PASSC3
+
+#define MSR_DEFINE_PASSC3(KK)                                                                  \
+    void launch_pass_c_3d_k##KK(const Apply3DArgs& a, hipStream_t st, const int* skip) {       \
+        const int nz2 = 2 * a.nz;                                                              \
+        bool ok = dispatch_fft_len(nz2, [&]<int N>() {                                         \
+            if constexpr (N <= 512) {                                                          \
+                if (a.lam_real) launch_pass_c_3d_t<N, KK, true>(a, st, skip);                  \
+                else launch_pass_c_3d_t<N, KK, false>(a, st, skip);                            \
+            } else {                                                                           \
+                fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Nz = %d above 512", N);             \
+            }                                                                                  \
+        });                                                                                    \
+        MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nz = %d", nz2);           \
+    }
+
+}  // namespace msr

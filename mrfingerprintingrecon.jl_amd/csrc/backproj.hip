@@ -77,11 +77,16 @@ void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
 
     // workspace: spread grid G (nch*Gos), then the line-pass buffer H
     // (nch*Gos/2) in the space the spreading tiles used before the gather
-    DevBuf work;
+    // kept on the trajectory between calls (calls on one trajectory are
+    // stream-ordered; a handle is not re-entrant)
+    DevBuf& work = traj->bp_work;
     const size_t gbytes = sizeof(float2) * (size_t)nch * Gos;
     const size_t hbytes = std::max(sizeof(float2) * (size_t)nch * (Gos / 2),
                                    spread_workspace_bytes(plan, nch));
-    work.alloc(gbytes + hbytes);
+    if (work.bytes < gbytes + hbytes) {
+        MSR_HIP(hipStreamSynchronize(st));  // earlier calls may still use the old buffer
+        work.alloc(gbytes + hbytes);
+    }
     float2* G = work.as<float2>();
     float2* H = reinterpret_cast<float2*>(reinterpret_cast<char*>(work.p) + gbytes);
     spread(traj, plan, src, nch, G, H, st);
@@ -174,8 +179,6 @@ void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
             MSR_CHECK_LAUNCH();
         }
     }
-    // the workspace is freed on return; keep the stream ordered before that
-    MSR_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace msr

@@ -19,6 +19,7 @@ constexpr int rows_block() {
 
 template <int NF, int DIR>
 __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const float2* __restrict__ tw) {
+    if (p.skip && *p.skip) return;
     constexpr int TP = FFTPlan<NF>::T;
     constexpr int BLK = rows_block<NF>();
     constexpr int F = BLK / TP;
@@ -54,6 +55,7 @@ constexpr int cols_wc() {
 
 template <int NF, int DIR>
 __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const float2* __restrict__ tw) {
+    if (p.skip && *p.skip) return;
     constexpr int TP = FFTPlan<NF>::T;
     constexpr int WC = cols_wc<NF>();
     constexpr int BLK = (WC * TP) < 256 ? WC * TP : 256;

@@ -37,6 +37,8 @@ struct LinePass {
     // lam[((qy*nx2 + qx)*P + p)*nz2 + qz], mask[(qz*ny2 + qy)*nx2 + qx]
     int lam3d = 0;
     int64_t ny2 = 0;
+    // device flag: the pass does nothing when *skip != 0 (early-stopped CG)
+    const int* skip = nullptr;
 };
 
 void run_rows(const LinePass& p, msr_ctx* ctx, hipStream_t st);

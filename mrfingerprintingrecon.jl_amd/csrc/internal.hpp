@@ -34,6 +34,7 @@ struct DevBuf {
     }
     void alloc(size_t n) {
         if (n == bytes && p) return;
+        if (p) MSR_HIP(hipDeviceSynchronize());  // in-flight kernels may still use p
         release();
         if (n == 0) return;
         MSR_HIP(hipMalloc(&p, n));
@@ -76,6 +77,7 @@ struct msr_traj {
     msr::DevBuf sel;                  // (M) int64 index of the masked sample in nsamp*nt, or empty if unmasked
     std::vector<int64_t> nsamp_t;     // host copy of the per-frame counts
     mutable std::map<std::array<int64_t, 3>, std::unique_ptr<msr::SpreadPlan>> plans;
+    mutable msr::DevBuf bp_work;      // backprojection workspace (grow-only)
 };
 
 struct msr_normalop {

@@ -188,6 +188,33 @@ def apply_normal(img_shape, Lp, kmask, cmaps, b, alpha=1.0, beta=0.0, x=None):
     return out
 
 
+class NormalOp:
+    """apply_normal with the dense kernel built once (for repeated applies)."""
+
+    def __init__(self, img_shape, Lp, kmask, cmaps):
+        self.N = tuple(int(v) for v in img_shape)
+        self.n = tuple(2 * v for v in self.N)
+        self.K = int((np.sqrt(8 * np.asarray(Lp).shape[1] + 1) - 1) // 2)
+        self.dense = dense_kernel(self.N, Lp, kmask, self.K)
+        self.maps = [None] if cmaps is None else list(np.asarray(cmaps, dtype=np.complex128))
+
+    def __call__(self, v):
+        K = self.K
+        d = len(self.N)
+        b = np.asarray(v, dtype=np.complex128).reshape((K,) + self.N[::-1])
+        out = np.zeros_like(b)
+        crop = tuple(slice(0, w) for w in self.N[::-1])
+        axes = tuple(range(1, d + 1))
+        for cm in self.maps:
+            kl1 = np.zeros((K,) + self.n[::-1], dtype=np.complex128)
+            kl1[(slice(None),) + crop] = b if cm is None else cm[None] * b
+            kl1 = sfft.fftn(kl1, axes=axes, workers=-1)
+            kl2 = np.einsum("kj...,k...->j...", self.dense, kl1)
+            img = sfft.ifftn(kl2, axes=axes, workers=-1)[(slice(None),) + crop]
+            out += img if cm is None else np.conj(cm)[None] * img
+        return out.reshape(-1)
+
+
 # --------------------------------------------------------------------------
 # Backprojection (src/BackProjection.jl:23-102,169-181; ext/.../BackProjection.jl)
 # --------------------------------------------------------------------------

@@ -317,3 +317,20 @@ def test_recon_recovers_truth_like_reference_test():
     xc = np.fft.ifft2(np.fft.ifftshift(xc, axes=(1, 2)))
     # coefficient 1 carries most energy; compare like the reference (rtol 1e-1)
     assert rel_err(xr, xc) < 0.2
+
+
+def test_cpu_port_matches_oracle():
+    """The FP32 CPU-baseline port computes the same operator and
+    backprojection as the FP64 oracle (to FP32 accuracy)."""
+    from oracle import cpu_port
+    N = (16, 16)
+    trj, U, cm, x = small_case(N=N, K=3, C=3, nt=6, nspk=3)
+    Lp, km = toeplitz.toeplitz_kernel(N, trj, U)
+    op = cpu_port.CpuNormalOp(N, Lp, km, cm, workers=2)
+    got = op.apply(x.astype(np.complex64)).reshape(x.shape)
+    ref = toeplitz.apply_normal(N, Lp, km, cm.astype(np.complex128), x)
+    assert rel_err(got, ref) < 1e-5
+    rng = np.random.default_rng(3)
+    data = (rng.normal(size=(3,) + trj.shape[:2]) + 1j * rng.normal(size=(3,) + trj.shape[:2])).astype(np.complex64)
+    bp = cpu_port.backprojection_port(data, trj, cm, U, workers=2)
+    assert rel_err(bp, toeplitz.backprojection(data, trj, cm, U=U)) < 1e-5
