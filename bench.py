@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end FP64 pipeline check")
+    ap.add_argument("--apply-only", type=int, default=0, metavar="N",
+                    help="profiling mode: build the operator, run N fused applies, N per-coil "
+                         "applies and a 256 MiB calibration copy, then exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py measures one GPU: this build has no multi-GPU path "
@@ -130,6 +133,22 @@ def main():
     torch.cuda.synchronize()
     build_ms = (time.perf_counter() - t0) * 1e3
     nvec = K * N_IMG[0] * N_IMG[1]
+    if args.apply_only:
+        xin = torch.randn(nvec, dtype=torch.complex64, device=dev)
+        y = torch.empty_like(xin)
+        for _ in range(args.apply_only):
+            A.A.mul_(y, xin)
+        for _ in range(args.apply_only):
+            A.A.mul_(y, xin, percoil=True)
+        src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device=dev)  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"apply_only": args.apply_only, "copy_bytes": 2 * src.numel() * 4,
+                          "apply_bytes": apply_bytes(A.A.nkm, A.A.lambda_real),
+                          "nkm": A.A.nkm}), flush=True)
+        return
     b = torch.empty((K,) + N_IMG[::-1], dtype=torch.complex64, device=dev)
     x = torch.empty(nvec, dtype=torch.complex64, device=dev)
     bflat = b.reshape(-1)

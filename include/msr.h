@@ -147,6 +147,19 @@ int msr_backprojection(msr_ctx *ctx, const msr_traj *traj, int d, const int64_t 
                        const void *const *cmaps, int dcf_mode, const float *dcf, void *out,
                        void *stream);
 
+/* --------------------------------------------------------- forward model */
+/* The tests' data simulation (test/reconstruct_radial.jl:62-74: per coil and
+ * frame, NonuniformFFTs exec_type2! of S_c * sum_k U[t,k] x_k), i.e. the
+ * adjoint of msr_backprojection without density compensation:
+ *   y_{j,c} = sum_k U[t_j,k] sum_n S_c[n] x_k[n] exp(-2 pi i k_j.(n - N/2)).
+ * x: (N..., K) ComplexF32 device; U: (Nt, K) or NULL for I(Nt); cmaps: ncoils
+ * device pointers or NULL (one coil, S = 1).  out: (nsamp, nt, C) ComplexF32;
+ * samples removed by the trajectory's mask are written as 0.  Synchronises
+ * the stream (its workspace is released on return). */
+int msr_forward(msr_ctx *ctx, const msr_traj *traj, int d, const int64_t *img_shape,
+                const void *x, int K, const void *U, int u_real, int ncoils,
+                const void *const *cmaps, void *out, void *stream);
+
 /* ------------------------------------------------------------------- CG */
 /* cg!(x, A, b; maxiter, reltol, abstol) of IterativeSolvers.jl 0.9 (caller of
  * the apply: test/reconstruct_radial_gpu.jl:99, src/CoilMaps.jl:136) run on

@@ -26,8 +26,8 @@ import math
 from . import _lib, synth  # noqa: F401
 from ._lib import ArgumentError, ErrorException  # noqa: F401
 
-__all__ = ["NFFTNormalOp", "calculate_backprojection", "cg", "cg_", "mul_", "LinearOperator",
-           "plan_trajectory", "ArgumentError", "ErrorException", "synth"]
+__all__ = ["NFFTNormalOp", "calculate_backprojection", "forward", "cg", "cg_", "mul_",
+           "LinearOperator", "plan_trajectory", "ArgumentError", "ErrorException", "synth"]
 
 _ctxs = {}
 
@@ -355,6 +355,37 @@ def calculate_backprojection(data, trj, cmaps_or_img_shape, U=None, sample_mask=
     _lib.call("msr_backprojection", _ctx(data.device.index), traj.h, len(img_shape),
               _i64arr(img_shape), _ptr(data), C, K, uptr, real, carr, mode, dcf_ptr, _ptr(out),
               _stream_ptr(data.device))
+    return out
+
+
+def forward(x, trj, cmaps=None, U=None, sample_mask=None, out=None):
+    """Forward subspace model y = A x (the reference tests' data simulation,
+    test/reconstruct_radial.jl:62-74): x (K, *N[::-1]) complex64 -> data
+    (C, Nt, Nsamp) complex64; samples removed by sample_mask are 0."""
+    torch = _torch()
+    x = _dev_tensor(x, "x", torch.complex64)
+    traj = trj if isinstance(trj, _Traj) else _Traj(trj, sample_mask)
+    cl = [_dev_tensor(c, "cmaps", torch.complex64) for c in _cmaps_list(cmaps)]
+    img_shape = tuple(int(s) for s in x.shape[1:][::-1])
+    if U is None:
+        K, uptr, real = traj.nt, None, 0
+    else:
+        U = _dev_tensor(U, "U")
+        if U.dim() == 1:
+            U = U.reshape(1, -1)
+        real = int(not U.is_complex())
+        U = U.to(torch.float32 if real else torch.complex64).contiguous()
+        K = U.shape[0]
+        uptr = _ptr(U)
+    if x.shape[0] != K:
+        raise ArgumentError(f"x has {x.shape[0]} coefficients, U {K}")
+    C = max(1, len(cl))
+    shape = (C, traj.nt, traj.nsamp)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    carr = (ctypes.c_void_p * len(cl))(*[c.data_ptr() for c in cl]) if cl else None
+    _lib.call("msr_forward", _ctx(x.device.index), traj.h, len(img_shape), _i64arr(img_shape),
+              _ptr(x), K, uptr, real, len(cl), carr, _ptr(out), _stream_ptr(x.device))
     return out
 
 

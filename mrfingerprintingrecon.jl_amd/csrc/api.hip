@@ -31,6 +31,11 @@ void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
                     int ncoils, int K, const void* U, int u_real, const void* const* cmaps,
                     int dcf_mode, const float* dcf, void* out, hipStream_t st);
 
+// forward.hip
+void forward_model(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, const void* x,
+                   int K, const void* U, int u_real, int ncoils, const void* const* cmaps, void* out,
+                   hipStream_t st);
+
 static thread_local std::string g_err;
 void set_last_error(const std::string& s) { g_err = s; }
 }  // namespace msr
@@ -243,6 +248,21 @@ int msr_backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t*
         ensure_device(ctx);
         backprojection(ctx, traj, d, img_shape, data, ncoils, K, U, u_real, cmaps, dcf_mode, dcf,
                        out, (hipStream_t)stream);
+    });
+}
+
+int msr_forward(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* img_shape,
+                const void* x, int K, const void* U, int u_real, int ncoils,
+                const void* const* cmaps, void* out, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(traj);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(x);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        forward_model(ctx, traj, d, img_shape, x, K, U, u_real, cmaps ? ncoils : 0, cmaps, out,
+                      (hipStream_t)stream);
     });
 }
 

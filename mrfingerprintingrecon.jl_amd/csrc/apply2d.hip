@@ -23,7 +23,7 @@ pass_a_2d(const float2* __restrict__ b, const float2* __restrict__ cmaps,
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rowpass_block<NX2>();
     constexpr int F = BLK / TP;          // rows transformed concurrently
-    constexpr int LD = NX2 + 1;          // padded LDS row pitch
+    constexpr int LD = lds_line<NX2>() + 1;  // LDS row pitch (odd: transposed access)
     __shared__ float2 s_tw[NX2];
     __shared__ float2 s_rows[RT2 * LD];
     const int tid = threadIdx.x;
@@ -42,7 +42,7 @@ pass_a_2d(const float2* __restrict__ b, const float2* __restrict__ cmaps,
             v = bk[(size_t)y * nx + xx];
             if (cm) v = cmul(cm[(size_t)y * nx + xx], v);
         }
-        s_rows[yl * LD + xx] = v;
+        s_rows[yl * LD + lds_pad(xx)] = v;
     }
     __syncthreads();
     const int g = tid / TP, t = tid - g * TP;
@@ -53,7 +53,7 @@ pass_a_2d(const float2* __restrict__ b, const float2* __restrict__ cmaps,
     for (int e = tid; e < RT2 * NX2; e += BLK) {
         const int yl = e % RT2, qx = e / RT2;
         const int y = y0 + yl;
-        if (y < ny) Tk[(size_t)qx * ny + y] = s_rows[yl * LD + qx];
+        if (y < ny) Tk[(size_t)qx * ny + y] = s_rows[yl * LD + lds_pad(qx)];
     }
 }
 
@@ -67,7 +67,7 @@ pass_c_2d(const float2* __restrict__ T, const float2* __restrict__ cmaps,
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rowpass_block<NX2>();
     constexpr int F = BLK / TP;
-    constexpr int LD = NX2 + 1;
+    constexpr int LD = lds_line<NX2>() + 1;
     constexpr int NXC = NX2 / 2;                 // = nx
     constexpr int EA = (RT2 * NXC + BLK - 1) / BLK;
     __shared__ float2 s_tw[NX2];
@@ -87,7 +87,7 @@ pass_c_2d(const float2* __restrict__ T, const float2* __restrict__ cmaps,
         for (int e = tid; e < RT2 * NX2; e += BLK) {
             const int yl = e % RT2, qx = e / RT2;
             const int y = y0 + yl;
-            s_rows[yl * LD + qx] = (y < ny) ? Tk[(size_t)qx * ny + y] : make_float2(0.f, 0.f);
+            s_rows[yl * LD + lds_pad(qx)] = (y < ny) ? Tk[(size_t)qx * ny + y] : make_float2(0.f, 0.f);
         }
         __syncthreads();
 #pragma unroll
@@ -100,7 +100,7 @@ pass_c_2d(const float2* __restrict__ T, const float2* __restrict__ cmaps,
                 const int yl = e / NXC, xx = e - yl * NXC;
                 const int y = y0 + yl;
                 if (y < ny) {
-                    float2 v = s_rows[yl * LD + xx];
+                    float2 v = s_rows[yl * LD + lds_pad(xx)];
                     if (cm) cfmac(acc[i], cm[(size_t)y * nx + xx], v);
                     else acc[i] = cadd(acc[i], v);
                 }

@@ -31,8 +31,9 @@ pass_b_2d(float2* __restrict__ T, const void* __restrict__ lam_v, const float2* 
     using S = PassBShape<NY2, K>;
     using LT = typename std::conditional<LREAL, float, float2>::type;
     constexpr int TP = S::TP, WC = S::WC, BLK = S::BLK, NV = S::NV, EV = S::EV, P = S::P;
+    constexpr int LL = lds_line<NY2>();
     __shared__ float2 s_tw[NY2];
-    __shared__ float2 s_buf[WC * K * NY2];
+    __shared__ float2 s_buf[WC * K * LL];
     const LT* lam = reinterpret_cast<const LT*>(lam_v);
     const int tid = threadIdx.x;
     const int qx0 = blockIdx.x * WC;
@@ -65,20 +66,20 @@ pass_b_2d(float2* __restrict__ T, const void* __restrict__ lam_v, const float2* 
             const int qx = qx0 + wc;
             float2 v = make_float2(0.f, 0.f);
             if (i < ny && qx < nx2) v = T[(((size_t)c * K + k) * nx2 + qx) * ny + i];
-            s_buf[el] = v;
+            s_buf[line * LL + lds_pad(i)] = v;
         }
         __syncthreads();
-        fft_lds<NY2, -1>(s_buf + g * NY2, s_tw, t);
+        fft_lds<NY2, -1>(s_buf + g * LL, s_tw, t);
         // per-voxel Hermitian (or real symmetric) K x K multiply
 #pragma unroll
         for (int e = 0; e < EV; ++e) {
             const int v = tid + e * BLK;
             if (v < NV) {
                 const int wc = v / NY2, qy = v - wc * NY2;
-                float2* col = s_buf + (size_t)wc * K * NY2 + qy;
+                float2* col = s_buf + (size_t)wc * K * LL + lds_pad(qy);
                 float2 in[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) in[k] = col[k * NY2];
+                for (int k = 0; k < K; ++k) in[k] = col[k * LL];
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     float2 acc = make_float2(0.f, 0.f);
@@ -93,17 +94,18 @@ pass_b_2d(float2* __restrict__ T, const void* __restrict__ lam_v, const float2* 
                             else cfmac(acc, lr[e][packed_index(k, j)], in[k]);
                         }
                     }
-                    col[j * NY2] = acc;
+                    col[j * LL] = acc;
                 }
             }
         }
         __syncthreads();
-        fft_lds<NY2, +1>(s_buf + g * NY2, s_tw, t);
+        fft_lds<NY2, +1>(s_buf + g * LL, s_tw, t);
         for (int el = tid; el < WC * K * NY2; el += BLK) {
             const int line = el / NY2, i = el - line * NY2;
             const int wc = line / K, k = line - wc * K;
             const int qx = qx0 + wc;
-            if (i < ny && qx < nx2) T[(((size_t)c * K + k) * nx2 + qx) * ny + i] = s_buf[el];
+            if (i < ny && qx < nx2)
+                T[(((size_t)c * K + k) * nx2 + qx) * ny + i] = s_buf[line * LL + lds_pad(i)];
         }
         __syncthreads();
     }

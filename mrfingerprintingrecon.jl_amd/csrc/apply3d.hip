@@ -25,7 +25,7 @@ pass_a_3d(const float2* __restrict__ b, const float2* __restrict__ cmaps, float2
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rows3_block<NX2>();
     constexpr int F = BLK / TP;
-    constexpr int LD = NX2 + 1;
+    constexpr int LD = lds_line<NX2>() + 1;
     __shared__ float2 s_tw[NX2];
     __shared__ float2 s[F * LD];
     load_twiddles<NX2>(s_tw, tw);
@@ -42,7 +42,7 @@ pass_a_3d(const float2* __restrict__ b, const float2* __restrict__ cmaps, float2
             v = bk[row * nx + xx];
             if (cm) v = cmul(cm[row * nx + xx], v);
         }
-        s[l * LD + xx] = v;
+        s[l * LD + lds_pad(xx)] = v;
     }
     __syncthreads();
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
@@ -51,7 +51,7 @@ pass_a_3d(const float2* __restrict__ b, const float2* __restrict__ cmaps, float2
     for (int e = threadIdx.x; e < F * NX2; e += BLK) {
         const int l = e / NX2, q = e - l * NX2;
         const int64_t row = r0 + l;
-        if (row < nrows) out[row * NX2 + q] = s[l * LD + q];
+        if (row < nrows) out[row * NX2 + q] = s[l * LD + lds_pad(q)];
     }
 }
 
@@ -64,7 +64,7 @@ pass_e_3d(const float2* __restrict__ T1, const float2* __restrict__ cmaps,
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rows3_block<NX2>();
     constexpr int F = BLK / TP;
-    constexpr int LD = NX2 + 1;
+    constexpr int LD = lds_line<NX2>() + 1;
     constexpr int NXC = NX2 / 2;
     constexpr int EA = (F * NXC + BLK - 1) / BLK;
     __shared__ float2 s_tw[NX2];
@@ -83,7 +83,7 @@ pass_e_3d(const float2* __restrict__ T1, const float2* __restrict__ cmaps,
         for (int e = threadIdx.x; e < F * NX2; e += BLK) {
             const int l = e / NX2, q = e - l * NX2;
             const int64_t row = r0 + l;
-            s[l * LD + q] = row < nrows ? in[row * NX2 + q] : make_float2(0.f, 0.f);
+            s[l * LD + lds_pad(q)] = row < nrows ? in[row * NX2 + q] : make_float2(0.f, 0.f);
         }
         __syncthreads();
         fft_lds<NX2, +1>(s + g * LD, s_tw, t);
@@ -95,7 +95,7 @@ pass_e_3d(const float2* __restrict__ T1, const float2* __restrict__ cmaps,
                 const int l = e / NXC, xx = e - l * NXC;
                 const int64_t row = r0 + l;
                 if (row < nrows) {
-                    const float2 v = s[l * LD + xx];
+                    const float2 v = s[l * LD + lds_pad(xx)];
                     if (cm) cfmac(acc[i], cm[row * nx + xx], v);
                     else acc[i] = cadd(acc[i], v);
                 }

@@ -28,7 +28,7 @@ template <int NZ2, int K>
 struct PassC3Shape {
     static constexpr int TP = FFTPlan<NZ2>::T;
     static constexpr int WCT = (1024 / (K * TP)) > 1 ? 1024 / (K * TP) : 1;  // thread limit
-    static constexpr int WCL = (12288 / (K * NZ2)) > 1 ? 12288 / (K * NZ2) : 1;  // <= 96 KB LDS
+    static constexpr int WCL = (10922 / (K * NZ2)) > 1 ? 10922 / (K * NZ2) : 1;  // <= 96 KB LDS
     static constexpr int WC0 = WCT < WCL ? WCT : WCL;
     static constexpr int WC = WC0 < 8 ? WC0 : 8;     // adjacent qx lines per WG
     static constexpr int BLK = WC * K * TP;
@@ -45,8 +45,9 @@ pass_c_3d(float2* __restrict__ T2, const void* __restrict__ lam_v, const float2*
     using S = PassC3Shape<NZ2, K>;
     using LT = typename std::conditional<LREAL, float, float2>::type;
     constexpr int TP = S::TP, WC = S::WC, BLK = S::BLK, NV = S::NV, EV = S::EV, P = S::P;
+    constexpr int LL = lds_line<NZ2>();
     __shared__ float2 s_tw[NZ2];
-    __shared__ float2 s_buf[WC * K * NZ2];
+    __shared__ float2 s_buf[WC * K * LL];
     const LT* lam = reinterpret_cast<const LT*>(lam_v);
     const int tid = threadIdx.x;
     // XCD-aware order: workgroups dispatched to one XCD (every 8th) take
@@ -90,19 +91,19 @@ pass_c_3d(float2* __restrict__ T2, const void* __restrict__ lam_v, const float2*
             float2 v = make_float2(0.f, 0.f);
             if (z < nz && qx < nx2)
                 v = T2[(((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx];
-            s_buf[(wc * K + k) * NZ2 + z] = v;
+            s_buf[(wc * K + k) * LL + lds_pad(z)] = v;
         }
         __syncthreads();
-        fft_lds<NZ2, -1>(s_buf + g * NZ2, s_tw, t);
+        fft_lds<NZ2, -1>(s_buf + g * LL, s_tw, t);
 #pragma unroll
         for (int e = 0; e < EV; ++e) {
             const int v = tid + e * BLK;
             if (v < NV) {
                 const int wc = v / NZ2, qz = v - wc * NZ2;
-                float2* col = s_buf + (size_t)wc * K * NZ2 + qz;
+                float2* col = s_buf + (size_t)wc * K * LL + lds_pad(qz);
                 float2 in[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) in[k] = col[k * NZ2];
+                for (int k = 0; k < K; ++k) in[k] = col[k * LL];
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     float2 acc = make_float2(0.f, 0.f);
@@ -117,12 +118,12 @@ pass_c_3d(float2* __restrict__ T2, const void* __restrict__ lam_v, const float2*
                             else cfmac(acc, lr[e][packed_index(k, j)], in[k]);
                         }
                     }
-                    col[j * NZ2] = acc;
+                    col[j * LL] = acc;
                 }
             }
         }
         __syncthreads();
-        fft_lds<NZ2, +1>(s_buf + g * NZ2, s_tw, t);
+        fft_lds<NZ2, +1>(s_buf + g * LL, s_tw, t);
         for (int el = tid; el < WC * K * NZ2; el += BLK) {
             const int wc = el % WC;
             const int r = el / WC;
@@ -130,7 +131,7 @@ pass_c_3d(float2* __restrict__ T2, const void* __restrict__ lam_v, const float2*
             const int qx = qx0 + wc;
             if (z < nz && qx < nx2)
                 T2[(((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx] =
-                    s_buf[(wc * K + k) * NZ2 + z];
+                    s_buf[(wc * K + k) * LL + lds_pad(z)];
         }
         __syncthreads();
     }

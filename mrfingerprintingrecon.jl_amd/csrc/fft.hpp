@@ -159,6 +159,14 @@ template <int DIR> struct Butterfly<5, DIR> {
     }
 };
 
+// ----------------------------------------------------------- LDS layout
+// Element i of a line lives at lds_pad(i): one float2 of padding after every
+// 8 elements, so the stride-R stores of the first Stockham stages and the
+// strided reads of the transposing passes spread over the 64 LDS banks.
+__host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 3); }
+template <int N>
+constexpr int lds_line() { return N + N / 8; }
+
 // ----------------------------------------------------------- stages
 template <int N, int R, int NS, int T, int DIR>
 __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw, int t) {
@@ -171,7 +179,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw, int t) 
         const int j = t + p * T;
         if (EXACT || j < NB) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[p][r] = buf[j + r * NB];
+            for (int r = 0; r < R; ++r) v[p][r] = buf[lds_pad(j + r * NB)];
         }
     }
     __syncthreads();
@@ -191,7 +199,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw, int t) 
             Butterfly<R, DIR>::run(v[p]);
             const int base = (j - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[base + r * NS] = v[p][r];
+            for (int r = 0; r < R; ++r) buf[lds_pad(base + r * NS)] = v[p][r];
         }
     }
     __syncthreads();
@@ -206,8 +214,8 @@ __device__ __forceinline__ void fft_stages(float2* buf, const float2* tw, int t)
     }
 }
 
-// In-place N-point FFT of buf[0..N) by the T = FFTPlan<N>::T threads with
-// local index t.  Requires a barrier before the call (data in LDS); ends with
+// In-place N-point FFT of a line (element i at buf[lds_pad(i)]) by the
+// T = FFTPlan<N>::T threads with local index t.  Requires a barrier before the call (data in LDS); ends with
 // a barrier.  All threads of the workgroup must call it.
 template <int N, int DIR>
 __device__ __forceinline__ void fft_lds(float2* buf, const float2* tw, int t) {

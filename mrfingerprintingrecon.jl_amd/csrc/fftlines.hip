@@ -23,17 +23,31 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const float2* __r
     constexpr int TP = FFTPlan<NF>::T;
     constexpr int BLK = rows_block<NF>();
     constexpr int F = BLK / TP;
-    constexpr int LD = NF + 1;
+    constexpr int LD = lds_line<NF>() + 1;
     __shared__ float2 s_tw[NF];
     __shared__ float2 s[F * LD];
     load_twiddles<NF>(s_tw, tw);
     const int64_t l0 = (int64_t)blockIdx.x * F;
-    for (int e = threadIdx.x; e < F * NF; e += BLK) {
-        const int l = e / NF, i = e - l * NF;
-        const int64_t line = l0 + l;
-        float2 v = make_float2(0.f, 0.f);
-        if (line < p.nlines && i < p.lin) v = p.in[line * p.in_pitch + i];
-        s[l * LD + i] = v;
+    if (p.isel == SEL_FIRST && !p.win) {
+        for (int e = threadIdx.x; e < F * NF; e += BLK) {
+            const int l = e / NF, i = e - l * NF;
+            const int64_t line = l0 + l;
+            float2 v = make_float2(0.f, 0.f);
+            if (line < p.nlines && i < p.lin) v = p.in[line * p.in_pitch + i];
+            s[l * LD + lds_pad(i)] = v;
+        }
+    } else {
+        for (int e = threadIdx.x; e < F * NF; e += BLK) s[(e / NF) * LD + lds_pad(e % NF)] = make_float2(0.f, 0.f);
+        __syncthreads();
+        for (int e = threadIdx.x; e < F * p.lin; e += BLK) {
+            const int l = e / p.lin, i = e - l * p.lin;
+            const int64_t line = l0 + l;
+            if (line < p.nlines) {
+                float2 v = p.in[line * p.in_pitch + i];
+                if (p.win) v = cscale(v, p.win[i]);
+                s[l * LD + lds_pad(select_bin(i, p.lin, NF, p.isel))] = v;
+            }
+        }
     }
     __syncthreads();
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
@@ -42,7 +56,7 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const float2* __r
         const int l = e / p.lout, j = e - l * p.lout;
         const int64_t line = l0 + l;
         if (line >= p.nlines) continue;
-        float2 v = s[l * LD + select_bin(j, p.lout, NF, p.sel)];
+        float2 v = s[l * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
         if (p.w) v = cscale(v, p.w[j]);
         p.out[line * p.out_pitch + j] = v;
     }
@@ -60,18 +74,32 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const float2* __r
     constexpr int WC = cols_wc<NF>();
     constexpr int BLK = (WC * TP) < 256 ? WC * TP : 256;
     constexpr int F = BLK / TP;
-    constexpr int LD = NF + 1;
+    constexpr int LD = lds_line<NF>() + 1;
     __shared__ float2 s_tw[NF];
     __shared__ float2 s[WC * LD];
     load_twiddles<NF>(s_tw, tw);
     const int64_t col0 = (int64_t)blockIdx.x * WC;
     const int64_t b = blockIdx.y;
-    for (int e = threadIdx.x; e < WC * NF; e += BLK) {
-        const int wc = e % WC, i = e / WC;
-        const int64_t col = col0 + wc;
-        float2 v = make_float2(0.f, 0.f);
-        if (col < p.ncols && i < p.lin) v = p.in[b * p.in_bstride + i * p.in_stride + col];
-        s[wc * LD + i] = v;
+    if (p.isel == SEL_FIRST && !p.win) {
+        for (int e = threadIdx.x; e < WC * NF; e += BLK) {
+            const int wc = e % WC, i = e / WC;
+            const int64_t col = col0 + wc;
+            float2 v = make_float2(0.f, 0.f);
+            if (col < p.ncols && i < p.lin) v = p.in[b * p.in_bstride + i * p.in_stride + col];
+            s[wc * LD + lds_pad(i)] = v;
+        }
+    } else {
+        for (int e = threadIdx.x; e < WC * NF; e += BLK) s[(e % WC) * LD + lds_pad(e / WC)] = make_float2(0.f, 0.f);
+        __syncthreads();
+        for (int e = threadIdx.x; e < WC * p.lin; e += BLK) {
+            const int wc = e % WC, i = e / WC;
+            const int64_t col = col0 + wc;
+            if (col < p.ncols) {
+                float2 v = p.in[b * p.in_bstride + i * p.in_stride + col];
+                if (p.win) v = cscale(v, p.win[i]);
+                s[wc * LD + lds_pad(select_bin(i, p.lin, NF, p.isel))] = v;
+            }
+        }
     }
     __syncthreads();
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
@@ -81,7 +109,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const float2* __r
         const int wc = e % WC, j = e / WC;
         const int64_t col = col0 + wc;
         if (col >= p.ncols) continue;
-        float2 v = s[wc * LD + select_bin(j, p.lout, NF, p.sel)];
+        float2 v = s[wc * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
         if (p.w) v = cscale(v, p.w[j]);
         if (!p.lam) {
             p.out[b * p.out_bstride + j * p.out_stride + col] = v;

@@ -6,7 +6,8 @@
 // Each line has Lin valid inputs (zero-padded to the FFT length NF) and Lout
 // selected outputs: SEL_FIRST bins 0..Lout-1, SEL_CENTER frequency
 // r = j - Lout/2 (fftshift order), SEL_FFTORDER r = j or j - Lout (FFT order),
-// each scaled by an optional real weight w[j] (deapodisation).
+// each scaled by an optional real weight w[j] (deapodisation).  Inputs can be
+// placed the same way (isel / win) for the forward (type-2) direction.
 #pragma once
 
 #include "internal.hpp"
@@ -22,6 +23,10 @@ struct LinePass {
     int dir = -1;        // -1 forward, +1 inverse
     int lin = 0, lout = 0, sel = SEL_FIRST;
     const float* w = nullptr;
+    // input placement: input i goes to bin select(i, lin) scaled by win[i]
+    // (SEL_FIRST: bins 0..lin-1, the rest zero; SEL_CENTER: frequency i - lin/2)
+    int isel = SEL_FIRST;
+    const float* win = nullptr;
     // rows: line l at in + l*in_pitch (contiguous), out + l*out_pitch
     int64_t nlines = 0, in_pitch = 0, out_pitch = 0;
     // cols: lines (b, col) with col < ncols adjacent in memory

@@ -3,6 +3,7 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -14,6 +15,7 @@ struct msr_ctx {
     int device = 0;
     std::map<int, float2*> twiddles;  // N -> device table exp(-2 pi i m/N)
     std::map<std::tuple<int, int, int>, float*> deapod;  // (n_os, n_out, sel) -> 1/phihat
+    std::mutex lock;  // guards the two lazily filled tables (calls may come from several threads)
     ~msr_ctx();
 };
 

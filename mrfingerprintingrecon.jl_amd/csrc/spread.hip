@@ -425,6 +425,7 @@ __global__ void deapod_kernel(int n_os, int n_out, int sel, float* __restrict__ 
 }
 
 const float* deapod_table(msr_ctx* ctx, int n_os, int n_out, int sel) {
+    std::lock_guard<std::mutex> guard(ctx->lock);
     auto key = std::make_tuple(n_os, n_out, sel);
     auto it = ctx->deapod.find(key);
     if (it != ctx->deapod.end()) return it->second;

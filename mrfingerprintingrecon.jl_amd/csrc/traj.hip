@@ -19,6 +19,7 @@ namespace msr {
 void ensure_device(const msr_ctx* ctx) { MSR_HIP(hipSetDevice(ctx->device)); }
 
 const float2* twiddle_table(msr_ctx* ctx, int n) {
+    std::lock_guard<std::mutex> guard(ctx->lock);
     auto it = ctx->twiddles.find(n);
     if (it != ctx->twiddles.end()) return it->second;
     std::vector<float2> h(n);

@@ -254,3 +254,30 @@ def test_backprojection_and_kernel_are_bit_reproducible(torch, msr):
     l1 = msr.NFFTNormalOp(N, dev(torch, trj), dev(torch, U), cmaps=cml).prod.A.Λ.cpu().numpy()
     l2 = msr.NFFTNormalOp(N, dev(torch, trj), dev(torch, U), cmaps=cml).prod.A.Λ.cpu().numpy()
     assert np.array_equal(l1, l2)
+
+
+@pytest.mark.parametrize("masked", [False, True])
+def test_forward_model_matches_oracle_and_is_adjoint(torch, msr, masked):
+    """Forward (type-2) model vs the FP64 oracle, and <A x, y> = <x, A^H y> with
+    the device backprojection as A^H."""
+    N = (32, 24)
+    trj, U, cm, x = radial_case(N, 3, 3, 8, 3)
+    mask = None
+    if masked:
+        mask = np.ones(trj.shape[:2], dtype=bool)
+        mask[2, 10:50] = False
+    md = None if mask is None else dev(torch, mask)
+    cml = [dev(torch, c) for c in cm]
+    y = msr.forward(dev(torch, x), dev(torch, trj), cml, U=dev(torch, U), sample_mask=md).cpu().numpy()
+    ref = toeplitz.forward(x.astype(np.complex128), trj, U, cm.astype(np.complex128), sample_mask=mask)
+    print(f"forward rel err {rel_err(y, ref):.3e}", flush=True)
+    assert rel_err(y, ref) < 1e-5
+    rng = np.random.default_rng(9)
+    d = (rng.normal(size=y.shape) + 1j * rng.normal(size=y.shape)).astype(np.complex64)
+    if mask is not None:
+        d[:, ~mask] = 0
+    b = msr.calculate_backprojection(dev(torch, d), dev(torch, trj), cml, U=dev(torch, U),
+                                     sample_mask=md).cpu().numpy()
+    lhs = np.vdot(y.astype(np.complex128), d.astype(np.complex128))
+    rhs = np.vdot(x.astype(np.complex128), b.astype(np.complex128))
+    assert abs(lhs - rhs) < 1e-5 * abs(lhs)
