@@ -59,6 +59,7 @@ extern "C" {
 typedef struct msr_ctx msr_ctx;
 typedef struct msr_traj msr_traj;
 typedef struct msr_normalop msr_normalop;
+typedef struct msr_batch msr_batch;
 
 /* ------------------------------------------------------------------ misc */
 /* Copies the calling thread's last error message into buf (NUL-terminated,
@@ -173,6 +174,42 @@ int msr_forward(msr_ctx *ctx, const msr_traj *traj, int d, const int64_t *img_sh
 int msr_cg(msr_normalop *op, void *x, const void *b, int maxiter, double reltol, double abstol,
            int initially_zero, int mode, int vec_fp64, int *iters_out, double *residual_out,
            void *stream);
+
+/* -------------------------------------------------------------- batches */
+/* Many reconstructions (uneven image size, coil count and K, 2D and 3D)
+ * applied and solved together: each pass of the fused apply is one launch per
+ * group of operators that share its FFT length (and K for the kernel
+ * multiply), and the CG vector steps are one launch for the whole batch.
+ * Every operator's result is the same as from its own msr_normalop_apply /
+ * msr_cg.  No counterpart in the reference (which applies one operator per
+ * call); it is the batched form of msr_normalop_apply and msr_cg.
+ * The batch borrows the operators: they must outlive it, must be distinct,
+ * and must not be used by other calls while a batch call runs on them (each
+ * operator owns one scratch and one CG state).  Not re-entrant. */
+int msr_batch_create(msr_ctx *ctx, int nops, msr_normalop *const *ops, msr_batch **out);
+int msr_batch_destroy(msr_batch *batch);
+/* x[j] = beta x[j] + alpha A_j b[j] for every operator (x, b: nops device pointers). */
+int msr_batch_apply(msr_batch *batch, void *const *x, const void *const *b, double alpha_re,
+                    double alpha_im, double beta_re, double beta_im, void *stream);
+/* msr_cg for every operator in lockstep; an operator that meets its tolerance
+ * stops (its blocks are skipped) while the others go on.  mode:
+ * MSR_CG_DEVICE or MSR_CG_GRAPH (the whole solve captured once per pointer
+ * set).  iters_out / residual_out: nops entries each, or NULL. */
+int msr_batch_cg(msr_batch *batch, void *const *x, const void *const *b, int maxiter,
+                 double reltol, double abstol, int initially_zero, int mode, int vec_fp64,
+                 int *iters_out, double *residual_out, void *stream);
+/* The same solve in steps, for schedulers that admit and retire jobs between
+ * iterations (continuous batching): start records each operator's CG state on
+ * the operator, so a later batch built from any mix of started operators can
+ * iterate and finish them.  finish writes the ComplexF32 solutions. */
+int msr_batch_cg_start(msr_batch *batch, void *const *x, const void *const *b, int maxiter,
+                       double reltol, double abstol, int initially_zero, int vec_fp64,
+                       void *stream);
+int msr_batch_cg_iterate(msr_batch *batch, int niter, void *stream);
+int msr_batch_cg_finish(msr_batch *batch, void *stream);
+/* Per-operator iteration count, residual norm and done flag (nops entries
+ * each, any may be NULL); synchronises the stream. */
+int msr_batch_cg_status(msr_batch *batch, int *iters, double *residual, int *done, void *stream);
 
 #ifdef __cplusplus
 }

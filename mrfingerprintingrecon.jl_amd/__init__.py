@@ -438,3 +438,108 @@ def cg(A, b, **kw):
     b = b.contiguous()
     x = torch.empty_like(b)
     return cg_(x, A, b, initially_zero=True, **kw)
+
+
+def _op_state(A):
+    return A.A if isinstance(A, LinearOperator) else A
+
+
+class NFFTNormalOpBatch:
+    """Many normal operators (any mix of image size, coil count, K, 2D/3D)
+    applied and solved together (msr_batch_*): every pass of the fused apply is
+    one launch per group of operators sharing its FFT length, and the CG
+    vector steps are one launch for the whole batch.  Each operator's result
+    is the one its own mul!/cg! gives.  The batch borrows the operators (they
+    must stay alive and must not be used elsewhere during a batch call)."""
+
+    def __init__(self, ops):
+        ops = [_op_state(A) for A in ops]
+        if not ops:
+            raise ArgumentError("a batch needs at least one operator")
+        dev = ops[0].device
+        if any(A.device != dev for A in ops):
+            raise ArgumentError("the operators of a batch must live on one device")
+        self.ops = ops
+        self.device = dev
+        arr = (ctypes.c_void_p * len(ops))(*[A.h.value for A in ops])
+        h = ctypes.c_void_p()
+        _lib.call("msr_batch_create", _ctx(dev.index), len(ops), arr, ctypes.byref(h))
+        self.h = h
+
+    def __len__(self):
+        return len(self.ops)
+
+    def _ptrs(self, ts, name):
+        if len(ts) != len(self.ops):
+            raise ArgumentError(f"{name} has {len(ts)} vectors for {len(self.ops)} operators")
+        for t, A in zip(ts, self.ops):
+            n = math.prod(A.shape) * A.Ncoeff
+            if t.numel() != n or not t.is_contiguous() or t.device != self.device:
+                raise ArgumentError(f"{name}: expected contiguous device vectors of length {n}")
+        return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+    def mul_(self, xs, bs, alpha=1.0, beta=0.0):
+        """x_j = β x_j + α A_j b_j for every operator."""
+        a, bt = complex(alpha), complex(beta)
+        _lib.call("msr_batch_apply", self.h, self._ptrs(xs, "x"), self._ptrs(bs, "b"),
+                  a.real, a.imag, bt.real, bt.imag, _stream_ptr(self.device))
+        return xs
+
+    def cg_(self, xs, bs, maxiter, reltol=None, abstol=0.0, initially_zero=False,
+            mode="device", vec_fp64=True, log=False):
+        """cg!(x_j, A_j, b_j) for every operator in lockstep (modes "device"
+        and "graph"); see cg_ for the recurrence and the precision."""
+        reltol = REL_TOL_F32 if reltol is None else reltol
+        torch = _torch()
+        cur = torch.cuda.current_stream(self.device)
+        side = None
+        if mode == "graph" and cur.cuda_stream == 0:
+            side = _side_stream(self.device)
+            side.wait_stream(cur)
+        st = ctypes.c_void_p((side or cur).cuda_stream)
+        n = len(self.ops)
+        its = (ctypes.c_int * n)()
+        res = (ctypes.c_double * n)()
+        _lib.call("msr_batch_cg", self.h, self._ptrs(xs, "x"), self._ptrs(bs, "b"), int(maxiter),
+                  float(reltol), float(abstol), int(bool(initially_zero)), _lib.CG_MODES[mode],
+                  int(bool(vec_fp64)), its if log else None, res if log else None, st)
+        if side is not None:
+            cur.wait_stream(side)
+        if log:
+            return xs, {"iters": list(its), "residual": list(res)}
+        return xs
+
+    # stepwise solve (continuous batching)
+    def cg_start(self, xs, bs, maxiter, reltol=None, abstol=0.0, initially_zero=True,
+                 vec_fp64=True, stream=None):
+        reltol = REL_TOL_F32 if reltol is None else reltol
+        _lib.call("msr_batch_cg_start", self.h, self._ptrs(xs, "x"), self._ptrs(bs, "b"),
+                  int(maxiter), float(reltol), float(abstol), int(bool(initially_zero)),
+                  int(bool(vec_fp64)), self._st(stream))
+
+    def cg_iterate(self, niter, stream=None):
+        _lib.call("msr_batch_cg_iterate", self.h, int(niter), self._st(stream))
+
+    def cg_finish(self, stream=None):
+        _lib.call("msr_batch_cg_finish", self.h, self._st(stream))
+
+    def cg_status(self, stream=None):
+        n = len(self.ops)
+        its = (ctypes.c_int * n)()
+        res = (ctypes.c_double * n)()
+        done = (ctypes.c_int * n)()
+        _lib.call("msr_batch_cg_status", self.h, its, res, done, self._st(stream))
+        return list(its), list(res), [bool(d) for d in done]
+
+    def _st(self, stream):
+        if stream is None:
+            return _stream_ptr(self.device)
+        return ctypes.c_void_p(stream.cuda_stream)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib.load().msr_batch_destroy(self.h)
+            self.h = None
+
+
+__all__ += ["NFFTNormalOpBatch"]

@@ -66,6 +66,14 @@ SIGNATURES = {
     "msr_backprojection": (_I, [_P, _P, _I, _P, _P, _I, _I, _P, _I, _P, _I, _P, _P, _P]),
     "msr_forward": (_I, [_P, _P, _I, _P, _P, _I, _P, _I, _I, _P, _P, _P]),
     "msr_cg": (_I, [_P, _P, _P, _I, _D, _D, _I, _I, _I, _P, _P, _P]),
+    "msr_batch_create": (_I, [_P, _I, _P, _P]),
+    "msr_batch_destroy": (_I, [_P]),
+    "msr_batch_apply": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),
+    "msr_batch_cg": (_I, [_P, _P, _P, _I, _D, _D, _I, _I, _I, _P, _P, _P]),
+    "msr_batch_cg_start": (_I, [_P, _P, _P, _I, _D, _D, _I, _I, _P]),
+    "msr_batch_cg_iterate": (_I, [_P, _I, _P]),
+    "msr_batch_cg_finish": (_I, [_P, _P]),
+    "msr_batch_cg_status": (_I, [_P, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -3,23 +3,10 @@
 #include <cstring>
 #include <string>
 
-#include "internal.hpp"
+#include "batch.hpp"
+#include "normalop.hpp"
 
 namespace msr {
-// normalop.hip
-msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real);
-void op_set_cmaps(msr_normalop* op, int ncoils, const void* const* cmaps, hipStream_t st);
-void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, int64_t nkm,
-                    hipStream_t st);
-void op_alloc_scratch(msr_normalop* op);
-void op_apply(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
-              bool beta_zero, hipStream_t st);
-void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
-                      bool beta_zero, hipStream_t st);
-void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hipStream_t st);
-void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
-           double abstol, bool initially_zero, int mode, bool fp64, int* iters_out,
-           double* res_out, hipStream_t st);
 // traj.hip
 msr_traj* traj_create(msr_ctx* ctx, int d, int64_t nsamp, int64_t nt, const float* trj,
                       const uint8_t* mask, hipStream_t st);
@@ -278,6 +265,83 @@ int msr_cg(msr_normalop* op, void* x, const void* b, int maxiter, double reltol,
         ensure_device(op->ctx);
         op_cg(op, (float2*)x, (const float2*)b, maxiter, reltol, abstol, initially_zero != 0, mode,
               vec_fp64 != 0, iters_out, residual_out, (hipStream_t)stream);
+    });
+}
+
+int msr_batch_create(msr_ctx* ctx, int nops, msr_normalop* const* ops, msr_batch** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        *out = batch_create(ctx, nops, ops);
+    });
+}
+
+int msr_batch_destroy(msr_batch* batch) {
+    return guarded([&] {
+        if (!batch) return;
+        ensure_device(batch->ctx);
+        delete batch;
+    });
+}
+
+int msr_batch_apply(msr_batch* batch, void* const* x, const void* const* b, double alpha_re,
+                    double alpha_im, double beta_re, double beta_im, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(batch);
+        ensure_device(batch->ctx);
+        batch_apply(batch, (float2* const*)x, (const float2* const*)b,
+                    make_float2((float)alpha_re, (float)alpha_im),
+                    make_float2((float)beta_re, (float)beta_im), beta_re == 0.0 && beta_im == 0.0,
+                    (hipStream_t)stream);
+    });
+}
+
+int msr_batch_cg(msr_batch* batch, void* const* x, const void* const* b, int maxiter,
+                 double reltol, double abstol, int initially_zero, int mode, int vec_fp64,
+                 int* iters_out, double* residual_out, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(batch);
+        ensure_device(batch->ctx);
+        batch_cg(batch, (float2* const*)x, (const float2* const*)b, maxiter, reltol, abstol,
+                 initially_zero != 0, mode, vec_fp64 != 0, (hipStream_t)stream);
+        if (iters_out || residual_out)
+            batch_cg_status(batch, iters_out, residual_out, nullptr, (hipStream_t)stream);
+    });
+}
+
+int msr_batch_cg_start(msr_batch* batch, void* const* x, const void* const* b, int maxiter,
+                       double reltol, double abstol, int initially_zero, int vec_fp64,
+                       void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(batch);
+        ensure_device(batch->ctx);
+        batch_cg_start(batch, (float2* const*)x, (const float2* const*)b, maxiter, reltol, abstol,
+                       initially_zero != 0, vec_fp64 != 0, (hipStream_t)stream);
+    });
+}
+
+int msr_batch_cg_iterate(msr_batch* batch, int niter, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(batch);
+        ensure_device(batch->ctx);
+        batch_cg_iterate(batch, niter, (hipStream_t)stream);
+    });
+}
+
+int msr_batch_cg_finish(msr_batch* batch, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(batch);
+        ensure_device(batch->ctx);
+        batch_cg_finish(batch, (hipStream_t)stream);
+    });
+}
+
+int msr_batch_cg_status(msr_batch* batch, int* iters, double* residual, int* done, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(batch);
+        ensure_device(batch->ctx);
+        batch_cg_status(batch, iters, residual, done, (hipStream_t)stream);
     });
 }
 

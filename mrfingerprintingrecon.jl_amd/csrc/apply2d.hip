@@ -1,12 +1,10 @@
 // Fused 2D Toeplitz subspace normal operator: passes A and C plus dispatch.
-// Pass B (the kernel multiply) lives in apply2d_b.hip.
+// Pass B (the kernel multiply) lives in apply2d_b.hpp.
 #include "apply2d.hpp"
 #include "apply2d_b.hpp"
 #include "fft.hpp"
 
 namespace msr {
-
-constexpr int RT2 = 16;  // rows per tile in passes A and C (16 * 8 B = 128 B segments)
 
 template <int NX2>
 constexpr int rowpass_block() {
@@ -14,25 +12,26 @@ constexpr int rowpass_block() {
 }
 
 // ----------------------------------------------------------------- pass A
+// block lb = (tile, k, c), tile fastest
 template <int NX2>
-__global__ void __launch_bounds__(256)
-pass_a_2d(const float2* __restrict__ b, const float2* __restrict__ cmaps,
-          float2* __restrict__ T, const float2* __restrict__ tw, int nx, int ny, int K,
-          const int* __restrict__ skip) {
-    if (skip && *skip) return;
+__device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
+    if (a.skip && *a.skip) return;
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rowpass_block<NX2>();
-    constexpr int F = BLK / TP;          // rows transformed concurrently
+    constexpr int F = BLK / TP;              // rows transformed concurrently
     constexpr int LD = lds_line<NX2>() + 1;  // LDS row pitch (odd: transposed access)
     __shared__ float2 s_tw[NX2];
     __shared__ float2 s_rows[RT2 * LD];
+    const int nx = a.nx, ny = a.ny, K = a.K;
+    const int tiles = tiles2d(ny);
+    const int tile = (int)(lb % tiles);
+    const int k = (int)((lb / tiles) % K);
+    const int c = (int)(lb / ((int64_t)tiles * K));
     const int tid = threadIdx.x;
-    const int y0 = blockIdx.x * RT2;
-    const int k = blockIdx.y;
-    const int c = blockIdx.z;
-    load_twiddles<NX2>(s_tw, tw);
-    const float2* bk = b + (size_t)k * nx * ny;
-    const float2* cm = cmaps ? cmaps + (size_t)c * nx * ny : nullptr;
+    const int y0 = tile * RT2;
+    load_twiddles<NX2>(s_tw, a.tw_x);
+    const float2* bk = a.b + (size_t)k * nx * ny;
+    const float2* cm = a.cmaps ? a.cmaps + (size_t)c * nx * ny : nullptr;
     // load RT2 rows, multiply by the coil map, zero-pad to NX2
     for (int e = tid; e < RT2 * NX2; e += BLK) {
         const int yl = e / NX2, xx = e - yl * NX2;
@@ -49,7 +48,7 @@ pass_a_2d(const float2* __restrict__ b, const float2* __restrict__ cmaps,
 #pragma unroll
     for (int r0 = 0; r0 < RT2; r0 += F) fft_lds<NX2, -1>(s_rows + (r0 + g) * LD, s_tw, t);
     // transposed write: T[c][k][qx][y], y fastest
-    float2* Tk = T + ((size_t)c * K + k) * NX2 * ny;
+    float2* Tk = a.T + ((size_t)c * K + k) * NX2 * ny;
     for (int e = tid; e < RT2 * NX2; e += BLK) {
         const int yl = e % RT2, qx = e / RT2;
         const int y = y0 + yl;
@@ -57,13 +56,23 @@ pass_a_2d(const float2* __restrict__ b, const float2* __restrict__ cmaps,
     }
 }
 
-// ----------------------------------------------------------------- pass C
+template <int NX2>
+__global__ void __launch_bounds__(256) pass_a_2d(Apply2DArgs a) {
+    pass_a_body<NX2>(a, blockIdx.x);
+}
+
 template <int NX2>
 __global__ void __launch_bounds__(256)
-pass_c_2d(const float2* __restrict__ T, const float2* __restrict__ cmaps,
-          float2* __restrict__ partial, const float2* __restrict__ tw, int nx, int ny, int K,
-          int C, int G, const int* __restrict__ skip) {
-    if (skip && *skip) return;
+pass_a_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pre, int nj) {
+    const int j = batch_job(pre, nj, blockIdx.x);
+    pass_a_body<NX2>(jobs[j], (int64_t)blockIdx.x - pre[j]);
+}
+
+// ----------------------------------------------------------------- pass C
+// block lb = (tile, k, coil group), tile fastest
+template <int NX2>
+__device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
+    if (a.skip && *a.skip) return;
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rowpass_block<NX2>();
     constexpr int F = BLK / TP;
@@ -72,18 +81,21 @@ pass_c_2d(const float2* __restrict__ T, const float2* __restrict__ cmaps,
     constexpr int EA = (RT2 * NXC + BLK - 1) / BLK;
     __shared__ float2 s_tw[NX2];
     __shared__ float2 s_rows[RT2 * LD];
+    const int nx = a.nx, ny = a.ny, K = a.K, C = a.C, G = a.G;
+    const int tiles = tiles2d(ny);
+    const int tile = (int)(lb % tiles);
+    const int k = (int)((lb / tiles) % K);
+    const int grp = (int)(lb / ((int64_t)tiles * K));
     const int tid = threadIdx.x;
-    const int y0 = blockIdx.x * RT2;
-    const int k = blockIdx.y;
-    const int grp = blockIdx.z;
+    const int y0 = tile * RT2;
     const int c0 = (int)((int64_t)C * grp / G), c1 = (int)((int64_t)C * (grp + 1) / G);
-    load_twiddles<NX2>(s_tw, tw);
+    load_twiddles<NX2>(s_tw, a.tw_x);
     float2 acc[EA];
 #pragma unroll
     for (int i = 0; i < EA; ++i) acc[i] = make_float2(0.f, 0.f);
     const int g = tid / TP, t = tid - g * TP;
     for (int c = c0; c < c1; ++c) {
-        const float2* Tk = T + ((size_t)c * K + k) * NX2 * ny;
+        const float2* Tk = a.T + ((size_t)c * K + k) * NX2 * ny;
         for (int e = tid; e < RT2 * NX2; e += BLK) {
             const int yl = e % RT2, qx = e / RT2;
             const int y = y0 + yl;
@@ -92,7 +104,7 @@ pass_c_2d(const float2* __restrict__ T, const float2* __restrict__ cmaps,
         __syncthreads();
 #pragma unroll
         for (int r0 = 0; r0 < RT2; r0 += F) fft_lds<NX2, +1>(s_rows + (r0 + g) * LD, s_tw, t);
-        const float2* cm = cmaps ? cmaps + (size_t)c * nx * ny : nullptr;
+        const float2* cm = a.cmaps ? a.cmaps + (size_t)c * nx * ny : nullptr;
 #pragma unroll
         for (int i = 0; i < EA; ++i) {
             const int e = tid + i * BLK;
@@ -108,7 +120,7 @@ pass_c_2d(const float2* __restrict__ T, const float2* __restrict__ cmaps,
         }
         __syncthreads();
     }
-    float2* pk = partial + ((size_t)grp * K + k) * nx * ny;
+    float2* pk = a.partial + ((size_t)grp * K + k) * nx * ny;
 #pragma unroll
     for (int i = 0; i < EA; ++i) {
         const int e = tid + i * BLK;
@@ -120,26 +132,81 @@ pass_c_2d(const float2* __restrict__ T, const float2* __restrict__ cmaps,
     }
 }
 
+template <int NX2>
+__global__ void __launch_bounds__(256) pass_c_2d(Apply2DArgs a) {
+    pass_c_body<NX2>(a, blockIdx.x);
+}
+
+template <int NX2>
+__global__ void __launch_bounds__(256)
+pass_c_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pre, int nj) {
+    const int j = batch_job(pre, nj, blockIdx.x);
+    pass_c_body<NX2>(jobs[j], (int64_t)blockIdx.x - pre[j]);
+}
+
 // ----------------------------------------------------------------- dispatch
-void launch_apply2d_passes(const Apply2DArgs& a, const float2* b, hipStream_t st,
-                           const int* skip) {
-    const int NX2 = 2 * a.nx, NY2 = 2 * a.ny;
-    const int tiles = (a.ny + RT2 - 1) / RT2;
+static unsigned grid1d(int64_t nblocks) {
+    MSR_REQUIRE(nblocks >= 1 && nblocks < (int64_t)1 << 31, MSR_ERR_ARGUMENT,
+                "launch of %lld blocks", (long long)nblocks);
+    return (unsigned)nblocks;
+}
+
+void launch_apply2d_passes(const Apply2DArgs& a, hipStream_t st) {
+    const int NX2 = 2 * a.nx;
     bool ok = dispatch_fft_len(NX2, [&]<int N>() {
-        constexpr int BLK = rowpass_block<N>();
-        hipLaunchKernelGGL(pass_a_2d<N>, dim3(tiles, a.K, a.C), dim3(BLK), 0, st, b, a.cmaps,
-                           a.T, a.tw_x, a.nx, a.ny, a.K, skip);
+        hipLaunchKernelGGL(pass_a_2d<N>, dim3(grid1d(blocks_a2d(a))), dim3(rowpass_block<N>()), 0,
+                           st, a);
     });
     MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", NX2);
     MSR_CHECK_LAUNCH();
-    launch_pass_b_2d(a, st, skip);
-    ok = dispatch_fft_len(NX2, [&]<int N>() {
-        constexpr int BLK = rowpass_block<N>();
-        hipLaunchKernelGGL(pass_c_2d<N>, dim3(tiles, a.K, a.G), dim3(BLK), 0, st, a.T, a.cmaps,
-                           a.partial, a.tw_x, a.nx, a.ny, a.K, a.C, a.G, skip);
+    launch_pass_b_2d(a, st);
+    dispatch_fft_len(NX2, [&]<int N>() {
+        hipLaunchKernelGGL(pass_c_2d<N>, dim3(grid1d(blocks_c2d(a))), dim3(rowpass_block<N>()), 0,
+                           st, a);
     });
     MSR_CHECK_LAUNCH();
-    (void)NY2;
+}
+
+void launch_pass_a_2d_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
+                            int64_t nblocks, hipStream_t st) {
+    bool ok = dispatch_fft_len(nx2, [&]<int N>() {
+        hipLaunchKernelGGL(pass_a_2d_batch<N>, dim3(grid1d(nblocks)), dim3(rowpass_block<N>()), 0,
+                           st, jobs, pre, nj);
+    });
+    MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", nx2);
+    MSR_CHECK_LAUNCH();
+}
+
+int pass_b_colblocks(int nx2, int ny2, int K) {
+    int r = 0;
+#define MSR_CALLB(KK) r = pass_b_2d_colblocks_k##KK(nx2, ny2)
+    MSR_PASSB_SWITCH(K, MSR_CALLB)
+#undef MSR_CALLB
+    return r;
+}
+
+void launch_pass_b_2d_batch(int ny2, int K, int lam_real, const Apply2DArgs* jobs,
+                            const int* pre, int nj, int64_t nblocks, hipStream_t st) {
+    // the template arguments come from (ny2, K, lam_real); the job table itself
+    // stays on the device
+    Apply2DArgs shape{};
+    shape.ny = ny2 / 2;
+    shape.K = K;
+    shape.lam_real = lam_real;
+#define MSR_CALLB(KK) pass_b_2d_k##KK(shape, jobs, pre, nj, nblocks, st)
+    MSR_PASSB_SWITCH(K, MSR_CALLB)
+#undef MSR_CALLB
+    MSR_CHECK_LAUNCH();
+}
+
+void launch_pass_c_2d_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
+                            int64_t nblocks, hipStream_t st) {
+    bool ok = dispatch_fft_len(nx2, [&]<int N>() {
+        hipLaunchKernelGGL(pass_c_2d_batch<N>, dim3(grid1d(nblocks)), dim3(rowpass_block<N>()), 0,
+                           st, jobs, pre, nj);
+    });
+    MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", nx2);
+    MSR_CHECK_LAUNCH();
 }
 
 }  // namespace msr

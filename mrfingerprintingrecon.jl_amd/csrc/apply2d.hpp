@@ -17,6 +17,13 @@
 //                       multiply by conj(S_c), accumulate over the group's
 //                       coils in registers; one partial image per group.
 //   R  reduce:          x = beta x + alpha/(4 Nx Ny) sum_g partial_g.
+//
+// Every pass is written as a per-operator block body that takes a linear block
+// index local to its operator.  The same body serves two launch forms: one
+// operator per launch, and a batched launch in which the blocks of several
+// operators (same FFT lengths / K, any coil count) are concatenated and each
+// block finds its operator in a job table.  An operator's arithmetic is
+// therefore the same alone and inside a batch.
 #pragma once
 
 #include "common.hpp"
@@ -24,8 +31,9 @@
 namespace msr {
 
 struct Apply2DArgs {
-    int nx, ny;        // image size
-    int K, C;          // coefficients, coils (C >= 1)
+    int nx, ny;           // image size
+    int K, C;             // coefficients, coils (C >= 1)
+    const float2* b;      // input (K, ny, nx)
     const float2* cmaps;  // (C, ny, nx) or nullptr for the identity map
     const void* lam;      // dense kernel (2nx, P, 2ny), float2 or float
     int lam_real;
@@ -35,12 +43,45 @@ struct Apply2DArgs {
     int Gb;               // coil groups of pass B (>= 1)
     const float2* tw_x;   // twiddles for 2nx
     const float2* tw_y;   // twiddles for 2ny
+    const int* skip;      // when non-null and *skip != 0 the passes do nothing
 };
 
-// Launches passes A, B, C.  After this, partial[g] holds the group sums
-// (unscaled).  The caller runs a reduction that applies alpha, beta and the
-// 1/(4 nx ny) normalisation (see reduce kernels in cg.hpp / api).
-void launch_apply2d_passes(const Apply2DArgs& a, const float2* b, hipStream_t st,
-                           const int* skip_flag);
+constexpr int RT2 = 16;  // rows per tile in passes A and C (16 * 8 B = 128 B segments)
+
+__host__ __device__ inline int tiles2d(int ny) { return (ny + RT2 - 1) / RT2; }
+int pass_b_colblocks(int nx2, int ny2, int K);  // column blocks of pass B per coil group
+
+// block counts per operator of each pass
+inline int64_t blocks_a2d(const Apply2DArgs& a) { return (int64_t)tiles2d(a.ny) * a.K * a.C; }
+inline int64_t blocks_c2d(const Apply2DArgs& a) { return (int64_t)tiles2d(a.ny) * a.K * a.G; }
+inline int64_t blocks_b2d(const Apply2DArgs& a) {
+    return (int64_t)pass_b_colblocks(2 * a.nx, 2 * a.ny, a.K) * a.Gb;
+}
+
+// Launches passes A, B, C of one operator.  After this, partial[g] holds the
+// group sums (unscaled); the caller's reduction applies alpha, beta and the
+// 1/(4 nx ny) normalisation.
+void launch_apply2d_passes(const Apply2DArgs& a, hipStream_t st);
+
+// Batched form.  The operators of one launch share 2nx (passes A, C) or
+// (2ny, K, lam_real) (pass B); `jobs` is a device array of nj operators and
+// `pre` a device array of nj + 1 block offsets (pre[0] = 0) for that pass.
+void launch_pass_a_2d_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
+                            int64_t nblocks, hipStream_t st);
+void launch_pass_b_2d_batch(int ny2, int K, int lam_real, const Apply2DArgs* jobs,
+                            const int* pre, int nj, int64_t nblocks, hipStream_t st);
+void launch_pass_c_2d_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
+                            int64_t nblocks, hipStream_t st);
+
+// Operator of block `blk` in a batched launch: the j with pre[j] <= blk < pre[j+1].
+__device__ __forceinline__ int batch_job(const int* __restrict__ pre, int nj, int blk) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 }  // namespace msr

@@ -23,23 +23,25 @@ struct PassBShape {
     static constexpr int P = K * (K + 1) / 2;
 };
 
+// block lb = (column block, coil group), column block fastest
 template <int NY2, int K, bool LREAL>
-__global__ void __launch_bounds__(512)
-pass_b_2d(float2* __restrict__ T, const void* __restrict__ lam_v, const float2* __restrict__ tw,
-          int nx2, int ny, int C, int Gb, const int* __restrict__ skip) {
-    if (skip && *skip) return;
+__device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
+    if (a.skip && *a.skip) return;
     using S = PassBShape<NY2, K>;
     using LT = typename std::conditional<LREAL, float, float2>::type;
     constexpr int TP = S::TP, WC = S::WC, BLK = S::BLK, NV = S::NV, EV = S::EV, P = S::P;
     constexpr int LL = lds_line<NY2>();
     __shared__ float2 s_tw[NY2];
     __shared__ float2 s_buf[WC * K * LL];
-    const LT* lam = reinterpret_cast<const LT*>(lam_v);
+    const LT* lam = reinterpret_cast<const LT*>(a.lam);
+    float2* T = a.T;
+    const int nx2 = 2 * a.nx, ny = a.ny, C = a.C, Gb = a.Gb;
+    const int ncb = (nx2 + WC - 1) / WC;
     const int tid = threadIdx.x;
-    const int qx0 = blockIdx.x * WC;
-    const int grp = blockIdx.y;
+    const int qx0 = (int)(lb % ncb) * WC;
+    const int grp = (int)(lb / ncb);
     const int c0 = (int)((int64_t)C * grp / Gb), c1 = (int)((int64_t)C * (grp + 1) / Gb);
-    load_twiddles<NY2>(s_tw, tw);
+    load_twiddles<NY2>(s_tw, a.tw_y);
 
     // kernel values of the voxels this thread owns, kept for all coils
     LT lr[EV][P];
@@ -112,48 +114,87 @@ pass_b_2d(float2* __restrict__ T, const void* __restrict__ lam_v, const float2* 
 }
 
 template <int NY2, int K, bool LREAL>
-void launch_pass_b_2d_t(const Apply2DArgs& a, hipStream_t st, const int* skip) {
-    using S = PassBShape<NY2, K>;
-    const int nx2 = 2 * a.nx;
-    dim3 grid((nx2 + S::WC - 1) / S::WC, a.Gb);
-    hipLaunchKernelGGL((pass_b_2d<NY2, K, LREAL>), grid, dim3(S::BLK), 0, st, a.T, a.lam, a.tw_y,
-                       nx2, a.ny, a.C, a.Gb, skip);
+__global__ void __launch_bounds__(512) pass_b_2d(Apply2DArgs a) {
+    pass_b_body<NY2, K, LREAL>(a, blockIdx.x);
 }
 
-// per-K dispatchers (defined in apply2d_b_k*.hip)
-#define MSR_DECL_PASSB(KK) void launch_pass_b_2d_k##KK(const Apply2DArgs& a, hipStream_t st, const int* skip);
+template <int NY2, int K, bool LREAL>
+__global__ void __launch_bounds__(512)
+pass_b_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pre, int nj) {
+    const int j = batch_job(pre, nj, blockIdx.x);
+    pass_b_body<NY2, K, LREAL>(jobs[j], (int64_t)blockIdx.x - pre[j]);
+}
+
+// per-K entry points (defined in apply2d_b_k*.hip); jobs == nullptr means the
+// single-operator launch of `a`
+#define MSR_DECL_PASSB(KK)                                                                     \
+    void pass_b_2d_k##KK(const Apply2DArgs& a, const Apply2DArgs* jobs, const int* pre, int nj, \
+                         int64_t nblocks, hipStream_t st);                                     \
+    int pass_b_2d_colblocks_k##KK(int nx2, int ny2);
 MSR_DECL_PASSB(1) MSR_DECL_PASSB(2) MSR_DECL_PASSB(3) MSR_DECL_PASSB(4)
 MSR_DECL_PASSB(5) MSR_DECL_PASSB(6) MSR_DECL_PASSB(7) MSR_DECL_PASSB(8)
 #undef MSR_DECL_PASSB
 
-inline void launch_pass_b_2d(const Apply2DArgs& a, hipStream_t st, const int* skip) {
-    switch (a.K) {
-        case 1: launch_pass_b_2d_k1(a, st, skip); break;
-        case 2: launch_pass_b_2d_k2(a, st, skip); break;
-        case 3: launch_pass_b_2d_k3(a, st, skip); break;
-        case 4: launch_pass_b_2d_k4(a, st, skip); break;
-        case 5: launch_pass_b_2d_k5(a, st, skip); break;
-        case 6: launch_pass_b_2d_k6(a, st, skip); break;
-        case 7: launch_pass_b_2d_k7(a, st, skip); break;
-        case 8: launch_pass_b_2d_k8(a, st, skip); break;
-        default: fail(MSR_ERR_UNSUPPORTED, "number of coefficients K = %d not in 1..8", a.K);
+#define MSR_PASSB_SWITCH(K, CALL)                                                              \
+    switch (K) {                                                                               \
+        case 1: CALL(1); break;                                                                \
+        case 2: CALL(2); break;                                                                \
+        case 3: CALL(3); break;                                                                \
+        case 4: CALL(4); break;                                                                \
+        case 5: CALL(5); break;                                                                \
+        case 6: CALL(6); break;                                                                \
+        case 7: CALL(7); break;                                                                \
+        case 8: CALL(8); break;                                                                \
+        default: fail(MSR_ERR_UNSUPPORTED, "number of coefficients K = %d not in 1..8", K);   \
     }
+
+inline void launch_pass_b_2d(const Apply2DArgs& a, hipStream_t st) {
+#define MSR_CALLB(KK) pass_b_2d_k##KK(a, nullptr, nullptr, 0, blocks_b2d(a), st)
+    MSR_PASSB_SWITCH(a.K, MSR_CALLB)
+#undef MSR_CALLB
     MSR_CHECK_LAUNCH();
 }
 
-// Body of a per-K dispatcher: switch over the supported column lengths.
+// Body of a per-K entry point: switch over the supported column lengths.
 #define MSR_DEFINE_PASSB(KK)                                                                   \
-    void launch_pass_b_2d_k##KK(const Apply2DArgs& a, hipStream_t st, const int* skip) {       \
+    void pass_b_2d_k##KK(const Apply2DArgs& a, const Apply2DArgs* jobs, const int* pre, int nj, \
+                         int64_t nblocks, hipStream_t st) {                                    \
         const int ny2 = 2 * a.ny;                                                              \
+        MSR_REQUIRE(nblocks >= 1 && nblocks < ((int64_t)1 << 31), MSR_ERR_ARGUMENT,            \
+                    "pass B launch of %lld blocks", (long long)nblocks);                       \
         bool ok = dispatch_fft_len(ny2, [&]<int N>() {                                         \
             if constexpr (N <= 512) {                                                          \
-                if (a.lam_real) launch_pass_b_2d_t<N, KK, true>(a, st, skip);                  \
-                else launch_pass_b_2d_t<N, KK, false>(a, st, skip);                            \
+                using S = PassBShape<N, KK>;                                                   \
+                dim3 grid((unsigned)nblocks);                                                  \
+                if (jobs) {                                                                    \
+                    if (a.lam_real)                                                            \
+                        hipLaunchKernelGGL((pass_b_2d_batch<N, KK, true>), grid, dim3(S::BLK), \
+                                           0, st, jobs, pre, nj);                              \
+                    else                                                                       \
+                        hipLaunchKernelGGL((pass_b_2d_batch<N, KK, false>), grid,              \
+                                           dim3(S::BLK), 0, st, jobs, pre, nj);                \
+                } else {                                                                       \
+                    if (a.lam_real)                                                            \
+                        hipLaunchKernelGGL((pass_b_2d<N, KK, true>), grid, dim3(S::BLK), 0,    \
+                                           st, a);                                             \
+                    else                                                                       \
+                        hipLaunchKernelGGL((pass_b_2d<N, KK, false>), grid, dim3(S::BLK), 0,   \
+                                           st, a);                                             \
+                }                                                                              \
             } else {                                                                           \
                 fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Ny = %d above 512", N);             \
             }                                                                                  \
         });                                                                                    \
         MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Ny = %d", ny2);           \
+    }                                                                                          \
+    int pass_b_2d_colblocks_k##KK(int nx2, int ny2) {                                          \
+        int r = -1;                                                                            \
+        dispatch_fft_len(ny2, [&]<int N>() {                                                   \
+            constexpr int WC = PassBShape<N, KK>::WC;                                          \
+            r = (nx2 + WC - 1) / WC;                                                           \
+        });                                                                                    \
+        MSR_REQUIRE(r > 0, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Ny = %d", ny2);        \
+        return r;                                                                              \
     }
 
 }  // namespace msr

@@ -1,0 +1,464 @@
+// Batched applies and lockstep CG over many normal operators (see batch.hpp).
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <set>
+#include <tuple>
+
+#include "batch.hpp"
+
+namespace msr {
+
+static_assert(sizeof(CgJob<float2>) == sizeof(CgJob<double2>), "CgJob layouts differ");
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+BatchTables::~BatchTables() {
+    if (ev) (void)hipEventDestroy(ev);
+    if (host) (void)hipHostFree(host);
+}
+
+void BatchTables::ensure(size_t n) {
+    if (bytes == n && host) return;
+    if (ev) MSR_HIP(hipEventSynchronize(ev));
+    if (host) (void)hipHostFree(host);
+    host = nullptr;
+    MSR_HIP(hipHostMalloc(&host, n, hipHostMallocDefault));
+    dev.alloc(n);
+    bytes = n;
+    if (!ev) MSR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+}
+
+// ------------------------------------------------------------------ construction
+msr_batch* batch_create(msr_ctx* ctx, int nops, msr_normalop* const* ops) {
+    MSR_REQUIRE(nops >= 1, MSR_ERR_ARGUMENT, "a batch needs at least one operator (got %d)", nops);
+    MSR_REQUIRE(ops != nullptr, MSR_ERR_ARGUMENT, "ops is NULL");
+    auto bt = std::make_unique<msr_batch>();
+    bt->ctx = ctx;
+    std::set<const msr_normalop*> seen;
+    for (int j = 0; j < nops; ++j) {
+        MSR_REQUIRE(ops[j] != nullptr, MSR_ERR_ARGUMENT, "ops[%d] is NULL", j);
+        MSR_REQUIRE(ops[j]->ctx == ctx, MSR_ERR_ARGUMENT, "ops[%d] belongs to another context", j);
+        MSR_REQUIRE(seen.insert(ops[j]).second, MSR_ERR_ARGUMENT,
+                    "ops[%d] appears twice (each operator owns one scratch and CG state)", j);
+        bt->ops.push_back(ops[j]);
+        (ops[j]->d == 2 ? bt->j2d : bt->j3d).push_back(j);
+    }
+    // launch groups of the 2D passes
+    std::map<int, int> ia;
+    std::map<std::tuple<int, int, int>, int> ib;
+    for (int j : bt->j2d) {
+        const msr_normalop* op = bt->ops[j];
+        const int nx2 = (int)(2 * op->N[0]), ny2 = (int)(2 * op->N[1]);
+        if (!ia.count(nx2)) {
+            ia[nx2] = (int)bt->ga.size();
+            bt->ga.emplace_back();
+            bt->ga.back().key[0] = nx2;
+            bt->gc.emplace_back();
+            bt->gc.back().key[0] = nx2;
+        }
+        bt->ga[ia[nx2]].jobs.push_back(j);
+        bt->gc[ia[nx2]].jobs.push_back(j);
+        const auto kb = std::make_tuple(ny2, op->K, op->lam_real);
+        if (!ib.count(kb)) {
+            ib[kb] = (int)bt->gb.size();
+            bt->gb.emplace_back();
+            bt->gb.back().key[0] = ny2;
+            bt->gb.back().key[1] = op->K;
+            bt->gb.back().key[2] = op->lam_real;
+        }
+        bt->gb[ib[kb]].jobs.push_back(j);
+    }
+    auto prefix = [&](BatchGroup& g, int64_t (*count)(const Apply2DArgs&)) {
+        g.pre.assign(1, 0);
+        int64_t tot = 0;
+        for (int j : g.jobs) {
+            tot += count(args2d(bt->ops[j]));
+            MSR_REQUIRE(tot < ((int64_t)1 << 31), MSR_ERR_UNSUPPORTED,
+                        "batched launch above 2^31 blocks");
+            g.pre.push_back((int)tot);
+        }
+        g.nblocks = tot;
+    };
+    for (auto& g : bt->ga) prefix(g, blocks_a2d);
+    for (auto& g : bt->gb) prefix(g, blocks_b2d);
+    for (auto& g : bt->gc) prefix(g, blocks_c2d);
+    bt->vpre.assign(1, 0);
+    for (auto* op : bt->ops) bt->vpre.push_back(bt->vpre.back() + red_grid(op->V * op->K));
+    bt->vblocks = bt->vpre.back();
+    // table layout
+    size_t off = 0;
+    for (auto* gv : {&bt->ga, &bt->gb, &bt->gc}) {
+        for (auto& g : *gv) {
+            g.off_tab = off;
+            off = align256(off + sizeof(Apply2DArgs) * g.jobs.size());
+            g.off_pre = off;
+            off = align256(off + sizeof(int) * g.pre.size());
+        }
+    }
+    bt->off_cgjobs = off;
+    off = align256(off + sizeof(CgJob<double2>) * nops);
+    bt->off_vpre = off;
+    off = align256(off + sizeof(int) * bt->vpre.size());
+    bt->table_bytes = off;
+    return bt.release();
+}
+
+// ------------------------------------------------------------------ tables
+// Apply-input of operator j for one use of the tables.
+struct TableUse {
+    const float2* const* b = nullptr;  // per operator; nullptr entry -> its CG u32
+    bool skip_done = false;            // gate the passes by the CG done flags
+};
+
+template <class VT>
+static void fill_tables(msr_batch* bt, BatchTables& T, const TableUse& use,
+                        const std::vector<CgJob<VT>>& cj, hipStream_t st) {
+    T.ensure(bt->table_bytes);
+    if (T.ev) MSR_HIP(hipEventSynchronize(T.ev));  // the previous upload has left the staging
+    char* h = static_cast<char*>(T.host);
+    auto put_group = [&](const BatchGroup& g) {
+        auto* tab = reinterpret_cast<Apply2DArgs*>(h + g.off_tab);
+        for (size_t i = 0; i < g.jobs.size(); ++i) {
+            const int j = g.jobs[i];
+            Apply2DArgs a = args2d(bt->ops[j]);
+            a.b = use.b[j];
+            a.skip = use.skip_done ? &bt->ops[j]->cg_scal.as<CgScalars>()->done : nullptr;
+            tab[i] = a;
+        }
+        memcpy(h + g.off_pre, g.pre.data(), sizeof(int) * g.pre.size());
+    };
+    for (auto& g : bt->ga) put_group(g);
+    for (auto& g : bt->gb) put_group(g);
+    for (auto& g : bt->gc) put_group(g);
+    memcpy(h + bt->off_cgjobs, cj.data(), sizeof(CgJob<VT>) * cj.size());
+    memcpy(h + bt->off_vpre, bt->vpre.data(), sizeof(int) * bt->vpre.size());
+    MSR_HIP(hipMemcpyAsync(T.dev.p, T.host, bt->table_bytes, hipMemcpyHostToDevice, st));
+    MSR_HIP(hipEventRecord(T.ev, st));
+}
+
+template <class VT>
+static CgJob<VT>* dev_cgjobs(msr_batch* bt, BatchTables& T) {
+    return reinterpret_cast<CgJob<VT>*>(T.dev.as<char>() + bt->off_cgjobs);
+}
+static const int* dev_vpre(msr_batch* bt, BatchTables& T) {
+    return reinterpret_cast<const int*>(T.dev.as<char>() + bt->off_vpre);
+}
+
+template <class VT>
+static CgJob<VT> base_job(msr_normalop* op, float2* x) {
+    CgJob<VT> J{};
+    J.v = op_cg_vecs<VT>(op, x);
+    J.partial = op->partial.as<float2>();
+    J.G = op->G;
+    J.scale = op_scale(op);
+    J.n = op->V * op->K;
+    J.S = op->cg_scal.as<CgScalars>();
+    J.part = op->cg_part.as<double>();
+    J.alpha = make_float2(1.f, 0.f);
+    J.beta = make_float2(0.f, 0.f);
+    J.beta_zero = 1;
+    return J;
+}
+
+// the three pass sequences of every operator, in pass order across the batch
+static void batch_passes(msr_batch* bt, BatchTables& T, const TableUse& use, hipStream_t st) {
+    char* base = T.dev.as<char>();
+    auto tab = [&](const BatchGroup& g) { return reinterpret_cast<const Apply2DArgs*>(base + g.off_tab); };
+    auto pre = [&](const BatchGroup& g) { return reinterpret_cast<const int*>(base + g.off_pre); };
+    for (auto& g : bt->ga)
+        launch_pass_a_2d_batch(g.key[0], tab(g), pre(g), (int)g.jobs.size(), g.nblocks, st);
+    for (auto& g : bt->gb)
+        launch_pass_b_2d_batch(g.key[0], g.key[1], g.key[2], tab(g), pre(g), (int)g.jobs.size(),
+                               g.nblocks, st);
+    for (auto& g : bt->gc)
+        launch_pass_c_2d_batch(g.key[0], tab(g), pre(g), (int)g.jobs.size(), g.nblocks, st);
+    for (int j : bt->j3d) {
+        msr_normalop* op = bt->ops[j];
+        op_passes(op, use.b[j], st,
+                  use.skip_done ? &op->cg_scal.as<CgScalars>()->done : nullptr);
+    }
+}
+
+// ------------------------------------------------------------------ apply
+void batch_apply(msr_batch* bt, float2* const* x, const float2* const* b, float2 alpha,
+                 float2 beta, bool beta_zero, hipStream_t st) {
+    const int nj = (int)bt->ops.size();
+    MSR_REQUIRE(x && b, MSR_ERR_ARGUMENT, "x or b is NULL");
+    std::vector<CgJob<float2>> cj(nj);
+    for (int j = 0; j < nj; ++j) {
+        MSR_REQUIRE(x[j] && b[j], MSR_ERR_ARGUMENT, "x[%d] or b[%d] is NULL", j, j);
+        MSR_REQUIRE((const void*)x[j] != (const void*)b[j], MSR_ERR_ARGUMENT,
+                    "x[%d] and b[%d] must not alias", j, j);
+        msr_normalop* op = bt->ops[j];
+        CgJob<float2> J{};
+        J.partial = op->partial.as<float2>();
+        J.G = op->G;
+        J.scale = op_scale(op);
+        J.n = op->V * op->K;
+        J.y = x[j];
+        J.alpha = alpha;
+        J.beta = beta;
+        J.beta_zero = beta_zero ? 1 : 0;
+        cj[j] = J;
+    }
+    TableUse use;
+    use.b = b;
+    fill_tables(bt, bt->t_apply, use, cj, st);
+    batch_passes(bt, bt->t_apply, use, st);
+    launch_epilogue_batch(dev_cgjobs<float2>(bt, bt->t_apply), dev_vpre(bt, bt->t_apply), nj,
+                          bt->vblocks, st);
+}
+
+// ------------------------------------------------------------------ CG
+// Per-operator inputs of the CG passes: each operator's FP32 search direction.
+template <class VT>
+static std::vector<const float2*> cg_inputs(msr_batch* bt) {
+    std::vector<const float2*> u(bt->ops.size());
+    for (size_t j = 0; j < bt->ops.size(); ++j)
+        u[j] = op_cg_vecs<VT>(bt->ops[j], bt->ops[j]->cg_x_user).u32;
+    return u;
+}
+
+template <class VT>
+static void build_cg_tables(msr_batch* bt, BatchTables& T, const float2* const* b, hipStream_t st) {
+    const int nj = (int)bt->ops.size();
+    std::vector<CgJob<VT>> cj(nj);
+    for (int j = 0; j < nj; ++j) {
+        msr_normalop* op = bt->ops[j];
+        CgJob<VT> J = base_job<VT>(op, op->cg_x_user);
+        J.b = b ? b[j] : nullptr;
+        J.x_in = op->cg_x_user;
+        J.x_out = op->cg_x_user;
+        cj[j] = J;
+    }
+    auto u = cg_inputs<VT>(bt);
+    TableUse use;
+    use.b = u.data();
+    use.skip_done = true;
+    fill_tables(bt, T, use, cj, st);
+}
+
+template <class VT>
+static void start_seq(msr_batch* bt, BatchTables& Tcg, BatchTables& Tx0, float2* const* x,
+                      int maxiter, double reltol, double abstol, bool initially_zero,
+                      hipStream_t st) {
+    const int nj = (int)bt->ops.size();
+    if (!initially_zero) {
+        // c = A x0 (the same apply and epilogue as a single operator's prologue)
+        TableUse use;
+        std::vector<const float2*> xin(x, x + nj);
+        use.b = xin.data();
+        batch_passes(bt, Tx0, use, st);
+        launch_epilogue_batch(reinterpret_cast<const CgJob<float2>*>(dev_cgjobs<VT>(bt, Tx0)),
+                              dev_vpre(bt, Tx0), nj, bt->vblocks, st);
+    }
+    launch_cg_start_batch<VT>(dev_cgjobs<VT>(bt, Tcg), dev_vpre(bt, Tcg), nj, bt->vblocks,
+                              initially_zero, !initially_zero, reltol, abstol, maxiter, st);
+}
+
+template <class VT>
+static void iterate_seq(msr_batch* bt, BatchTables& T, int niter, hipStream_t st) {
+    const int nj = (int)bt->ops.size();
+    auto u = cg_inputs<VT>(bt);
+    TableUse use;
+    use.b = u.data();
+    use.skip_done = true;
+    CgJob<VT>* cj = dev_cgjobs<VT>(bt, T);
+    const int* vp = dev_vpre(bt, T);
+    for (int it = 0; it < niter; ++it) {
+        launch_cg_u_batch<VT>(cj, vp, nj, bt->vblocks, st);
+        batch_passes(bt, T, use, st);
+        launch_cg_reduce_dot_batch<VT>(cj, vp, nj, bt->vblocks, st);
+        launch_cg_xr_batch<VT>(cj, vp, nj, bt->vblocks, st);
+    }
+}
+
+template <class VT>
+static void finish_seq(msr_batch* bt, BatchTables& T, hipStream_t st) {
+    if constexpr (std::is_same<VT, double2>::value)
+        launch_to_f32_batch(dev_cgjobs<double2>(bt, T), dev_vpre(bt, T), (int)bt->ops.size(),
+                            bt->vblocks, st);
+}
+
+// The x0 tables: inputs x_j, epilogue target c_j.
+template <class VT>
+static void build_x0_tables(msr_batch* bt, BatchTables& T, float2* const* x, hipStream_t st) {
+    const int nj = (int)bt->ops.size();
+    std::vector<CgJob<VT>> cj(nj);
+    for (int j = 0; j < nj; ++j) {
+        CgJob<VT> J = base_job<VT>(bt->ops[j], x[j]);
+        J.y = J.v.c;
+        cj[j] = J;
+    }
+    std::vector<const float2*> xin(x, x + nj);
+    TableUse use;
+    use.b = xin.data();
+    fill_tables(bt, T, use, cj, st);
+}
+
+static void check_vectors(msr_batch* bt, float2* const* x, const float2* const* b) {
+    MSR_REQUIRE(x && b, MSR_ERR_ARGUMENT, "x or b is NULL");
+    for (size_t j = 0; j < bt->ops.size(); ++j) {
+        MSR_REQUIRE(x[j] && b[j], MSR_ERR_ARGUMENT, "x[%zu] or b[%zu] is NULL", j, j);
+        MSR_REQUIRE((const void*)x[j] != (const void*)b[j], MSR_ERR_ARGUMENT,
+                    "x[%zu] and b[%zu] must not alias", j, j);
+    }
+}
+
+static void record_start(msr_batch* bt, float2* const* x, bool fp64) {
+    for (size_t j = 0; j < bt->ops.size(); ++j) {
+        msr_normalop* op = bt->ops[j];
+        op_cg_buffers(op, fp64);
+        op->cg_fp64 = fp64 ? 1 : 0;
+        op->cg_x_user = x[j];
+    }
+}
+
+template <class VT>
+static void cg_start_t(msr_batch* bt, float2* const* x, const float2* const* b, int maxiter,
+                       double reltol, double abstol, bool initially_zero, hipStream_t st) {
+    build_cg_tables<VT>(bt, bt->t_cg, b, st);
+    if (!initially_zero) build_x0_tables<VT>(bt, bt->t_x0, x, st);
+    start_seq<VT>(bt, bt->t_cg, bt->t_x0, x, maxiter, reltol, abstol, initially_zero, st);
+    bt->cg_ready = true;
+}
+
+void batch_cg_start(msr_batch* bt, float2* const* x, const float2* const* b, int maxiter,
+                    double reltol, double abstol, bool initially_zero, bool fp64, hipStream_t st) {
+    check_vectors(bt, x, b);
+    MSR_REQUIRE(maxiter >= 0, MSR_ERR_ARGUMENT, "maxiter = %d", maxiter);
+    record_start(bt, x, fp64);
+    if (fp64) cg_start_t<double2>(bt, x, b, maxiter, reltol, abstol, initially_zero, st);
+    else cg_start_t<float2>(bt, x, b, maxiter, reltol, abstol, initially_zero, st);
+}
+
+static int batch_fp64(msr_batch* bt) {
+    const int f = bt->ops[0]->cg_fp64;
+    for (size_t j = 0; j < bt->ops.size(); ++j) {
+        const msr_normalop* op = bt->ops[j];
+        MSR_REQUIRE(op->cg_fp64 >= 0 && op->cg_x_user, MSR_ERR_ARGUMENT,
+                    "ops[%zu] has no started CG (call msr_batch_cg_start first)", j);
+        MSR_REQUIRE(op->cg_fp64 == f, MSR_ERR_ARGUMENT,
+                    "the operators of a batch were started with different vector precisions");
+    }
+    return f;
+}
+
+void batch_cg_iterate(msr_batch* bt, int niter, hipStream_t st) {
+    MSR_REQUIRE(niter >= 0, MSR_ERR_ARGUMENT, "niter = %d", niter);
+    const int f = batch_fp64(bt);
+    if (!bt->cg_ready) {
+        if (f) build_cg_tables<double2>(bt, bt->t_cg, nullptr, st);
+        else build_cg_tables<float2>(bt, bt->t_cg, nullptr, st);
+        bt->cg_ready = true;
+    }
+    if (f) iterate_seq<double2>(bt, bt->t_cg, niter, st);
+    else iterate_seq<float2>(bt, bt->t_cg, niter, st);
+}
+
+void batch_cg_finish(msr_batch* bt, hipStream_t st) {
+    const int f = batch_fp64(bt);
+    if (!bt->cg_ready) {
+        if (f) build_cg_tables<double2>(bt, bt->t_cg, nullptr, st);
+        else build_cg_tables<float2>(bt, bt->t_cg, nullptr, st);
+        bt->cg_ready = true;
+    }
+    if (f) finish_seq<double2>(bt, bt->t_cg, st);
+}
+
+template <class VT>
+static void cg_graph_t(msr_batch* bt, float2* const* x, const float2* const* b, int maxiter,
+                       double reltol, double abstol, bool initially_zero, hipStream_t st,
+                       const std::vector<const void*>& key) {
+    if (bt->graph && bt->graph_key == key) {
+        MSR_HIP(hipGraphLaunch(bt->graph, st));
+        return;
+    }
+    if (bt->graph) {
+        MSR_HIP(hipStreamSynchronize(st));  // the old graph may still read its tables
+        (void)hipGraphExecDestroy(bt->graph);
+        bt->graph = nullptr;
+    }
+    // tables of the graph, uploaded before capture and left untouched afterwards
+    build_cg_tables<VT>(bt, bt->g_cg, b, st);
+    if (!initially_zero) build_x0_tables<VT>(bt, bt->g_x0, x, st);
+    hipGraph_t g = nullptr;
+    MSR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        start_seq<VT>(bt, bt->g_cg, bt->g_x0, x, maxiter, reltol, abstol, initially_zero, st);
+        iterate_seq<VT>(bt, bt->g_cg, maxiter, st);
+        finish_seq<VT>(bt, bt->g_cg, st);
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    MSR_HIP(hipStreamEndCapture(st, &g));
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MSR_HIP(e);
+    bt->graph = ge;
+    bt->graph_key = key;
+    MSR_HIP(hipGraphLaunch(bt->graph, st));
+}
+
+void batch_cg(msr_batch* bt, float2* const* x, const float2* const* b, int maxiter, double reltol,
+              double abstol, bool initially_zero, int mode, bool fp64, hipStream_t st) {
+    check_vectors(bt, x, b);
+    MSR_REQUIRE(maxiter >= 0, MSR_ERR_ARGUMENT, "maxiter = %d", maxiter);
+    if (mode == MSR_CG_DEVICE) {
+        batch_cg_start(bt, x, b, maxiter, reltol, abstol, initially_zero, fp64, st);
+        batch_cg_iterate(bt, maxiter, st);
+        batch_cg_finish(bt, st);
+        return;
+    }
+    MSR_REQUIRE(mode == MSR_CG_GRAPH, MSR_ERR_UNSUPPORTED,
+                "batched CG runs in MSR_CG_DEVICE or MSR_CG_GRAPH mode (got %d)", mode);
+    MSR_REQUIRE(st != nullptr, MSR_ERR_ARGUMENT, "MSR_CG_GRAPH needs an explicit (non-default) stream");
+    record_start(bt, x, fp64);
+    std::vector<const void*> key;
+    for (size_t j = 0; j < bt->ops.size(); ++j) {
+        const msr_normalop* op = bt->ops[j];
+        key.push_back(x[j]);
+        key.push_back(b[j]);
+        // the operator's CG buffers (re-allocated if another caller changed
+        // the vector precision in between)
+        for (const DevBuf* d : {&op->cg_r, &op->cg_u, &op->cg_c, &op->cg_scal, &op->cg_part,
+                                &op->cg_x64, &op->cg_u32})
+            key.push_back(d->p);
+    }
+    uint64_t bits[2];
+    memcpy(&bits[0], &reltol, 8);
+    memcpy(&bits[1], &abstol, 8);
+    key.push_back((const void*)(uintptr_t)bits[0]);
+    key.push_back((const void*)(uintptr_t)bits[1]);
+    key.push_back((const void*)(intptr_t)maxiter);
+    key.push_back((const void*)(intptr_t)((initially_zero ? 1 : 0) + (fp64 ? 2 : 0)));
+    key.push_back(st);
+    if (fp64) cg_graph_t<double2>(bt, x, b, maxiter, reltol, abstol, initially_zero, st, key);
+    else cg_graph_t<float2>(bt, x, b, maxiter, reltol, abstol, initially_zero, st, key);
+    bt->cg_ready = false;  // eager calls rebuild t_cg from the operators' state
+}
+
+void batch_cg_status(msr_batch* bt, int* iters, double* res, int* done, hipStream_t st) {
+    std::vector<CgScalars> hs(bt->ops.size());
+    for (size_t j = 0; j < bt->ops.size(); ++j) {
+        MSR_REQUIRE(bt->ops[j]->cg_scal.p, MSR_ERR_ARGUMENT, "ops[%zu] has no CG state", j);
+        MSR_HIP(hipMemcpyAsync(&hs[j], bt->ops[j]->cg_scal.p, sizeof(CgScalars),
+                               hipMemcpyDeviceToHost, st));
+    }
+    MSR_HIP(hipStreamSynchronize(st));
+    for (size_t j = 0; j < bt->ops.size(); ++j) {
+        if (iters) iters[j] = hs[j].iter;
+        if (res) res[j] = hs[j].res;
+        if (done) done[j] = hs[j].done;
+    }
+}
+
+}  // namespace msr
+
+msr_batch::~msr_batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+}

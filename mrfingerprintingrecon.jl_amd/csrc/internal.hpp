@@ -100,8 +100,10 @@ struct msr_normalop {
     int G = 1, Gb = 1;
     // CG state (allocated on first use)
     msr::DevBuf cg_r, cg_u, cg_c, cg_scal, cg_part, cg_x64, cg_u32;
+    int cg_fp64 = -1;                 // vector precision of the last started CG
+    float2* cg_x_user = nullptr;      // its caller-side FP32 solution
     hipGraphExec_t graph = nullptr;
-    const void* graph_key[3] = {nullptr, nullptr, nullptr};
+    std::vector<const void*> graph_key;
     int graph_maxiter = -1;
     double graph_tol[2] = {0.0, 0.0};
     int graph_fp64 = -1;

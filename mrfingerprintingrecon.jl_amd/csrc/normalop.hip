@@ -11,10 +11,9 @@
 #include <cstring>
 #include <type_traits>
 
-#include "apply2d.hpp"
 #include "apply3d.hpp"
 #include "fft.hpp"
-#include "vecops.hpp"
+#include "normalop.hpp"
 
 namespace msr {
 
@@ -87,7 +86,7 @@ static int env_int(const char* name, int dflt) {
 void op_alloc_scratch(msr_normalop* op) {
     const int64_t nx = op->N[0], ny = op->N[1];
     if (op->d == 2) {
-        const int tiles = (int)((ny + 15) / 16);
+        const int tiles = tiles2d((int)ny);
         int G = (int)ceil_div(512, (int64_t)tiles * op->K);
         G = std::max(1, std::min(G, op->C));
         op->G = env_int("MSR_G", G);
@@ -196,7 +195,7 @@ msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real)
 }
 
 // ------------------------------------------------------------------ applies
-static Apply2DArgs args2d(msr_normalop* op) {
+Apply2DArgs args2d(msr_normalop* op) {
     Apply2DArgs a;
     a.nx = (int)op->N[0];
     a.ny = (int)op->N[1];
@@ -211,6 +210,8 @@ static Apply2DArgs args2d(msr_normalop* op) {
     a.Gb = op->Gb;
     a.tw_x = twiddle_table(op->ctx, 2 * a.nx);
     a.tw_y = twiddle_table(op->ctx, 2 * a.ny);
+    a.b = nullptr;
+    a.skip = nullptr;
     return a;
 }
 
@@ -218,8 +219,14 @@ float op_scale(const msr_normalop* op) { return (float)(1.0 / (double)op_grid_ce
 
 // partial[g] <- unscaled group sums of A^H A b
 void op_passes(msr_normalop* op, const float2* b, hipStream_t st, const int* skip) {
-    if (op->d == 2) launch_apply2d_passes(args2d(op), b, st, skip);
-    else launch_apply3d_passes(op, b, st, skip);
+    if (op->d == 2) {
+        Apply2DArgs a = args2d(op);
+        a.b = b;
+        a.skip = skip;
+        launch_apply2d_passes(a, st);
+    } else {
+        launch_apply3d_passes(op, b, st, skip);
+    }
 }
 
 void op_apply(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
@@ -240,7 +247,8 @@ void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha
             a.G = 1;
             a.Gb = 1;
             if (a.cmaps) a.cmaps += (size_t)c * op->V;
-            launch_apply2d_passes(a, b, st, nullptr);
+            a.b = b;
+            launch_apply2d_passes(a, st);
         } else {
             launch_apply3d_passes_coil(op, b, c, st);
         }
@@ -270,7 +278,7 @@ void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hip
 }
 
 // ------------------------------------------------------------------ CG
-static void cg_buffers(msr_normalop* op, bool fp64) {
+void op_cg_buffers(msr_normalop* op, bool fp64) {
     const size_t n = (size_t)op->V * op->K;
     const size_t es = fp64 ? sizeof(double2) : sizeof(float2);
     op->cg_r.alloc(es * n);
@@ -285,7 +293,7 @@ static void cg_buffers(msr_normalop* op, bool fp64) {
 }
 
 template <class VT>
-static CgVecs<VT> cg_vecs(msr_normalop* op, float2* x) {
+CgVecs<VT> op_cg_vecs(msr_normalop* op, float2* x) {
     CgVecs<VT> v;
     v.r = op->cg_r.as<VT>();
     v.u = op->cg_u.as<VT>();
@@ -299,6 +307,9 @@ static CgVecs<VT> cg_vecs(msr_normalop* op, float2* x) {
     }
     return v;
 }
+
+template CgVecs<float2> op_cg_vecs<float2>(msr_normalop*, float2*);
+template CgVecs<double2> op_cg_vecs<double2>(msr_normalop*, float2*);
 
 // one CG iteration with device scalars (all kernels skip once done is set)
 template <class VT>
@@ -342,7 +353,7 @@ static void cg_epilogue(msr_normalop* op, const CgVecs<VT>& v, float2* x, hipStr
 template <class VT>
 static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
                    double abstol, bool initially_zero, int mode, hipStream_t st) {
-    const CgVecs<VT> v = cg_vecs<VT>(op, x);
+    const CgVecs<VT> v = op_cg_vecs<VT>(op, x);
     auto* S = op->cg_scal.as<CgScalars>();
     if (mode == MSR_CG_DEVICE) {
         cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
@@ -350,13 +361,18 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
         cg_epilogue(op, v, x, st);
     } else if (mode == MSR_CG_GRAPH) {
         const int fp = std::is_same<VT, double2>::value ? 1 : 0;
-        const bool hit = op->graph && op->graph_key[0] == x && op->graph_key[1] == b &&
-                         op->graph_key[2] == (const void*)(intptr_t)(initially_zero ? 1 : 2) &&
-                         op->graph_maxiter == maxiter && op->graph_stream == st &&
+        // the captured kernels hold the CG buffer addresses: they are part of
+        // the key (another call with the other vector precision re-allocates them)
+        std::vector<const void*> key = {x, b, (const void*)(intptr_t)(initially_zero ? 1 : 2), st};
+        for (const DevBuf* d : {&op->cg_r, &op->cg_u, &op->cg_c, &op->cg_scal, &op->cg_part,
+                                &op->cg_x64, &op->cg_u32})
+            key.push_back(d->p);
+        const bool hit = op->graph && op->graph_key == key && op->graph_maxiter == maxiter &&
                          op->graph_tol[0] == reltol && op->graph_tol[1] == abstol &&
                          op->graph_fp64 == fp;
         if (!hit) {
             if (op->graph) {
+                MSR_HIP(hipStreamSynchronize(op->graph_stream));  // a replay may still run
                 (void)hipGraphExecDestroy(op->graph);
                 op->graph = nullptr;
             }
@@ -379,11 +395,9 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
             (void)hipGraphDestroy(g);
             MSR_HIP(e);
             op->graph = ge;
-            op->graph_key[0] = x;
-            op->graph_key[1] = b;
-            op->graph_key[2] = (const void*)(intptr_t)(initially_zero ? 1 : 2);
-            op->graph_maxiter = maxiter;
+            op->graph_key = key;
             op->graph_stream = st;
+            op->graph_maxiter = maxiter;
             op->graph_tol[0] = reltol;
             op->graph_tol[1] = abstol;
             op->graph_fp64 = fp;
@@ -417,7 +431,7 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
 void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
            double abstol, bool initially_zero, int mode, bool fp64, int* iters_out,
            double* res_out, hipStream_t st) {
-    cg_buffers(op, fp64);
+    op_cg_buffers(op, fp64);
     if (fp64) cg_run<double2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, st);
     else cg_run<float2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, st);
     if (iters_out || res_out) {

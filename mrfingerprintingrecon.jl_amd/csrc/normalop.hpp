@@ -1,0 +1,32 @@
+// Normal-operator internals shared by the API layer and the batch engine.
+#pragma once
+
+#include "apply2d.hpp"
+#include "internal.hpp"
+#include "vecops.hpp"
+
+namespace msr {
+
+msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real);
+void op_set_cmaps(msr_normalop* op, int ncoils, const void* const* cmaps, hipStream_t st);
+void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, int64_t nkm,
+                    hipStream_t st);
+void op_alloc_scratch(msr_normalop* op);
+void op_apply(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
+              bool beta_zero, hipStream_t st);
+void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
+                      bool beta_zero, hipStream_t st);
+void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hipStream_t st);
+void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
+           double abstol, bool initially_zero, int mode, bool fp64, int* iters_out,
+           double* res_out, hipStream_t st);
+
+// pieces used by the batch engine
+Apply2DArgs args2d(msr_normalop* op);                 // b = skip = nullptr
+void op_passes(msr_normalop* op, const float2* b, hipStream_t st, const int* skip);
+float op_scale(const msr_normalop* op);               // 1 / Toeplitz grid cells
+void op_cg_buffers(msr_normalop* op, bool fp64);      // allocates; not during capture
+template <class VT>
+CgVecs<VT> op_cg_vecs(msr_normalop* op, float2* x);   // x: caller's FP32 solution
+
+}  // namespace msr

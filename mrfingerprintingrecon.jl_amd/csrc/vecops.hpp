@@ -1,5 +1,12 @@
 // Vector kernels: apply epilogue (group reduction with alpha/beta), and the
 // device conjugate-gradient steps with FP64 reductions.
+//
+// Each CG step is a per-operator block body taking (block, number of blocks)
+// of its operator, so that the same arithmetic runs in a single-operator
+// launch and in a batched launch that concatenates the blocks of many
+// operators (each block finds its operator in a job table): every operator's
+// reductions see the same block partition, hence the same rounding, alone and
+// in a batch.
 #pragma once
 
 #include "internal.hpp"
@@ -8,6 +15,13 @@ namespace msr {
 
 constexpr int RED_BLOCKS = 512;   // blocks of the two-level FP64 reductions
 constexpr int RED_THREADS = 256;
+
+inline int red_grid(int64_t n) {
+    int64_t g = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
+    if (g < 1) g = 1;
+    if (g > RED_BLOCKS) g = RED_BLOCKS;
+    return (int)g;
+}
 
 // x = beta*x + alpha*scale*sum_g partial[g]   (beta_zero: overwrite)
 void launch_epilogue(const float2* partial, int G, int64_t n, float2* x, float2 alpha, float2 beta,
@@ -42,5 +56,43 @@ template <class VT>
 void launch_cg_xr(const CgVecs<VT>& v, int64_t n, double* part, CgScalars* S, hipStream_t st);
 // x32 = (float) x64
 void launch_to_f32(const double2* x64, float2* x32, int64_t n, hipStream_t st);
+
+// ---------------------------------------------------------------- batched
+// One operator's CG state in a batched launch.
+template <class VT>
+struct CgJob {
+    CgVecs<VT> v;
+    const float2* b;         // right-hand side (start)
+    const float2* x_in;      // initial guess (start, not initially zero)
+    float2* x_out;           // FP32 result (finish, FP64 vectors)
+    float2* y;               // target of the batched apply epilogue
+    const float2* partial;   // apply group sums
+    int G;                   // groups in partial
+    float scale;             // apply normalisation 1/grid cells
+    int64_t n;               // vector length V*K
+    CgScalars* S;
+    double* part;            // reduction partials (2 * RED_BLOCKS)
+    float2 alpha, beta;      // batched apply epilogue
+    int beta_zero;
+};
+
+// Block prefix of a batch: job j owns blocks [pre[j], pre[j+1]) of every
+// batched vector launch, pre[j+1] - pre[j] = red_grid(n_j).
+template <class VT>
+void launch_cg_start_batch(const CgJob<VT>* jobs, const int* pre, int nj, int nblocks,
+                           bool zero_x, bool r_minus_c, double reltol, double abstol, int maxiter,
+                           hipStream_t st);
+template <class VT>
+void launch_cg_u_batch(const CgJob<VT>* jobs, const int* pre, int nj, int nblocks, hipStream_t st);
+template <class VT>
+void launch_cg_reduce_dot_batch(const CgJob<VT>* jobs, const int* pre, int nj, int nblocks,
+                                hipStream_t st);
+template <class VT>
+void launch_cg_xr_batch(const CgJob<VT>* jobs, const int* pre, int nj, int nblocks, hipStream_t st);
+void launch_to_f32_batch(const CgJob<double2>* jobs, const int* pre, int nj, int nblocks,
+                         hipStream_t st);
+// batched apply epilogue: y = beta y + alpha scale sum_g partial (per job)
+void launch_epilogue_batch(const CgJob<float2>* jobs, const int* pre, int nj, int nblocks,
+                           hipStream_t st);
 
 }  // namespace msr
