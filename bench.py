@@ -34,6 +34,7 @@ K = 5
 NT = 1000
 NR = 512
 CG_ITERS = 25
+PRECISION = "f64"          # CG precision of the measured solve (FP64 vectors and apply)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -63,24 +64,27 @@ def make_inputs(msr, seed=1002):
     return trj, U, cm, np.ascontiguousarray(data)
 
 
-def apply_bytes(nkm, lam_real):
-    """Algorithmic bytes of one fused apply: 8*(2VK + 2CV + 8CKV) + G*P*s_lam
+def apply_bytes(nkm, lam_real, f64):
+    """Algorithmic bytes of one fused apply: s*(2VK + 8CKV) + 8*2CV + Nkm*P*s_lam
     (b read and x written once, coil maps read twice, the x-FFT and y-FFT
-    intermediates written and read once each, the kernel read once)."""
+    intermediates written and read once each, the kernel read once on its
+    mask), with s = 16 for the ComplexF64 apply (vectors and intermediates in
+    FP64) and 8 for the ComplexF32 one; coil maps and kernel are ComplexF32."""
     V = N_IMG[0] * N_IMG[1]
     P = K * (K + 1) // 2
     s_lam = 4 if lam_real else 8
-    return 8 * (2 * V * K + 2 * COILS * V + 8 * COILS * K * V) + nkm * P * s_lam
+    s = 16 if f64 else 8
+    return s * (2 * V * K + 8 * COILS * K * V) + 8 * 2 * COILS * V + nkm * P * s_lam
 
 
-def read_traffic():
+def read_traffic(key):
     """HBM bytes per apply from the committed rocprofv3 counter summary, if any
-    (line 'HBM_BYTES_PER_APPLY = <int>')."""
+    (line '<key> = <int>', key HBM_BYTES_PER_APPLY_F64 or _F32)."""
     p = os.path.join(ROOT, "profiles", "apply2d_256_c20_k5_counters.md")
     if not os.path.exists(p):
         return None
     for line in open(p):
-        if line.strip().startswith("HBM_BYTES_PER_APPLY"):
+        if line.strip().startswith(key + " "):
             try:
                 return int(line.split("=")[1].strip().split()[0])
             except ValueError:
@@ -136,6 +140,10 @@ def main():
     if args.apply_only:
         xin = torch.randn(nvec, dtype=torch.complex64, device=dev)
         y = torch.empty_like(xin)
+        x64 = xin.to(torch.complex128)
+        y64 = torch.empty_like(x64)
+        for _ in range(args.apply_only):
+            A.A.mul_f64_(y64, x64)
         for _ in range(args.apply_only):
             A.A.mul_(y, xin)
         for _ in range(args.apply_only):
@@ -146,7 +154,8 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"apply_only": args.apply_only, "copy_bytes": 2 * src.numel() * 4,
-                          "apply_bytes": apply_bytes(A.A.nkm, A.A.lambda_real),
+                          "apply_bytes_f64": apply_bytes(A.A.nkm, A.A.lambda_real, True),
+                          "apply_bytes_f32": apply_bytes(A.A.nkm, A.A.lambda_real, False),
                           "nkm": A.A.nkm}), flush=True)
         return
     b = torch.empty((K,) + N_IMG[::-1], dtype=torch.complex64, device=dev)
@@ -155,12 +164,13 @@ def main():
 
     def solve():
         msr.calculate_backprojection(data_d, traj, cml, U=U_d, out=b)
-        msr.cg_(x, A, bflat, maxiter=CG_ITERS, reltol=0.0, initially_zero=True, mode="graph")
+        msr.cg_(x, A, bflat, maxiter=CG_ITERS, reltol=0.0, initially_zero=True, mode="graph",
+                precision=PRECISION)
 
     def solve_plain():
         msr.calculate_backprojection(data_d, traj, cml, U=U_d, out=b)
         msr.cg_(x, A, bflat, maxiter=CG_ITERS, reltol=0.0, initially_zero=True, mode="host",
-                vec_fp64=False)
+                precision="f32")
 
     def timed(fn, steps, warmup):
         for _ in range(warmup):
@@ -190,27 +200,37 @@ def main():
     plain_sps = args.plain_steps / pwall
     log(f"plain: {1e3 * pwall / args.plain_steps:.3f} ms/solve")
 
-    # ---- roofline of the Toeplitz apply, timed with events on its stream
-    y = torch.empty_like(x)
+    # ---- roofline of the Toeplitz apply (the solve's dominant work), timed
+    # with events on the stream the applies are launched on (the current one)
     xin = torch.randn(nvec, dtype=torch.complex64, device=dev)
-    for _ in range(5):
-        A.A.mul_(y, xin)
-    torch.cuda.synchronize()
-    napp = 50
     s_ev, e_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s_ev.record()
-    for _ in range(napp):
-        A.A.mul_(y, xin)
-    e_ev.record()
-    torch.cuda.synchronize()
-    t_apply_ms = s_ev.elapsed_time(e_ev) / napp
-    B = apply_bytes(A.A.nkm, A.A.lambda_real)  # kernel bytes counted on the kmask only
-    achieved = B / (t_apply_ms * 1e-3) / 1e9
-    traffic = read_traffic()
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "Toeplitz apply (3 fused passes + epilogue)",
-                "bytes_per_launch": B, "ms_per_launch": round(t_apply_ms, 4)}
+
+    def time_apply(fn, napp=50):
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        s_ev.record()
+        for _ in range(napp):
+            fn()
+        e_ev.record()
+        torch.cuda.synchronize()
+        return s_ev.elapsed_time(e_ev) / napp
+
+    def roofline_of(t_ms, f64, name):
+        B = apply_bytes(A.A.nkm, A.A.lambda_real, f64)  # kernel bytes counted on the kmask only
+        achieved = B / (t_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic("HBM_BYTES_PER_APPLY_F64" if f64 else "HBM_BYTES_PER_APPLY_F32"),
+                "kernel": name, "bytes_per_launch": B, "ms_per_launch": round(t_ms, 4)}
+
+    x64 = xin.to(torch.complex128)
+    y64 = torch.empty_like(x64)
+    y = torch.empty_like(xin)
+    t64 = time_apply(lambda: A.A.mul_f64_(y64, x64))
+    t32 = time_apply(lambda: A.A.mul_(y, xin))
+    roofline = roofline_of(t64, True, "Toeplitz apply, ComplexF64 (3 fused passes + epilogue)")
+    roofline_f32 = roofline_of(t32, False, "Toeplitz apply, ComplexF32 (3 fused passes + epilogue)")
     # plain (per-coil) apply for comparison
     s_ev.record()
     for _ in range(5):
@@ -230,7 +250,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (complex64 FFT/kernel/gridding; fp64 CG reductions and vectors)",
+        "dtype": "f64 apply and CG vectors (complex128 FFTs / kernel multiply / intermediates); "
+                 "f32 kernel, coil maps, gridding and backprojection",
         "data": "synthetic",
         "config": {"workload": "single 2D recon 256x256 (grid 512x512), 20 coils, 1000 frames x "
                                "1 golden-angle spoke x 512 samples, K=5, BP + 25-iteration CG",
@@ -244,6 +265,7 @@ def main():
                       "apply_ms": round(t_apply_plain, 4)},
         "kernel_build_ms": round(build_ms, 1),
         "roofline": roofline,
+        "roofline_f32_apply": roofline_f32,
         "cpu_baseline": None,
         "rel_err_vs_fp64": None,
     }

@@ -56,6 +56,12 @@ extern "C" {
 #define MSR_CG_HOST 1   /* IterativeSolvers-style: dot/norm read back to the host each iteration */
 #define MSR_CG_GRAPH 2  /* as MSR_CG_DEVICE, iterations captured once into a hipGraph and replayed */
 
+/* CG precision (the `precision` argument of msr_cg / msr_batch_cg*) */
+#define MSR_PREC_F32 0   /* ComplexF32 vectors and apply (the reference's arithmetic) */
+#define MSR_PREC_MIXED 1 /* FP64 vectors, ComplexF32 apply on an FP32 copy of the search vector */
+#define MSR_PREC_F64 2   /* FP64 vectors and ComplexF64 apply (FFTs, kernel multiply and
+                            intermediates in FP64; kernel and coil maps stored as ComplexF32) */
+
 typedef struct msr_ctx msr_ctx;
 typedef struct msr_traj msr_traj;
 typedef struct msr_normalop msr_normalop;
@@ -166,14 +172,20 @@ int msr_forward(msr_ctx *ctx, const msr_traj *traj, int d, const int64_t *img_sh
  * the apply: test/reconstruct_radial_gpu.jl:99, src/CoilMaps.jl:136) run on
  * the device with FP64 inner products, norms and step lengths.  initially_zero
  * = 1 reproduces cg(A, b) (x starts at 0 and is not read).  mode: MSR_CG_*.
- * vec_fp64 = 1 keeps the solution, residual and search vectors in FP64 (the
- * applies still run in FP32 on an FP32 copy of the search vector); 0 keeps
- * them ComplexF32 like the reference.  x and b are ComplexF32 either way.
+ * precision: MSR_PREC_* (MSR_CG_HOST supports MSR_PREC_F32 and
+ * MSR_PREC_MIXED).  x and b are ComplexF32 in every precision.
  * iters_out / residual_out (host, may be NULL) receive the iteration count and
  * final residual norm; reading them synchronises the stream. */
 int msr_cg(msr_normalop *op, void *x, const void *b, int maxiter, double reltol, double abstol,
-           int initially_zero, int mode, int vec_fp64, int *iters_out, double *residual_out,
+           int initially_zero, int mode, int precision, int *iters_out, double *residual_out,
            void *stream);
+
+/* msr_normalop_apply on ComplexF64 vectors with ComplexF64 arithmetic (the
+ * apply of MSR_PREC_F64).  No counterpart in the reference's GPU extension,
+ * which is ComplexF32 only; the reference's CPU mul! is generic in the element
+ * type (src/NFFTNormalOp.jl:227-267). */
+int msr_normalop_apply_f64(msr_normalop *op, void *x, const void *b, double alpha_re,
+                           double alpha_im, double beta_re, double beta_im, void *stream);
 
 /* -------------------------------------------------------------- batches */
 /* Many reconstructions (uneven image size, coil count and K, 2D and 3D)
@@ -196,14 +208,14 @@ int msr_batch_apply(msr_batch *batch, void *const *x, const void *const *b, doub
  * MSR_CG_DEVICE or MSR_CG_GRAPH (the whole solve captured once per pointer
  * set).  iters_out / residual_out: nops entries each, or NULL. */
 int msr_batch_cg(msr_batch *batch, void *const *x, const void *const *b, int maxiter,
-                 double reltol, double abstol, int initially_zero, int mode, int vec_fp64,
+                 double reltol, double abstol, int initially_zero, int mode, int precision,
                  int *iters_out, double *residual_out, void *stream);
 /* The same solve in steps, for schedulers that admit and retire jobs between
  * iterations (continuous batching): start records each operator's CG state on
  * the operator, so a later batch built from any mix of started operators can
  * iterate and finish them.  finish writes the ComplexF32 solutions. */
 int msr_batch_cg_start(msr_batch *batch, void *const *x, const void *const *b, int maxiter,
-                       double reltol, double abstol, int initially_zero, int vec_fp64,
+                       double reltol, double abstol, int initially_zero, int precision,
                        void *stream);
 int msr_batch_cg_iterate(msr_batch *batch, int niter, void *stream);
 int msr_batch_cg_finish(msr_batch *batch, void *stream);

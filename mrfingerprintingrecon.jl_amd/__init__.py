@@ -175,6 +175,18 @@ class _NFFTNormalOp:
                   _stream_ptr(self.device))
         return x
 
+    def mul_f64_(self, x, b, alpha=1.0, beta=0.0):
+        """mul! on ComplexF64 vectors with ComplexF64 arithmetic (the apply of
+        the "f64" CG precision; the kernel and coil maps stay ComplexF32)."""
+        torch = _torch()
+        if x.dtype != torch.complex128 or b.dtype != torch.complex128:
+            raise ArgumentError("mul_f64_ takes complex128 vectors")
+        a = complex(alpha)
+        bt = complex(beta)
+        _lib.call("msr_normalop_apply_f64", self.h, _ptr(x), _ptr(b), a.real, a.imag, bt.real,
+                  bt.imag, _stream_ptr(self.device))
+        return x
+
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib._lib is not None:
             _lib.load().msr_normalop_destroy(self.h)
@@ -400,14 +412,24 @@ def _side_stream(device):
     return _side[device.index]
 
 
+def _prec(precision):
+    try:
+        return _lib.CG_PRECISIONS[precision]
+    except KeyError:
+        raise ArgumentError(f"precision must be one of {sorted(_lib.CG_PRECISIONS)}, "
+                            f"got {precision!r}") from None
+
+
 def cg_(x, A, b, maxiter=None, reltol=REL_TOL_F32, abstol=0.0, initially_zero=False,
-        mode="device", vec_fp64=True, log=False):
+        mode="device", precision="f64", log=False):
     """cg!(x, A, b; maxiter, reltol, abstol) of IterativeSolvers.jl 0.9 on the
     device (FP64 dot products, norms and step lengths).  mode: "device"
     (device-resident scalars), "graph" (hipGraph-captured iterations) or
     "host" (scalars read back every iteration, per-coil applies: the plain
-    reference-order GPU path).  vec_fp64 keeps x, r and u in FP64 on the device
-    (the applies stay FP32); False keeps them ComplexF32 like the reference."""
+    reference-order GPU path).  precision: "f64" (FP64 vectors and a
+    ComplexF64 apply), "mixed" (FP64 vectors, ComplexF32 apply) or "f32"
+    (ComplexF32 vectors and apply, the reference's arithmetic).  x and b are
+    ComplexF32 in every precision."""
     op = A.A if isinstance(A, LinearOperator) else A
     if maxiter is None:
         maxiter = A.ncol if isinstance(A, LinearOperator) else math.prod(op.shape) * op.Ncoeff
@@ -423,7 +445,7 @@ def cg_(x, A, b, maxiter=None, reltol=REL_TOL_F32, abstol=0.0, initially_zero=Fa
         side.wait_stream(cur)
     st = ctypes.c_void_p((side or cur).cuda_stream)
     _lib.call("msr_cg", op.h, _ptr(x), _ptr(b), int(maxiter), float(reltol), float(abstol),
-              int(bool(initially_zero)), _lib.CG_MODES[mode], int(bool(vec_fp64)),
+              int(bool(initially_zero)), _lib.CG_MODES[mode], _prec(precision),
               ctypes.byref(it), ctypes.byref(res), st)
     if side is not None:
         cur.wait_stream(side)
@@ -486,7 +508,7 @@ class NFFTNormalOpBatch:
         return xs
 
     def cg_(self, xs, bs, maxiter, reltol=None, abstol=0.0, initially_zero=False,
-            mode="device", vec_fp64=True, log=False):
+            mode="device", precision="f64", log=False):
         """cg!(x_j, A_j, b_j) for every operator in lockstep (modes "device"
         and "graph"); see cg_ for the recurrence and the precision."""
         reltol = REL_TOL_F32 if reltol is None else reltol
@@ -502,7 +524,7 @@ class NFFTNormalOpBatch:
         res = (ctypes.c_double * n)()
         _lib.call("msr_batch_cg", self.h, self._ptrs(xs, "x"), self._ptrs(bs, "b"), int(maxiter),
                   float(reltol), float(abstol), int(bool(initially_zero)), _lib.CG_MODES[mode],
-                  int(bool(vec_fp64)), its if log else None, res if log else None, st)
+                  _prec(precision), its if log else None, res if log else None, st)
         if side is not None:
             cur.wait_stream(side)
         if log:
@@ -511,11 +533,11 @@ class NFFTNormalOpBatch:
 
     # stepwise solve (continuous batching)
     def cg_start(self, xs, bs, maxiter, reltol=None, abstol=0.0, initially_zero=True,
-                 vec_fp64=True, stream=None):
+                 precision="f64", stream=None):
         reltol = REL_TOL_F32 if reltol is None else reltol
         _lib.call("msr_batch_cg_start", self.h, self._ptrs(xs, "x"), self._ptrs(bs, "b"),
                   int(maxiter), float(reltol), float(abstol), int(bool(initially_zero)),
-                  int(bool(vec_fp64)), self._st(stream))
+                  _prec(precision), self._st(stream))
 
     def cg_iterate(self, niter, stream=None):
         _lib.call("msr_batch_cg_iterate", self.h, int(niter), self._st(stream))

@@ -20,6 +20,7 @@ MSR_ERR_HIP = 6
 DCF_MODES = {"none": 0, "radial_2D": 1, "radial_3D": 2}
 DCF_ARRAY = 3
 CG_MODES = {"device": 0, "host": 1, "graph": 2}
+CG_PRECISIONS = {"f32": 0, "mixed": 1, "f64": 2}
 
 
 class ArgumentError(ValueError):
@@ -66,6 +67,7 @@ SIGNATURES = {
     "msr_backprojection": (_I, [_P, _P, _I, _P, _P, _I, _I, _P, _I, _P, _I, _P, _P, _P]),
     "msr_forward": (_I, [_P, _P, _I, _P, _P, _I, _P, _I, _I, _P, _P, _P]),
     "msr_cg": (_I, [_P, _P, _P, _I, _D, _D, _I, _I, _I, _P, _P, _P]),
+    "msr_normalop_apply_f64": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),
     "msr_batch_create": (_I, [_P, _I, _P, _P]),
     "msr_batch_destroy": (_I, [_P]),
     "msr_batch_apply": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),

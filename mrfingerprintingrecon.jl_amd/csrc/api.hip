@@ -253,8 +253,22 @@ int msr_forward(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* img_sh
     });
 }
 
+int msr_normalop_apply_f64(msr_normalop* op, void* x, const void* b, double alpha_re,
+                           double alpha_im, double beta_re, double beta_im, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        MSR_NOTNULL(x);
+        MSR_NOTNULL(b);
+        MSR_REQUIRE(x != b, MSR_ERR_ARGUMENT, "x and b must not alias");
+        ensure_device(op->ctx);
+        op_apply_f64(op, (double2*)x, (const double2*)b, make_double2(alpha_re, alpha_im),
+                     make_double2(beta_re, beta_im), beta_re == 0.0 && beta_im == 0.0,
+                     (hipStream_t)stream);
+    });
+}
+
 int msr_cg(msr_normalop* op, void* x, const void* b, int maxiter, double reltol, double abstol,
-           int initially_zero, int mode, int vec_fp64, int* iters_out, double* residual_out,
+           int initially_zero, int mode, int precision, int* iters_out, double* residual_out,
            void* stream) {
     return guarded([&] {
         MSR_NOTNULL(op);
@@ -264,7 +278,7 @@ int msr_cg(msr_normalop* op, void* x, const void* b, int maxiter, double reltol,
         MSR_REQUIRE(maxiter >= 0, MSR_ERR_ARGUMENT, "maxiter = %d", maxiter);
         ensure_device(op->ctx);
         op_cg(op, (float2*)x, (const float2*)b, maxiter, reltol, abstol, initially_zero != 0, mode,
-              vec_fp64 != 0, iters_out, residual_out, (hipStream_t)stream);
+              precision, iters_out, residual_out, (hipStream_t)stream);
     });
 }
 
@@ -298,26 +312,26 @@ int msr_batch_apply(msr_batch* batch, void* const* x, const void* const* b, doub
 }
 
 int msr_batch_cg(msr_batch* batch, void* const* x, const void* const* b, int maxiter,
-                 double reltol, double abstol, int initially_zero, int mode, int vec_fp64,
+                 double reltol, double abstol, int initially_zero, int mode, int precision,
                  int* iters_out, double* residual_out, void* stream) {
     return guarded([&] {
         MSR_NOTNULL(batch);
         ensure_device(batch->ctx);
         batch_cg(batch, (float2* const*)x, (const float2* const*)b, maxiter, reltol, abstol,
-                 initially_zero != 0, mode, vec_fp64 != 0, (hipStream_t)stream);
+                 initially_zero != 0, mode, precision, (hipStream_t)stream);
         if (iters_out || residual_out)
             batch_cg_status(batch, iters_out, residual_out, nullptr, (hipStream_t)stream);
     });
 }
 
 int msr_batch_cg_start(msr_batch* batch, void* const* x, const void* const* b, int maxiter,
-                       double reltol, double abstol, int initially_zero, int vec_fp64,
+                       double reltol, double abstol, int initially_zero, int precision,
                        void* stream) {
     return guarded([&] {
         MSR_NOTNULL(batch);
         ensure_device(batch->ctx);
         batch_cg_start(batch, (float2* const*)x, (const float2* const*)b, maxiter, reltol, abstol,
-                       initially_zero != 0, vec_fp64 != 0, (hipStream_t)stream);
+                       initially_zero != 0, precision, (hipStream_t)stream);
     });
 }
 

@@ -1,130 +1,134 @@
 // Fused 2D Toeplitz subspace normal operator: passes A and C plus dispatch.
-// Pass B (the kernel multiply) lives in apply2d_b.hpp.
+// Pass B (the kernel multiply) lives in apply2d_b.hpp.  Every pass is
+// templated on the complex arithmetic type CT (float2 or double2).
 #include "apply2d.hpp"
 #include "apply2d_b.hpp"
 #include "fft.hpp"
 
 namespace msr {
 
-template <int NX2>
+template <int NX2, class CT>
 constexpr int rowpass_block() {
-    return (RT2 * FFTPlan<NX2>::T) < 256 ? RT2 * FFTPlan<NX2>::T : 256;
+    return (rows_per_tile<CT>() * FFTPlan<NX2>::T) < 256 ? rows_per_tile<CT>() * FFTPlan<NX2>::T
+                                                         : 256;
 }
 
 // ----------------------------------------------------------------- pass A
 // block lb = (tile, k, c), tile fastest
-template <int NX2>
+template <int NX2, class CT>
 __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     if (a.skip && *a.skip) return;
+    constexpr int RT = rows_per_tile<CT>();
     constexpr int TP = FFTPlan<NX2>::T;
-    constexpr int BLK = rowpass_block<NX2>();
+    constexpr int BLK = rowpass_block<NX2, CT>();
     constexpr int F = BLK / TP;              // rows transformed concurrently
     constexpr int LD = lds_line<NX2>() + 1;  // LDS row pitch (odd: transposed access)
-    __shared__ float2 s_tw[NX2];
-    __shared__ float2 s_rows[RT2 * LD];
+    __shared__ CT s_tw[NX2];
+    __shared__ CT s_rows[RT * LD];
     const int nx = a.nx, ny = a.ny, K = a.K;
-    const int tiles = tiles2d(ny);
+    const int tiles = tiles2d(ny, sizeof(CT) == 16);
     const int tile = (int)(lb % tiles);
     const int k = (int)((lb / tiles) % K);
     const int c = (int)(lb / ((int64_t)tiles * K));
     const int tid = threadIdx.x;
-    const int y0 = tile * RT2;
-    load_twiddles<NX2>(s_tw, a.tw_x);
-    const float2* bk = a.b + (size_t)k * nx * ny;
+    const int y0 = tile * RT;
+    load_twiddles<NX2>(s_tw, static_cast<const CT*>(a.tw_x));
+    const CT* bk = static_cast<const CT*>(a.b) + (size_t)k * nx * ny;
     const float2* cm = a.cmaps ? a.cmaps + (size_t)c * nx * ny : nullptr;
-    // load RT2 rows, multiply by the coil map, zero-pad to NX2
-    for (int e = tid; e < RT2 * NX2; e += BLK) {
+    // load RT rows, multiply by the coil map, zero-pad to NX2
+    for (int e = tid; e < RT * NX2; e += BLK) {
         const int yl = e / NX2, xx = e - yl * NX2;
         const int y = y0 + yl;
-        float2 v = make_float2(0.f, 0.f);
+        CT v = cz<CT>();
         if (xx < nx && y < ny) {
             v = bk[(size_t)y * nx + xx];
-            if (cm) v = cmul(cm[(size_t)y * nx + xx], v);
+            if (cm) v = cmul(cto<CT>(cm[(size_t)y * nx + xx]), v);
         }
         s_rows[yl * LD + lds_pad(xx)] = v;
     }
     __syncthreads();
     const int g = tid / TP, t = tid - g * TP;
 #pragma unroll
-    for (int r0 = 0; r0 < RT2; r0 += F) fft_lds<NX2, -1>(s_rows + (r0 + g) * LD, s_tw, t);
+    for (int r0 = 0; r0 < RT; r0 += F) fft_lds<NX2, -1>(s_rows + (r0 + g) * LD, s_tw, t);
     // transposed write: T[c][k][qx][y], y fastest
-    float2* Tk = a.T + ((size_t)c * K + k) * NX2 * ny;
-    for (int e = tid; e < RT2 * NX2; e += BLK) {
-        const int yl = e % RT2, qx = e / RT2;
+    CT* Tk = static_cast<CT*>(a.T) + ((size_t)c * K + k) * NX2 * ny;
+    for (int e = tid; e < RT * NX2; e += BLK) {
+        const int yl = e % RT, qx = e / RT;
         const int y = y0 + yl;
         if (y < ny) Tk[(size_t)qx * ny + y] = s_rows[yl * LD + lds_pad(qx)];
     }
 }
 
-template <int NX2>
+template <int NX2, class CT>
 __global__ void __launch_bounds__(256) pass_a_2d(Apply2DArgs a) {
-    pass_a_body<NX2>(a, blockIdx.x);
+    pass_a_body<NX2, CT>(a, blockIdx.x);
 }
 
-template <int NX2>
+template <int NX2, class CT>
 __global__ void __launch_bounds__(256)
 pass_a_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pre, int nj) {
     const int j = batch_job(pre, nj, blockIdx.x);
-    pass_a_body<NX2>(jobs[j], (int64_t)blockIdx.x - pre[j]);
+    pass_a_body<NX2, CT>(jobs[j], (int64_t)blockIdx.x - pre[j]);
 }
 
 // ----------------------------------------------------------------- pass C
 // block lb = (tile, k, coil group), tile fastest
-template <int NX2>
+template <int NX2, class CT>
 __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     if (a.skip && *a.skip) return;
+    constexpr int RT = rows_per_tile<CT>();
     constexpr int TP = FFTPlan<NX2>::T;
-    constexpr int BLK = rowpass_block<NX2>();
+    constexpr int BLK = rowpass_block<NX2, CT>();
     constexpr int F = BLK / TP;
     constexpr int LD = lds_line<NX2>() + 1;
     constexpr int NXC = NX2 / 2;                 // = nx
-    constexpr int EA = (RT2 * NXC + BLK - 1) / BLK;
-    __shared__ float2 s_tw[NX2];
-    __shared__ float2 s_rows[RT2 * LD];
+    constexpr int EA = (RT * NXC + BLK - 1) / BLK;
+    __shared__ CT s_tw[NX2];
+    __shared__ CT s_rows[RT * LD];
     const int nx = a.nx, ny = a.ny, K = a.K, C = a.C, G = a.G;
-    const int tiles = tiles2d(ny);
+    const int tiles = tiles2d(ny, sizeof(CT) == 16);
     const int tile = (int)(lb % tiles);
     const int k = (int)((lb / tiles) % K);
     const int grp = (int)(lb / ((int64_t)tiles * K));
     const int tid = threadIdx.x;
-    const int y0 = tile * RT2;
+    const int y0 = tile * RT;
     const int c0 = (int)((int64_t)C * grp / G), c1 = (int)((int64_t)C * (grp + 1) / G);
-    load_twiddles<NX2>(s_tw, a.tw_x);
-    float2 acc[EA];
+    load_twiddles<NX2>(s_tw, static_cast<const CT*>(a.tw_x));
+    CT acc[EA];
 #pragma unroll
-    for (int i = 0; i < EA; ++i) acc[i] = make_float2(0.f, 0.f);
+    for (int i = 0; i < EA; ++i) acc[i] = cz<CT>();
     const int g = tid / TP, t = tid - g * TP;
     for (int c = c0; c < c1; ++c) {
-        const float2* Tk = a.T + ((size_t)c * K + k) * NX2 * ny;
-        for (int e = tid; e < RT2 * NX2; e += BLK) {
-            const int yl = e % RT2, qx = e / RT2;
+        const CT* Tk = static_cast<const CT*>(a.T) + ((size_t)c * K + k) * NX2 * ny;
+        for (int e = tid; e < RT * NX2; e += BLK) {
+            const int yl = e % RT, qx = e / RT;
             const int y = y0 + yl;
-            s_rows[yl * LD + lds_pad(qx)] = (y < ny) ? Tk[(size_t)qx * ny + y] : make_float2(0.f, 0.f);
+            s_rows[yl * LD + lds_pad(qx)] = (y < ny) ? Tk[(size_t)qx * ny + y] : cz<CT>();
         }
         __syncthreads();
 #pragma unroll
-        for (int r0 = 0; r0 < RT2; r0 += F) fft_lds<NX2, +1>(s_rows + (r0 + g) * LD, s_tw, t);
+        for (int r0 = 0; r0 < RT; r0 += F) fft_lds<NX2, +1>(s_rows + (r0 + g) * LD, s_tw, t);
         const float2* cm = a.cmaps ? a.cmaps + (size_t)c * nx * ny : nullptr;
 #pragma unroll
         for (int i = 0; i < EA; ++i) {
             const int e = tid + i * BLK;
-            if (e < RT2 * NXC) {
+            if (e < RT * NXC) {
                 const int yl = e / NXC, xx = e - yl * NXC;
                 const int y = y0 + yl;
                 if (y < ny) {
-                    float2 v = s_rows[yl * LD + lds_pad(xx)];
-                    if (cm) cfmac(acc[i], cm[(size_t)y * nx + xx], v);
+                    CT v = s_rows[yl * LD + lds_pad(xx)];
+                    if (cm) cfmac(acc[i], cto<CT>(cm[(size_t)y * nx + xx]), v);
                     else acc[i] = cadd(acc[i], v);
                 }
             }
         }
         __syncthreads();
     }
-    float2* pk = a.partial + ((size_t)grp * K + k) * nx * ny;
+    CT* pk = static_cast<CT*>(a.partial) + ((size_t)grp * K + k) * nx * ny;
 #pragma unroll
     for (int i = 0; i < EA; ++i) {
         const int e = tid + i * BLK;
-        if (e < RT2 * NXC) {
+        if (e < RT * NXC) {
             const int yl = e / NXC, xx = e - yl * NXC;
             const int y = y0 + yl;
             if (y < ny) pk[(size_t)y * nx + xx] = acc[i];
@@ -132,16 +136,16 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     }
 }
 
-template <int NX2>
+template <int NX2, class CT>
 __global__ void __launch_bounds__(256) pass_c_2d(Apply2DArgs a) {
-    pass_c_body<NX2>(a, blockIdx.x);
+    pass_c_body<NX2, CT>(a, blockIdx.x);
 }
 
-template <int NX2>
+template <int NX2, class CT>
 __global__ void __launch_bounds__(256)
 pass_c_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pre, int nj) {
     const int j = batch_job(pre, nj, blockIdx.x);
-    pass_c_body<NX2>(jobs[j], (int64_t)blockIdx.x - pre[j]);
+    pass_c_body<NX2, CT>(jobs[j], (int64_t)blockIdx.x - pre[j]);
 }
 
 // ----------------------------------------------------------------- dispatch
@@ -151,62 +155,99 @@ static unsigned grid1d(int64_t nblocks) {
     return (unsigned)nblocks;
 }
 
-void launch_apply2d_passes(const Apply2DArgs& a, hipStream_t st) {
+// the row passes need 2Nx <= 512 (LDS); pass B is limited the same way
+template <int N>
+constexpr bool row_len_ok() { return N <= 512; }
+
+template <class CT>
+static void launch_a(const Apply2DArgs& a, hipStream_t st) {
     const int NX2 = 2 * a.nx;
     bool ok = dispatch_fft_len(NX2, [&]<int N>() {
-        hipLaunchKernelGGL(pass_a_2d<N>, dim3(grid1d(blocks_a2d(a))), dim3(rowpass_block<N>()), 0,
-                           st, a);
+        if constexpr (row_len_ok<N>())
+            hipLaunchKernelGGL((pass_a_2d<N, CT>), dim3(grid1d(blocks_a2d(a))),
+                               dim3(rowpass_block<N, CT>()), 0, st, a);
+        else
+            fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Nx = %d above 512", N);
     });
     MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", NX2);
     MSR_CHECK_LAUNCH();
+}
+
+template <class CT>
+static void launch_c(const Apply2DArgs& a, hipStream_t st) {
+    dispatch_fft_len(2 * a.nx, [&]<int N>() {
+        if constexpr (row_len_ok<N>())
+            hipLaunchKernelGGL((pass_c_2d<N, CT>), dim3(grid1d(blocks_c2d(a))),
+                               dim3(rowpass_block<N, CT>()), 0, st, a);
+    });
+    MSR_CHECK_LAUNCH();
+}
+
+void launch_apply2d_passes(const Apply2DArgs& a, hipStream_t st) {
+    if (a.f64) launch_a<double2>(a, st);
+    else launch_a<float2>(a, st);
     launch_pass_b_2d(a, st);
-    dispatch_fft_len(NX2, [&]<int N>() {
-        hipLaunchKernelGGL(pass_c_2d<N>, dim3(grid1d(blocks_c2d(a))), dim3(rowpass_block<N>()), 0,
-                           st, a);
-    });
-    MSR_CHECK_LAUNCH();
+    if (a.f64) launch_c<double2>(a, st);
+    else launch_c<float2>(a, st);
 }
 
-void launch_pass_a_2d_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
-                            int64_t nblocks, hipStream_t st) {
-    bool ok = dispatch_fft_len(nx2, [&]<int N>() {
-        hipLaunchKernelGGL(pass_a_2d_batch<N>, dim3(grid1d(nblocks)), dim3(rowpass_block<N>()), 0,
-                           st, jobs, pre, nj);
-    });
-    MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", nx2);
-    MSR_CHECK_LAUNCH();
-}
-
-int pass_b_colblocks(int nx2, int ny2, int K) {
+int pass_b_colblocks(int nx2, int ny2, int K, int f64) {
     int r = 0;
-#define MSR_CALLB(KK) r = pass_b_2d_colblocks_k##KK(nx2, ny2)
+#define MSR_CALLB(KK) r = pass_b_2d_colblocks_k##KK(nx2, ny2, f64)
     MSR_PASSB_SWITCH(K, MSR_CALLB)
 #undef MSR_CALLB
     return r;
 }
 
-void launch_pass_b_2d_batch(int ny2, int K, int lam_real, const Apply2DArgs* jobs,
+void launch_pass_b_2d_batch(int ny2, int K, int lam_real, int f64, const Apply2DArgs* jobs,
                             const int* pre, int nj, int64_t nblocks, hipStream_t st) {
-    // the template arguments come from (ny2, K, lam_real); the job table itself
-    // stays on the device
+    // the template arguments come from (ny2, K, lam_real, f64); the job table
+    // itself stays on the device
     Apply2DArgs shape{};
     shape.ny = ny2 / 2;
     shape.K = K;
     shape.lam_real = lam_real;
+    shape.f64 = f64;
 #define MSR_CALLB(KK) pass_b_2d_k##KK(shape, jobs, pre, nj, nblocks, st)
     MSR_PASSB_SWITCH(K, MSR_CALLB)
 #undef MSR_CALLB
     MSR_CHECK_LAUNCH();
 }
 
-void launch_pass_c_2d_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
-                            int64_t nblocks, hipStream_t st) {
+template <class CT>
+static void launch_a_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
+                           int64_t nblocks, hipStream_t st) {
     bool ok = dispatch_fft_len(nx2, [&]<int N>() {
-        hipLaunchKernelGGL(pass_c_2d_batch<N>, dim3(grid1d(nblocks)), dim3(rowpass_block<N>()), 0,
-                           st, jobs, pre, nj);
+        if constexpr (row_len_ok<N>())
+            hipLaunchKernelGGL((pass_a_2d_batch<N, CT>), dim3(grid1d(nblocks)),
+                               dim3(rowpass_block<N, CT>()), 0, st, jobs, pre, nj);
     });
-    MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", nx2);
+    MSR_REQUIRE(ok && nx2 <= 512, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", nx2);
     MSR_CHECK_LAUNCH();
+}
+
+template <class CT>
+static void launch_c_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
+                           int64_t nblocks, hipStream_t st) {
+    bool ok = dispatch_fft_len(nx2, [&]<int N>() {
+        if constexpr (row_len_ok<N>())
+            hipLaunchKernelGGL((pass_c_2d_batch<N, CT>), dim3(grid1d(nblocks)),
+                               dim3(rowpass_block<N, CT>()), 0, st, jobs, pre, nj);
+    });
+    MSR_REQUIRE(ok && nx2 <= 512, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", nx2);
+    MSR_CHECK_LAUNCH();
+}
+
+void launch_pass_a_2d_batch(int nx2, int f64, const Apply2DArgs* jobs, const int* pre, int nj,
+                            int64_t nblocks, hipStream_t st) {
+    if (f64) launch_a_batch<double2>(nx2, jobs, pre, nj, nblocks, st);
+    else launch_a_batch<float2>(nx2, jobs, pre, nj, nblocks, st);
+}
+
+void launch_pass_c_2d_batch(int nx2, int f64, const Apply2DArgs* jobs, const int* pre, int nj,
+                            int64_t nblocks, hipStream_t st) {
+    if (f64) launch_c_batch<double2>(nx2, jobs, pre, nj, nblocks, st);
+    else launch_c_batch<float2>(nx2, jobs, pre, nj, nblocks, st);
 }
 
 }  // namespace msr

@@ -30,32 +30,42 @@
 
 namespace msr {
 
+// Arithmetic and intermediates are ComplexF32 (f64 = 0) or ComplexF64
+// (f64 = 1); the kernel and the coil maps are ComplexF32 in both cases
+// (promoted on load).  b, T, partial and the twiddles have the element type of
+// the arithmetic.
 struct Apply2DArgs {
     int nx, ny;           // image size
     int K, C;             // coefficients, coils (C >= 1)
-    const float2* b;      // input (K, ny, nx)
+    int f64;              // arithmetic / intermediate precision
+    const void* b;        // input (K, ny, nx)
     const float2* cmaps;  // (C, ny, nx) or nullptr for the identity map
     const void* lam;      // dense kernel (2nx, P, 2ny), float2 or float
     int lam_real;
-    float2* T;            // scratch (C, K, 2nx, ny)
-    float2* partial;      // scratch (G, K, ny, nx)
+    void* T;              // scratch (C, K, 2nx, ny)
+    void* partial;        // scratch (G, K, ny, nx)
     int G;                // coil groups of pass C (>= 1)
     int Gb;               // coil groups of pass B (>= 1)
-    const float2* tw_x;   // twiddles for 2nx
-    const float2* tw_y;   // twiddles for 2ny
+    const void* tw_x;     // twiddles for 2nx
+    const void* tw_y;     // twiddles for 2ny
     const int* skip;      // when non-null and *skip != 0 the passes do nothing
 };
 
-constexpr int RT2 = 16;  // rows per tile in passes A and C (16 * 8 B = 128 B segments)
-
-__host__ __device__ inline int tiles2d(int ny) { return (ny + RT2 - 1) / RT2; }
-int pass_b_colblocks(int nx2, int ny2, int K);  // column blocks of pass B per coil group
+// rows per tile in passes A and C: 128-byte segments of the transposed
+// intermediate (16 ComplexF32 or 8 ComplexF64 rows)
+template <class CT>
+constexpr int rows_per_tile() { return 128 / (int)sizeof(CT); }
+__host__ __device__ inline int tiles2d(int ny, int f64) {
+    const int rt = f64 ? 8 : 16;
+    return (ny + rt - 1) / rt;
+}
+int pass_b_colblocks(int nx2, int ny2, int K, int f64);  // column blocks of pass B per coil group
 
 // block counts per operator of each pass
-inline int64_t blocks_a2d(const Apply2DArgs& a) { return (int64_t)tiles2d(a.ny) * a.K * a.C; }
-inline int64_t blocks_c2d(const Apply2DArgs& a) { return (int64_t)tiles2d(a.ny) * a.K * a.G; }
+inline int64_t blocks_a2d(const Apply2DArgs& a) { return (int64_t)tiles2d(a.ny, a.f64) * a.K * a.C; }
+inline int64_t blocks_c2d(const Apply2DArgs& a) { return (int64_t)tiles2d(a.ny, a.f64) * a.K * a.G; }
 inline int64_t blocks_b2d(const Apply2DArgs& a) {
-    return (int64_t)pass_b_colblocks(2 * a.nx, 2 * a.ny, a.K) * a.Gb;
+    return (int64_t)pass_b_colblocks(2 * a.nx, 2 * a.ny, a.K, a.f64) * a.Gb;
 }
 
 // Launches passes A, B, C of one operator.  After this, partial[g] holds the
@@ -63,14 +73,15 @@ inline int64_t blocks_b2d(const Apply2DArgs& a) {
 // 1/(4 nx ny) normalisation.
 void launch_apply2d_passes(const Apply2DArgs& a, hipStream_t st);
 
-// Batched form.  The operators of one launch share 2nx (passes A, C) or
-// (2ny, K, lam_real) (pass B); `jobs` is a device array of nj operators and
-// `pre` a device array of nj + 1 block offsets (pre[0] = 0) for that pass.
-void launch_pass_a_2d_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
+// Batched form.  The operators of one launch share the precision and 2nx
+// (passes A, C) or (2ny, K, lam_real) (pass B); `jobs` is a device array of nj
+// operators and `pre` a device array of nj + 1 block offsets (pre[0] = 0) for
+// that pass.
+void launch_pass_a_2d_batch(int nx2, int f64, const Apply2DArgs* jobs, const int* pre, int nj,
                             int64_t nblocks, hipStream_t st);
-void launch_pass_b_2d_batch(int ny2, int K, int lam_real, const Apply2DArgs* jobs,
+void launch_pass_b_2d_batch(int ny2, int K, int lam_real, int f64, const Apply2DArgs* jobs,
                             const int* pre, int nj, int64_t nblocks, hipStream_t st);
-void launch_pass_c_2d_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
+void launch_pass_c_2d_batch(int nx2, int f64, const Apply2DArgs* jobs, const int* pre, int nj,
                             int64_t nblocks, hipStream_t st);
 
 // Operator of block `blk` in a batched launch: the j with pre[j] <= blk < pre[j+1].

@@ -12,36 +12,46 @@ namespace msr {
 // (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:239, 1-based j + k(k-1)/2).
 __host__ __device__ constexpr int packed_index(int j, int k) { return j + k * (k + 1) / 2; }
 
-template <int NY2, int K>
+// Workgroup shape: WC grid columns, each with K lines of NY2 (FFT by
+// TP = FFTPlan<NY2>::T threads per line), and at least one thread per voxel of
+// the WC columns, so every thread holds the kernel of one voxel in registers
+// (threads beyond the K*WC*TP line threads idle during the FFT stages).
+template <int NY2, int K, class CT>
 struct PassBShape {
     static constexpr int TP = FFTPlan<NY2>::T;
-    static constexpr int WC0 = (256 / (K * TP)) > 1 ? 256 / (K * TP) : 1;
-    static constexpr int WC = WC0 < 16 ? WC0 : 16;  // columns per WG
-    static constexpr int BLK = WC * K * TP;
+    static constexpr int PER = (K * TP > NY2) ? K * TP : NY2;  // threads per column
+    static constexpr int WC0 = (512 / PER) > 1 ? 512 / PER : 1;
+    static constexpr int WCM = sizeof(CT) == 16 ? 8 : 16;  // LDS: 8 ComplexF64 columns
+    static constexpr int WC = WC0 < WCM ? WC0 : WCM;        // columns per WG
+    static constexpr int FT = WC * K * TP;                  // FFT threads
+    static constexpr int BLK = WC * PER;
     static constexpr int NV = WC * NY2;                // voxels per WG
-    static constexpr int EV = (NV + BLK - 1) / BLK;    // voxels per thread
+    static constexpr int EV = (NV + BLK - 1) / BLK;    // voxels per thread (1)
     static constexpr int P = K * (K + 1) / 2;
 };
 
-// block lb = (column block, coil group), column block fastest
-template <int NY2, int K, bool LREAL>
+// block lb = (column block, coil group), column block fastest.  The kernel
+// stays ComplexF32 (or Float32) and is promoted to the arithmetic type CT.
+template <int NY2, int K, bool LREAL, class CT>
 __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     if (a.skip && *a.skip) return;
-    using S = PassBShape<NY2, K>;
+    using S = PassBShape<NY2, K, CT>;
     using LT = typename std::conditional<LREAL, float, float2>::type;
+    using RT = real_t<CT>;
     constexpr int TP = S::TP, WC = S::WC, BLK = S::BLK, NV = S::NV, EV = S::EV, P = S::P;
+    constexpr int FT = S::FT;
     constexpr int LL = lds_line<NY2>();
-    __shared__ float2 s_tw[NY2];
-    __shared__ float2 s_buf[WC * K * LL];
+    __shared__ CT s_tw[NY2];
+    __shared__ CT s_buf[WC * K * LL];
     const LT* lam = reinterpret_cast<const LT*>(a.lam);
-    float2* T = a.T;
+    CT* T = static_cast<CT*>(a.T);
     const int nx2 = 2 * a.nx, ny = a.ny, C = a.C, Gb = a.Gb;
     const int ncb = (nx2 + WC - 1) / WC;
     const int tid = threadIdx.x;
     const int qx0 = (int)(lb % ncb) * WC;
     const int grp = (int)(lb / ncb);
     const int c0 = (int)((int64_t)C * grp / Gb), c1 = (int)((int64_t)C * (grp + 1) / Gb);
-    load_twiddles<NY2>(s_tw, a.tw_y);
+    load_twiddles<NY2>(s_tw, static_cast<const CT*>(a.tw_y));
 
     // kernel values of the voxels this thread owns, kept for all coils
     LT lr[EV][P];
@@ -60,40 +70,41 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     }
 
     const int g = tid / TP, t = tid - g * TP;
+    const bool fft_thread = tid < FT;
     for (int c = c0; c < c1; ++c) {
         // load K columns per owned qx, zero-padded from ny to NY2
         for (int el = tid; el < WC * K * NY2; el += BLK) {
             const int line = el / NY2, i = el - line * NY2;
             const int wc = line / K, k = line - wc * K;
             const int qx = qx0 + wc;
-            float2 v = make_float2(0.f, 0.f);
+            CT v = cz<CT>();
             if (i < ny && qx < nx2) v = T[(((size_t)c * K + k) * nx2 + qx) * ny + i];
             s_buf[line * LL + lds_pad(i)] = v;
         }
         __syncthreads();
-        fft_lds<NY2, -1>(s_buf + g * LL, s_tw, t);
+        fft_lds<NY2, -1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
         // per-voxel Hermitian (or real symmetric) K x K multiply
 #pragma unroll
         for (int e = 0; e < EV; ++e) {
             const int v = tid + e * BLK;
             if (v < NV) {
                 const int wc = v / NY2, qy = v - wc * NY2;
-                float2* col = s_buf + (size_t)wc * K * LL + lds_pad(qy);
-                float2 in[K];
+                CT* col = s_buf + (size_t)wc * K * LL + lds_pad(qy);
+                CT in[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) in[k] = col[k * LL];
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    float2 acc = make_float2(0.f, 0.f);
+                    CT acc = cz<CT>();
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
                         if constexpr (LREAL) {
-                            const float l = lr[e][j <= k ? packed_index(j, k) : packed_index(k, j)];
-                            acc.x = fmaf(l, in[k].x, acc.x);
-                            acc.y = fmaf(l, in[k].y, acc.y);
+                            const RT l = (RT)lr[e][j <= k ? packed_index(j, k) : packed_index(k, j)];
+                            acc.x = fma(l, in[k].x, acc.x);
+                            acc.y = fma(l, in[k].y, acc.y);
                         } else {
-                            if (j <= k) cfma(acc, lr[e][packed_index(j, k)], in[k]);
-                            else cfmac(acc, lr[e][packed_index(k, j)], in[k]);
+                            if (j <= k) cfma(acc, cto<CT>(lr[e][packed_index(j, k)]), in[k]);
+                            else cfmac(acc, cto<CT>(lr[e][packed_index(k, j)]), in[k]);
                         }
                     }
                     col[j * LL] = acc;
@@ -101,7 +112,7 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
             }
         }
         __syncthreads();
-        fft_lds<NY2, +1>(s_buf + g * LL, s_tw, t);
+        fft_lds<NY2, +1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
         for (int el = tid; el < WC * K * NY2; el += BLK) {
             const int line = el / NY2, i = el - line * NY2;
             const int wc = line / K, k = line - wc * K;
@@ -113,16 +124,28 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     }
 }
 
-template <int NY2, int K, bool LREAL>
+template <int NY2, int K, bool LREAL, class CT>
 __global__ void __launch_bounds__(512) pass_b_2d(Apply2DArgs a) {
-    pass_b_body<NY2, K, LREAL>(a, blockIdx.x);
+    pass_b_body<NY2, K, LREAL, CT>(a, blockIdx.x);
 }
 
-template <int NY2, int K, bool LREAL>
+template <int NY2, int K, bool LREAL, class CT>
 __global__ void __launch_bounds__(512)
 pass_b_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pre, int nj) {
     const int j = batch_job(pre, nj, blockIdx.x);
-    pass_b_body<NY2, K, LREAL>(jobs[j], (int64_t)blockIdx.x - pre[j]);
+    pass_b_body<NY2, K, LREAL, CT>(jobs[j], (int64_t)blockIdx.x - pre[j]);
+}
+
+template <int NY2, int K, bool LREAL, class CT>
+void launch_pass_b_t(const Apply2DArgs& a, const Apply2DArgs* jobs, const int* pre, int nj,
+                     int64_t nblocks, hipStream_t st) {
+    using S = PassBShape<NY2, K, CT>;
+    dim3 grid((unsigned)nblocks);
+    if (jobs)
+        hipLaunchKernelGGL((pass_b_2d_batch<NY2, K, LREAL, CT>), grid, dim3(S::BLK), 0, st, jobs,
+                           pre, nj);
+    else
+        hipLaunchKernelGGL((pass_b_2d<NY2, K, LREAL, CT>), grid, dim3(S::BLK), 0, st, a);
 }
 
 // per-K entry points (defined in apply2d_b_k*.hip); jobs == nullptr means the
@@ -130,7 +153,7 @@ pass_b_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pr
 #define MSR_DECL_PASSB(KK)                                                                     \
     void pass_b_2d_k##KK(const Apply2DArgs& a, const Apply2DArgs* jobs, const int* pre, int nj, \
                          int64_t nblocks, hipStream_t st);                                     \
-    int pass_b_2d_colblocks_k##KK(int nx2, int ny2);
+    int pass_b_2d_colblocks_k##KK(int nx2, int ny2, int f64);
 MSR_DECL_PASSB(1) MSR_DECL_PASSB(2) MSR_DECL_PASSB(3) MSR_DECL_PASSB(4)
 MSR_DECL_PASSB(5) MSR_DECL_PASSB(6) MSR_DECL_PASSB(7) MSR_DECL_PASSB(8)
 #undef MSR_DECL_PASSB
@@ -164,22 +187,12 @@ inline void launch_pass_b_2d(const Apply2DArgs& a, hipStream_t st) {
                     "pass B launch of %lld blocks", (long long)nblocks);                       \
         bool ok = dispatch_fft_len(ny2, [&]<int N>() {                                         \
             if constexpr (N <= 512) {                                                          \
-                using S = PassBShape<N, KK>;                                                   \
-                dim3 grid((unsigned)nblocks);                                                  \
-                if (jobs) {                                                                    \
-                    if (a.lam_real)                                                            \
-                        hipLaunchKernelGGL((pass_b_2d_batch<N, KK, true>), grid, dim3(S::BLK), \
-                                           0, st, jobs, pre, nj);                              \
-                    else                                                                       \
-                        hipLaunchKernelGGL((pass_b_2d_batch<N, KK, false>), grid,              \
-                                           dim3(S::BLK), 0, st, jobs, pre, nj);                \
+                if (a.f64) {                                                                   \
+                    if (a.lam_real) launch_pass_b_t<N, KK, true, double2>(a, jobs, pre, nj, nblocks, st);  \
+                    else launch_pass_b_t<N, KK, false, double2>(a, jobs, pre, nj, nblocks, st);            \
                 } else {                                                                       \
-                    if (a.lam_real)                                                            \
-                        hipLaunchKernelGGL((pass_b_2d<N, KK, true>), grid, dim3(S::BLK), 0,    \
-                                           st, a);                                             \
-                    else                                                                       \
-                        hipLaunchKernelGGL((pass_b_2d<N, KK, false>), grid, dim3(S::BLK), 0,   \
-                                           st, a);                                             \
+                    if (a.lam_real) launch_pass_b_t<N, KK, true, float2>(a, jobs, pre, nj, nblocks, st);   \
+                    else launch_pass_b_t<N, KK, false, float2>(a, jobs, pre, nj, nblocks, st);             \
                 }                                                                              \
             } else {                                                                           \
                 fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Ny = %d above 512", N);             \
@@ -187,11 +200,12 @@ inline void launch_pass_b_2d(const Apply2DArgs& a, hipStream_t st) {
         });                                                                                    \
         MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Ny = %d", ny2);           \
     }                                                                                          \
-    int pass_b_2d_colblocks_k##KK(int nx2, int ny2) {                                          \
+    int pass_b_2d_colblocks_k##KK(int nx2, int ny2, int f64) {                                 \
         int r = -1;                                                                            \
         dispatch_fft_len(ny2, [&]<int N>() {                                                   \
-            constexpr int WC = PassBShape<N, KK>::WC;                                          \
-            r = (nx2 + WC - 1) / WC;                                                           \
+            constexpr int WC = PassBShape<N, KK, float2>::WC;                                  \
+            constexpr int WC64 = PassBShape<N, KK, double2>::WC;                               \
+            r = (nx2 + (f64 ? WC64 : WC) - 1) / (f64 ? WC64 : WC);                             \
         });                                                                                    \
         MSR_REQUIRE(r > 0, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Ny = %d", ny2);        \
         return r;                                                                              \

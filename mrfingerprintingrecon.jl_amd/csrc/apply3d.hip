@@ -12,42 +12,43 @@
 
 namespace msr {
 
-template <int NX2>
+template <int NX2, class CT>
 constexpr int rows3_block() {
-    return (16 * FFTPlan<NX2>::T) < 256 ? 16 * FFTPlan<NX2>::T : 256;
+    constexpr int R = sizeof(CT) == 16 ? 8 : 16;  // rows per workgroup (LDS)
+    return (R * FFTPlan<NX2>::T) < 256 ? R * FFTPlan<NX2>::T : 256;
 }
 
-template <int NX2>
+template <int NX2, class CT>
 __global__ void __launch_bounds__(256)
-pass_a_3d(const float2* __restrict__ b, const float2* __restrict__ cmaps, float2* __restrict__ T1,
-          const float2* __restrict__ tw, int nx, int64_t nrows, int K, const int* __restrict__ skip) {
+pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, CT* __restrict__ T1,
+          const CT* __restrict__ tw, int nx, int64_t nrows, int K, const int* __restrict__ skip) {
     if (skip && *skip) return;
     constexpr int TP = FFTPlan<NX2>::T;
-    constexpr int BLK = rows3_block<NX2>();
+    constexpr int BLK = rows3_block<NX2, CT>();
     constexpr int F = BLK / TP;
     constexpr int LD = lds_line<NX2>() + 1;
-    __shared__ float2 s_tw[NX2];
-    __shared__ float2 s[F * LD];
+    __shared__ CT s_tw[NX2];
+    __shared__ CT s[F * LD];
     load_twiddles<NX2>(s_tw, tw);
     const int64_t r0 = (int64_t)blockIdx.x * F;
     const int k = blockIdx.y, c = blockIdx.z;
     const int64_t V = nrows * nx;
-    const float2* bk = b + (int64_t)k * V;
+    const CT* bk = b + (int64_t)k * V;
     const float2* cm = cmaps ? cmaps + (int64_t)c * V : nullptr;
     for (int e = threadIdx.x; e < F * NX2; e += BLK) {
         const int l = e / NX2, xx = e - l * NX2;
         const int64_t row = r0 + l;
-        float2 v = make_float2(0.f, 0.f);
+        CT v = cz<CT>();
         if (row < nrows && xx < nx) {
             v = bk[row * nx + xx];
-            if (cm) v = cmul(cm[row * nx + xx], v);
+            if (cm) v = cmul(cto<CT>(cm[row * nx + xx]), v);
         }
         s[l * LD + lds_pad(xx)] = v;
     }
     __syncthreads();
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
     fft_lds<NX2, -1>(s + g * LD, s_tw, t);
-    float2* out = T1 + ((int64_t)c * K + k) * nrows * NX2;
+    CT* out = T1 + ((int64_t)c * K + k) * nrows * NX2;
     for (int e = threadIdx.x; e < F * NX2; e += BLK) {
         const int l = e / NX2, q = e - l * NX2;
         const int64_t row = r0 + l;
@@ -55,35 +56,35 @@ pass_a_3d(const float2* __restrict__ b, const float2* __restrict__ cmaps, float2
     }
 }
 
-template <int NX2>
+template <int NX2, class CT>
 __global__ void __launch_bounds__(256)
-pass_e_3d(const float2* __restrict__ T1, const float2* __restrict__ cmaps,
-          float2* __restrict__ partial, const float2* __restrict__ tw, int nx, int64_t nrows,
+pass_e_3d(const CT* __restrict__ T1, const float2* __restrict__ cmaps,
+          CT* __restrict__ partial, const CT* __restrict__ tw, int nx, int64_t nrows,
           int K, int C, int G, const int* __restrict__ skip) {
     if (skip && *skip) return;
     constexpr int TP = FFTPlan<NX2>::T;
-    constexpr int BLK = rows3_block<NX2>();
+    constexpr int BLK = rows3_block<NX2, CT>();
     constexpr int F = BLK / TP;
     constexpr int LD = lds_line<NX2>() + 1;
     constexpr int NXC = NX2 / 2;
     constexpr int EA = (F * NXC + BLK - 1) / BLK;
-    __shared__ float2 s_tw[NX2];
-    __shared__ float2 s[F * LD];
+    __shared__ CT s_tw[NX2];
+    __shared__ CT s[F * LD];
     load_twiddles<NX2>(s_tw, tw);
     const int64_t r0 = (int64_t)blockIdx.x * F;
     const int k = blockIdx.y, grp = blockIdx.z;
     const int c0 = (int)((int64_t)C * grp / G), c1 = (int)((int64_t)C * (grp + 1) / G);
     const int64_t V = nrows * nx;
-    float2 acc[EA];
+    CT acc[EA];
 #pragma unroll
-    for (int i = 0; i < EA; ++i) acc[i] = make_float2(0.f, 0.f);
+    for (int i = 0; i < EA; ++i) acc[i] = cz<CT>();
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
     for (int c = c0; c < c1; ++c) {
-        const float2* in = T1 + ((int64_t)c * K + k) * nrows * NX2;
+        const CT* in = T1 + ((int64_t)c * K + k) * nrows * NX2;
         for (int e = threadIdx.x; e < F * NX2; e += BLK) {
             const int l = e / NX2, q = e - l * NX2;
             const int64_t row = r0 + l;
-            s[l * LD + lds_pad(q)] = row < nrows ? in[row * NX2 + q] : make_float2(0.f, 0.f);
+            s[l * LD + lds_pad(q)] = row < nrows ? in[row * NX2 + q] : cz<CT>();
         }
         __syncthreads();
         fft_lds<NX2, +1>(s + g * LD, s_tw, t);
@@ -95,15 +96,15 @@ pass_e_3d(const float2* __restrict__ T1, const float2* __restrict__ cmaps,
                 const int l = e / NXC, xx = e - l * NXC;
                 const int64_t row = r0 + l;
                 if (row < nrows) {
-                    const float2 v = s[l * LD + lds_pad(xx)];
-                    if (cm) cfmac(acc[i], cm[row * nx + xx], v);
+                    const CT v = s[l * LD + lds_pad(xx)];
+                    if (cm) cfmac(acc[i], cto<CT>(cm[row * nx + xx]), v);
                     else acc[i] = cadd(acc[i], v);
                 }
             }
         }
         __syncthreads();
     }
-    float2* pk = partial + ((int64_t)grp * K + k) * V;
+    CT* pk = partial + ((int64_t)grp * K + k) * V;
 #pragma unroll
     for (int i = 0; i < EA; ++i) {
         const int e = threadIdx.x + i * BLK;
@@ -130,22 +131,50 @@ static void launch_pass_c_3d(const Apply3DArgs& a, hipStream_t st, const int* sk
     MSR_CHECK_LAUNCH();
 }
 
-static void run_passes(const Apply3DArgs& a, msr_ctx* ctx, const float2* b, hipStream_t st,
-                       const int* skip) {
+template <class CT>
+static void run_row_passes_a(const Apply3DArgs& a, const void* b, hipStream_t st, const int* skip) {
     const int NX2 = 2 * a.nx;
     const int64_t nrows = (int64_t)a.ny * a.nz;
     bool ok = dispatch_fft_len(NX2, [&]<int N>() {
-        constexpr int F = rows3_block<N>() / FFTPlan<N>::T;
-        dim3 grid((unsigned)ceil_div(nrows, F), a.K, a.C);
-        hipLaunchKernelGGL(pass_a_3d<N>, grid, dim3(rows3_block<N>()), 0, st, b, a.cmaps, a.T1,
-                           a.tw_x, a.nx, nrows, a.K, skip);
+        if constexpr (N <= 512) {
+            constexpr int F = rows3_block<N, CT>() / FFTPlan<N>::T;
+            dim3 grid((unsigned)ceil_div(nrows, F), a.K, a.C);
+            hipLaunchKernelGGL((pass_a_3d<N, CT>), grid, dim3(rows3_block<N, CT>()), 0, st,
+                               static_cast<const CT*>(b), a.cmaps, static_cast<CT*>(a.T1),
+                               static_cast<const CT*>(a.tw_x), a.nx, nrows, a.K, skip);
+        } else {
+            fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Nx = %d above 512", N);
+        }
     });
     MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", NX2);
     MSR_CHECK_LAUNCH();
+}
+
+template <class CT>
+static void run_row_passes_e(const Apply3DArgs& a, hipStream_t st, const int* skip) {
+    const int64_t nrows = (int64_t)a.ny * a.nz;
+    dispatch_fft_len(2 * a.nx, [&]<int N>() {
+        if constexpr (N <= 512) {
+            constexpr int F = rows3_block<N, CT>() / FFTPlan<N>::T;
+            dim3 grid((unsigned)ceil_div(nrows, F), a.K, a.G);
+            hipLaunchKernelGGL((pass_e_3d<N, CT>), grid, dim3(rows3_block<N, CT>()), 0, st,
+                               static_cast<const CT*>(a.T1), a.cmaps, static_cast<CT*>(a.partial),
+                               static_cast<const CT*>(a.tw_x), a.nx, nrows, a.K, a.C, a.G, skip);
+        }
+    });
+    MSR_CHECK_LAUNCH();
+}
+
+static void run_passes(const Apply3DArgs& a, msr_ctx* ctx, const void* b, hipStream_t st,
+                       const int* skip) {
+    const int NX2 = 2 * a.nx;
+    if (a.f64) run_row_passes_a<double2>(a, b, st, skip);
+    else run_row_passes_a<float2>(a, b, st, skip);
     // B: y columns of T1 [(c,k,z)][y][qx] -> T2 [(c,k,z)][qy][qx]
     LinePass pb;
     pb.in = a.T1;
     pb.out = a.T2;
+    pb.f64 = a.f64;
     pb.nf = 2 * a.ny;
     pb.dir = -1;
     pb.lin = a.ny;
@@ -170,51 +199,45 @@ static void run_passes(const Apply3DArgs& a, msr_ctx* ctx, const float2* b, hipS
     pd.in_bstride = (int64_t)2 * a.ny * NX2;
     pd.out_bstride = (int64_t)a.ny * NX2;
     run_cols(pd, ctx, st);
-    ok = dispatch_fft_len(NX2, [&]<int N>() {
-        constexpr int F = rows3_block<N>() / FFTPlan<N>::T;
-        dim3 grid((unsigned)ceil_div(nrows, F), a.K, a.G);
-        hipLaunchKernelGGL(pass_e_3d<N>, grid, dim3(rows3_block<N>()), 0, st, a.T1, a.cmaps,
-                           a.partial, a.tw_x, a.nx, nrows, a.K, a.C, a.G, skip);
-    });
-    MSR_CHECK_LAUNCH();
+    if (a.f64) run_row_passes_e<double2>(a, st, skip);
+    else run_row_passes_e<float2>(a, st, skip);
 }
 
-static Apply3DArgs args3d(msr_normalop* op) {
+static Apply3DArgs args3d(msr_normalop* op, bool f64) {
     Apply3DArgs a;
     a.nx = (int)op->N[0];
     a.ny = (int)op->N[1];
     a.nz = (int)op->N[2];
     a.K = op->K;
     a.C = op->C;
+    a.f64 = f64 ? 1 : 0;
     a.cmaps = op->identity_cmaps ? nullptr : op->cmaps.as<float2>();
     a.lam = op->lam.p;
     a.lam_real = op->lam_real;
-    a.T1 = op->T.as<float2>();
-    a.T2 = op->T2.as<float2>();
-    a.partial = op->partial.as<float2>();
+    a.T1 = op->T.p;
+    a.T2 = op->T2.p;
+    a.partial = op->partial.p;
     a.G = op->G;
     a.Gb = op->Gb;
-    a.tw_x = twiddle_table(op->ctx, 2 * a.nx);
-    a.tw_y = twiddle_table(op->ctx, 2 * a.ny);
-    a.tw_z = twiddle_table(op->ctx, 2 * a.nz);
+    if (f64) {
+        a.tw_x = twiddle_table64(op->ctx, 2 * a.nx);
+        a.tw_y = twiddle_table64(op->ctx, 2 * a.ny);
+        a.tw_z = twiddle_table64(op->ctx, 2 * a.nz);
+    } else {
+        a.tw_x = twiddle_table(op->ctx, 2 * a.nx);
+        a.tw_y = twiddle_table(op->ctx, 2 * a.ny);
+        a.tw_z = twiddle_table(op->ctx, 2 * a.nz);
+    }
     return a;
 }
 
-void alloc_scratch_3d(msr_normalop* op) {
-    const int64_t nx = op->N[0], ny = op->N[1], nz = op->N[2];
-    op->G = 1;
-    op->Gb = 1;
-    op->T.alloc(sizeof(float2) * (size_t)op->C * op->K * nz * ny * 2 * nx);
-    op->T2.alloc(sizeof(float2) * (size_t)op->C * op->K * nz * 2 * ny * 2 * nx);
-    op->partial.alloc(sizeof(float2) * (size_t)op->G * op->K * op->V);
-}
-
-void launch_apply3d_passes(msr_normalop* op, const float2* b, hipStream_t st, const int* skip) {
-    run_passes(args3d(op), op->ctx, b, st, skip);
+void launch_apply3d_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st,
+                           const int* skip) {
+    run_passes(args3d(op, f64), op->ctx, b, st, skip);
 }
 
 void launch_apply3d_passes_coil(msr_normalop* op, const float2* b, int coil, hipStream_t st) {
-    Apply3DArgs a = args3d(op);
+    Apply3DArgs a = args3d(op, false);
     a.C = 1;
     a.G = 1;
     a.Gb = 1;

@@ -5,8 +5,9 @@
 
 namespace msr {
 
-void alloc_scratch_3d(msr_normalop* op);
-void launch_apply3d_passes(msr_normalop* op, const float2* b, hipStream_t st, const int* skip);
+// b, T1, T2 and partial in ComplexF64 when f64, else ComplexF32
+void launch_apply3d_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st,
+                           const int* skip);
 void launch_apply3d_passes_coil(msr_normalop* op, const float2* b, int coil, hipStream_t st);
 
 }  // namespace msr

@@ -12,42 +12,53 @@
 
 namespace msr {
 
+// Intermediates and arithmetic in ComplexF32 (f64 = 0) or ComplexF64 (f64 =
+// 1); the kernel and coil maps stay ComplexF32.
 struct Apply3DArgs {
     int nx, ny, nz, K, C;
+    int f64;
     const float2* cmaps;  // (C, nz, ny, nx) or nullptr
     const void* lam;
     int lam_real;
-    float2* T1;           // (C, K, nz, ny, 2nx)
-    float2* T2;           // (C, K, nz, 2ny, 2nx)
-    float2* partial;      // (G, K, V)
+    void* T1;             // (C, K, nz, ny, 2nx)
+    void* T2;             // (C, K, nz, 2ny, 2nx)
+    void* partial;        // (G, K, V)
     int G, Gb;
-    const float2 *tw_x, *tw_y, *tw_z;
+    const void *tw_x, *tw_y, *tw_z;
 };
 
-template <int NZ2, int K>
+// WC adjacent qx columns per workgroup, K z lines each (FFT by TP threads per
+// line), at least one thread per voxel (the kernel of one voxel per thread in
+// registers; threads beyond the K*WC*TP line threads idle in the FFT stages).
+template <int NZ2, int K, class CT>
 struct PassC3Shape {
     static constexpr int TP = FFTPlan<NZ2>::T;
-    static constexpr int WCT = (1024 / (K * TP)) > 1 ? 1024 / (K * TP) : 1;  // thread limit
-    static constexpr int WCL = (10922 / (K * NZ2)) > 1 ? 10922 / (K * NZ2) : 1;  // <= 96 KB LDS
+    static constexpr int PER = (K * TP > NZ2) ? K * TP : NZ2;                // threads per column
+    static constexpr int WCT = (512 / PER) > 1 ? 512 / PER : 1;              // thread limit
+    static constexpr int LB = 87376 / (int)sizeof(CT);                        // <= ~87 KB LDS
+    static constexpr int WCL = (LB / (K * NZ2)) > 1 ? LB / (K * NZ2) : 1;
     static constexpr int WC0 = WCT < WCL ? WCT : WCL;
     static constexpr int WC = WC0 < 8 ? WC0 : 8;     // adjacent qx lines per WG
-    static constexpr int BLK = WC * K * TP;
+    static constexpr int FT = WC * K * TP;           // FFT threads
+    static constexpr int BLK = WC * PER;
     static constexpr int NV = WC * NZ2;
-    static constexpr int EV = (NV + BLK - 1) / BLK;
+    static constexpr int EV = (NV + BLK - 1) / BLK;  // voxels per thread (1)
     static constexpr int P = K * (K + 1) / 2;
 };
 
-template <int NZ2, int K, bool LREAL>
-__global__ void __launch_bounds__(1024)
-pass_c_3d(float2* __restrict__ T2, const void* __restrict__ lam_v, const float2* __restrict__ tw,
+template <int NZ2, int K, bool LREAL, class CT>
+__global__ void __launch_bounds__(512)
+pass_c_3d(CT* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restrict__ tw,
           int nx2, int ny2, int nz, int C, int Gb, const int* __restrict__ skip) {
     if (skip && *skip) return;
-    using S = PassC3Shape<NZ2, K>;
+    using S = PassC3Shape<NZ2, K, CT>;
     using LT = typename std::conditional<LREAL, float, float2>::type;
+    using RT = real_t<CT>;
     constexpr int TP = S::TP, WC = S::WC, BLK = S::BLK, NV = S::NV, EV = S::EV, P = S::P;
+    constexpr int FT = S::FT;
     constexpr int LL = lds_line<NZ2>();
-    __shared__ float2 s_tw[NZ2];
-    __shared__ float2 s_buf[WC * K * LL];
+    __shared__ CT s_tw[NZ2];
+    __shared__ CT s_buf[WC * K * LL];
     const LT* lam = reinterpret_cast<const LT*>(lam_v);
     const int tid = threadIdx.x;
     // XCD-aware order: workgroups dispatched to one XCD (every 8th) take
@@ -81,6 +92,7 @@ pass_c_3d(float2* __restrict__ T2, const void* __restrict__ lam_v, const float2*
     }
 
     const int g = tid / TP, t = tid - g * TP;
+    const bool fft_thread = tid < FT;
     for (int c = c0; c < c1; ++c) {
         // element (line = k*WC + wc, z): lanes run over wc fastest for coalescing
         for (int el = tid; el < WC * K * NZ2; el += BLK) {
@@ -88,34 +100,34 @@ pass_c_3d(float2* __restrict__ T2, const void* __restrict__ lam_v, const float2*
             const int r = el / WC;
             const int k = r % K, z = r / K;
             const int qx = qx0 + wc;
-            float2 v = make_float2(0.f, 0.f);
+            CT v = cz<CT>();
             if (z < nz && qx < nx2)
                 v = T2[(((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx];
             s_buf[(wc * K + k) * LL + lds_pad(z)] = v;
         }
         __syncthreads();
-        fft_lds<NZ2, -1>(s_buf + g * LL, s_tw, t);
+        fft_lds<NZ2, -1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
 #pragma unroll
         for (int e = 0; e < EV; ++e) {
             const int v = tid + e * BLK;
             if (v < NV) {
                 const int wc = v / NZ2, qz = v - wc * NZ2;
-                float2* col = s_buf + (size_t)wc * K * LL + lds_pad(qz);
-                float2 in[K];
+                CT* col = s_buf + (size_t)wc * K * LL + lds_pad(qz);
+                CT in[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) in[k] = col[k * LL];
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    float2 acc = make_float2(0.f, 0.f);
+                    CT acc = cz<CT>();
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
                         if constexpr (LREAL) {
-                            const float l = lr[e][j <= k ? packed_index(j, k) : packed_index(k, j)];
-                            acc.x = fmaf(l, in[k].x, acc.x);
-                            acc.y = fmaf(l, in[k].y, acc.y);
+                            const RT l = (RT)lr[e][j <= k ? packed_index(j, k) : packed_index(k, j)];
+                            acc.x = fma(l, in[k].x, acc.x);
+                            acc.y = fma(l, in[k].y, acc.y);
                         } else {
-                            if (j <= k) cfma(acc, lr[e][packed_index(j, k)], in[k]);
-                            else cfmac(acc, lr[e][packed_index(k, j)], in[k]);
+                            if (j <= k) cfma(acc, cto<CT>(lr[e][packed_index(j, k)]), in[k]);
+                            else cfmac(acc, cto<CT>(lr[e][packed_index(k, j)]), in[k]);
                         }
                     }
                     col[j * LL] = acc;
@@ -123,7 +135,7 @@ pass_c_3d(float2* __restrict__ T2, const void* __restrict__ lam_v, const float2*
             }
         }
         __syncthreads();
-        fft_lds<NZ2, +1>(s_buf + g * LL, s_tw, t);
+        fft_lds<NZ2, +1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
         for (int el = tid; el < WC * K * NZ2; el += BLK) {
             const int wc = el % WC;
             const int r = el / WC;
@@ -137,13 +149,14 @@ pass_c_3d(float2* __restrict__ T2, const void* __restrict__ lam_v, const float2*
     }
 }
 
-template <int NZ2, int K, bool LREAL>
+template <int NZ2, int K, bool LREAL, class CT>
 void launch_pass_c_3d_t(const Apply3DArgs& a, hipStream_t st, const int* skip) {
-    using S = PassC3Shape<NZ2, K>;
+    using S = PassC3Shape<NZ2, K, CT>;
     const int nx2 = 2 * a.nx, ny2 = 2 * a.ny;
     dim3 grid((nx2 + S::WC - 1) / S::WC, ny2, a.Gb);
-    hipLaunchKernelGGL((pass_c_3d<NZ2, K, LREAL>), grid, dim3(S::BLK), 0, st, a.T2, a.lam, a.tw_z,
-                       nx2, ny2, a.nz, a.C, a.Gb, skip);
+    hipLaunchKernelGGL((pass_c_3d<NZ2, K, LREAL, CT>), grid, dim3(S::BLK), 0, st,
+                       static_cast<CT*>(a.T2), a.lam, static_cast<const CT*>(a.tw_z), nx2, ny2,
+                       a.nz, a.C, a.Gb, skip);
 }
 
 #define MSR_DECL_PASSC3(KK) void launch_pass_c_3d_k##KK(const Apply3DArgs& a, hipStream_t st, const int* skip);
@@ -156,8 +169,13 @@ MSR_DECL_PASSC3(5) MSR_DECL_PASSC3(6) MSR_DECL_PASSC3(7) MSR_DECL_PASSC3(8)
         const int nz2 = 2 * a.nz;                                                              \
         bool ok = dispatch_fft_len(nz2, [&]<int N>() {                                         \
             if constexpr (N <= 512) {                                                          \
-                if (a.lam_real) launch_pass_c_3d_t<N, KK, true>(a, st, skip);                  \
-                else launch_pass_c_3d_t<N, KK, false>(a, st, skip);                            \
+                if (a.f64) {                                                                   \
+                    if (a.lam_real) launch_pass_c_3d_t<N, KK, true, double2>(a, st, skip);     \
+                    else launch_pass_c_3d_t<N, KK, false, double2>(a, st, skip);               \
+                } else {                                                                       \
+                    if (a.lam_real) launch_pass_c_3d_t<N, KK, true, float2>(a, st, skip);      \
+                    else launch_pass_c_3d_t<N, KK, false, float2>(a, st, skip);                \
+                }                                                                              \
             } else {                                                                           \
                 fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Nz = %d above 512", N);             \
             }                                                                                  \

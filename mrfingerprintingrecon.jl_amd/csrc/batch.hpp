@@ -25,8 +25,8 @@ namespace msr {
 struct BatchGroup {
     int key[3] = {0, 0, 0};       // A/C: {2Nx}; B: {2Ny, K, lam_real}
     std::vector<int> jobs;        // batch indices of the group's operators
-    std::vector<int> pre;         // block prefix (jobs.size() + 1)
-    int64_t nblocks = 0;
+    std::vector<int> pre[2];      // block prefix (jobs.size() + 1) for FP32 / FP64 applies
+    int64_t nblocks[2] = {0, 0};
     size_t off_tab = 0, off_pre = 0;  // byte offsets in a table set
 };
 
@@ -64,10 +64,10 @@ msr_batch* batch_create(msr_ctx* ctx, int nops, msr_normalop* const* ops);
 void batch_apply(msr_batch* bt, float2* const* x, const float2* const* b, float2 alpha,
                  float2 beta, bool beta_zero, hipStream_t st);
 void batch_cg_start(msr_batch* bt, float2* const* x, const float2* const* b, int maxiter,
-                    double reltol, double abstol, bool initially_zero, bool fp64, hipStream_t st);
+                    double reltol, double abstol, bool initially_zero, int prec, hipStream_t st);
 void batch_cg_iterate(msr_batch* bt, int niter, hipStream_t st);
 void batch_cg_finish(msr_batch* bt, hipStream_t st);
 void batch_cg(msr_batch* bt, float2* const* x, const float2* const* b, int maxiter, double reltol,
-              double abstol, bool initially_zero, int mode, bool fp64, hipStream_t st);
+              double abstol, bool initially_zero, int mode, int prec, hipStream_t st);
 void batch_cg_status(msr_batch* bt, int* iters, double* res, int* done, hipStream_t st);
 }  // namespace msr

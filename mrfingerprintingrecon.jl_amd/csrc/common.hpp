@@ -74,6 +74,48 @@ __host__ __device__ __forceinline__ void cfmac(float2& acc, float2 a, float2 b) 
     acc.y = fmaf(-a.y, b.x, acc.y);
 }
 
+// double-precision complex (same operations)
+__host__ __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ void cfma(double2& acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+__host__ __device__ __forceinline__ void cfmac(double2& acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+}
+
+// Complex type traits: CT is float2 (ComplexF32) or double2 (ComplexF64).
+template <class CT> struct CTraits;
+template <> struct CTraits<float2> {
+    using R = float;
+    __host__ __device__ static float2 make(double a, double b) { return make_float2((float)a, (float)b); }
+};
+template <> struct CTraits<double2> {
+    using R = double;
+    __host__ __device__ static double2 make(double a, double b) { return make_double2(a, b); }
+};
+template <class CT> using real_t = typename CTraits<CT>::R;
+template <class CT>
+__host__ __device__ __forceinline__ CT cz() { return CTraits<CT>::make(0.0, 0.0); }
+// promote / convert a ComplexF32 value
+template <class CT>
+__host__ __device__ __forceinline__ CT cto(float2 a) { return CTraits<CT>::make(a.x, a.y); }
+template <class CT>
+__host__ __device__ __forceinline__ CT cto(double2 a) { return CTraits<CT>::make(a.x, a.y); }
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace msr

@@ -10,7 +10,9 @@
 //
 // Sign: DIR = -1 forward (exp(-2 pi i ...)), DIR = +1 inverse, unnormalised.
 // Twiddles come from a table tw[m] = exp(-2 pi i m / N) computed on the host in
-// double precision and staged into LDS by the caller.
+// double precision (stored as ComplexF32 or ComplexF64 for the arithmetic type)
+// and staged into LDS by the caller.  The arithmetic type is the element type
+// of the line: float2 or double2.
 #pragma once
 
 #include "common.hpp"
@@ -79,25 +81,37 @@ inline bool fft_len_supported(int64_t n) {
 
 // ----------------------------------------------------------- butterflies
 // multiply by DIR * i
-template <int DIR>
-__device__ __forceinline__ float2 mul_i(float2 a) {
-    return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+template <int DIR, class CT>
+__device__ __forceinline__ CT mul_i(CT a) {
+    CT r;
+    if (DIR > 0) {
+        r.x = -a.y;
+        r.y = a.x;
+    } else {
+        r.x = a.y;
+        r.y = -a.x;
+    }
+    return r;
 }
 
+// Butterflies of radix R on values of complex type CT (float2 or double2);
+// constants are rounded to the arithmetic type.
 template <int R, int DIR> struct Butterfly;
 
 template <int DIR> struct Butterfly<2, DIR> {
-    __device__ __forceinline__ static void run(float2* v) {
-        float2 a = v[0], b = v[1];
+    template <class CT>
+    __device__ __forceinline__ static void run(CT* v) {
+        CT a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     }
 };
 
 template <int DIR> struct Butterfly<4, DIR> {
-    __device__ __forceinline__ static void run(float2* v) {
-        float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-        float2 t2 = cadd(v[1], v[3]), t3 = mul_i<DIR>(csub(v[1], v[3]));
+    template <class CT>
+    __device__ __forceinline__ static void run(CT* v) {
+        CT t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        CT t2 = cadd(v[1], v[3]), t3 = mul_i<DIR>(csub(v[1], v[3]));
         v[0] = cadd(t0, t2);
         v[2] = csub(t0, t2);
         v[1] = cadd(t1, t3);
@@ -106,16 +120,18 @@ template <int DIR> struct Butterfly<4, DIR> {
 };
 
 template <int DIR> struct Butterfly<8, DIR> {
-    __device__ __forceinline__ static void run(float2* v) {
-        constexpr float s = 0.70710678118654752440f;
-        float2 e[4] = {v[0], v[2], v[4], v[6]};
-        float2 o[4] = {v[1], v[3], v[5], v[7]};
+    template <class CT>
+    __device__ __forceinline__ static void run(CT* v) {
+        using RT = real_t<CT>;
+        constexpr RT s = (RT)0.70710678118654752440;
+        CT e[4] = {v[0], v[2], v[4], v[6]};
+        CT o[4] = {v[1], v[3], v[5], v[7]};
         Butterfly<4, DIR>::run(e);
         Butterfly<4, DIR>::run(o);
         // w8^q for q = 0..3 with w8 = (1 + DIR i)/sqrt2
-        float2 o1 = cscale(cadd(o[1], mul_i<DIR>(o[1])), s);
-        float2 o2 = mul_i<DIR>(o[2]);
-        float2 o3 = cscale(csub(mul_i<DIR>(o[3]), o[3]), s);
+        CT o1 = cscale(cadd(o[1], mul_i<DIR>(o[1])), s);
+        CT o2 = mul_i<DIR>(o[2]);
+        CT o3 = cscale(csub(mul_i<DIR>(o[3]), o[3]), s);
         v[0] = cadd(e[0], o[0]);
         v[4] = csub(e[0], o[0]);
         v[1] = cadd(e[1], o1);
@@ -128,11 +144,15 @@ template <int DIR> struct Butterfly<8, DIR> {
 };
 
 template <int DIR> struct Butterfly<3, DIR> {
-    __device__ __forceinline__ static void run(float2* v) {
-        constexpr float h = 0.86602540378443864676f;  // sin(2 pi / 3)
-        float2 s = cadd(v[1], v[2]);
-        float2 d = mul_i<DIR>(cscale(csub(v[1], v[2]), h));
-        float2 m = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
+    template <class CT>
+    __device__ __forceinline__ static void run(CT* v) {
+        using RT = real_t<CT>;
+        constexpr RT h = (RT)0.86602540378443864676;  // sin(2 pi / 3)
+        CT s = cadd(v[1], v[2]);
+        CT d = mul_i<DIR>(cscale(csub(v[1], v[2]), h));
+        CT m;
+        m.x = v[0].x - (RT)0.5 * s.x;
+        m.y = v[0].y - (RT)0.5 * s.y;
         v[0] = cadd(v[0], s);
         v[1] = cadd(m, d);
         v[2] = csub(m, d);
@@ -140,17 +160,26 @@ template <int DIR> struct Butterfly<3, DIR> {
 };
 
 template <int DIR> struct Butterfly<5, DIR> {
-    __device__ __forceinline__ static void run(float2* v) {
-        constexpr float c1 = 0.30901699437494742410f;   // cos(2 pi/5)
-        constexpr float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
-        constexpr float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
-        constexpr float s2 = 0.58778525229247312917f;   // sin(4 pi/5)
-        float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
-        float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
-        float2 p1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
-        float2 p2 = make_float2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
-        float2 q1 = mul_i<DIR>(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
-        float2 q2 = mul_i<DIR>(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+    template <class CT>
+    __device__ __forceinline__ static void run(CT* v) {
+        using RT = real_t<CT>;
+        constexpr RT c1 = (RT)0.30901699437494742410;   // cos(2 pi/5)
+        constexpr RT c2 = (RT)-0.80901699437494742410;  // cos(4 pi/5)
+        constexpr RT s1 = (RT)0.95105651629515357212;   // sin(2 pi/5)
+        constexpr RT s2 = (RT)0.58778525229247312917;   // sin(4 pi/5)
+        CT a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        CT a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        CT p1, p2, q1, q2;
+        p1.x = v[0].x + c1 * a1.x + c2 * a2.x;
+        p1.y = v[0].y + c1 * a1.y + c2 * a2.y;
+        p2.x = v[0].x + c2 * a1.x + c1 * a2.x;
+        p2.y = v[0].y + c2 * a1.y + c1 * a2.y;
+        q1.x = s1 * b1.x + s2 * b2.x;
+        q1.y = s1 * b1.y + s2 * b2.y;
+        q2.x = s2 * b1.x - s1 * b2.x;
+        q2.y = s2 * b1.y - s1 * b2.y;
+        q1 = mul_i<DIR>(q1);
+        q2 = mul_i<DIR>(q2);
         v[0] = cadd(v[0], cadd(a1, a2));
         v[1] = cadd(p1, q1);
         v[4] = csub(p1, q1);
@@ -160,7 +189,7 @@ template <int DIR> struct Butterfly<5, DIR> {
 };
 
 // ----------------------------------------------------------- LDS layout
-// Element i of a line lives at lds_pad(i): one float2 of padding after every
+// Element i of a line lives at lds_pad(i): one complex of padding after every
 // 8 elements, so the stride-R stores of the first Stockham stages and the
 // strided reads of the transposing passes spread over the 64 LDS banks.
 __host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 3); }
@@ -168,16 +197,16 @@ template <int N>
 constexpr int lds_line() { return N + N / 8; }
 
 // ----------------------------------------------------------- stages
-template <int N, int R, int NS, int T, int DIR>
-__device__ __forceinline__ void fft_stage(float2* buf, const float2* tw, int t) {
+template <int N, int R, int NS, int T, int DIR, class CT>
+__device__ __forceinline__ void fft_stage(CT* buf, const CT* tw, int t, bool active) {
     constexpr int NB = N / R;
     constexpr int PER = (NB + T - 1) / T;
     constexpr bool EXACT = (NB % T) == 0;
-    float2 v[PER][R];
+    CT v[PER][R];
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         const int j = t + p * T;
-        if (EXACT || j < NB) {
+        if (active && (EXACT || j < NB)) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[p][r] = buf[lds_pad(j + r * NB)];
         }
@@ -186,12 +215,12 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw, int t) 
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         const int j = t + p * T;
-        if (EXACT || j < NB) {
+        if (active && (EXACT || j < NB)) {
             const int k = j % NS;
             if constexpr (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    float2 w = tw[(k * r) * (N / (NS * R))];
+                    CT w = tw[(k * r) * (N / (NS * R))];
                     if constexpr (DIR > 0) w.y = -w.y;
                     v[p][r] = cmul(v[p][r], w);
                 }
@@ -205,26 +234,28 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw, int t) 
     __syncthreads();
 }
 
-template <int N, int T, int DIR, int S, int NS>
-__device__ __forceinline__ void fft_stages(float2* buf, const float2* tw, int t) {
+template <int N, int T, int DIR, int S, int NS, class CT>
+__device__ __forceinline__ void fft_stages(CT* buf, const CT* tw, int t, bool active) {
     if constexpr (S < FFTPlan<N>::NST) {
         constexpr int R = FFTPlan<N>::R[S];
-        fft_stage<N, R, NS, T, DIR>(buf, tw, t);
-        fft_stages<N, T, DIR, S + 1, NS * R>(buf, tw, t);
+        fft_stage<N, R, NS, T, DIR, CT>(buf, tw, t, active);
+        fft_stages<N, T, DIR, S + 1, NS * R, CT>(buf, tw, t, active);
     }
 }
 
 // In-place N-point FFT of a line (element i at buf[lds_pad(i)]) by the
-// T = FFTPlan<N>::T threads with local index t.  Requires a barrier before the call (data in LDS); ends with
-// a barrier.  All threads of the workgroup must call it.
-template <int N, int DIR>
-__device__ __forceinline__ void fft_lds(float2* buf, const float2* tw, int t) {
-    fft_stages<N, FFTPlan<N>::T, DIR, 0, 1>(buf, tw, t);
+// T = FFTPlan<N>::T threads with local index t.  Requires a barrier before the
+// call (data in LDS); ends with a barrier.  All threads of the workgroup must
+// call it; threads with active = false only take part in the barriers (buf is
+// not touched), for workgroups with more threads than line work.
+template <int N, int DIR, class CT>
+__device__ __forceinline__ void fft_lds(CT* buf, const CT* tw, int t, bool active = true) {
+    fft_stages<N, FFTPlan<N>::T, DIR, 0, 1, CT>(buf, tw, t, active);
 }
 
 // Copy an N-entry twiddle table from global memory into LDS (no barrier).
-template <int N>
-__device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* g_tw) {
+template <int N, class CT>
+__device__ __forceinline__ void load_twiddles(CT* s_tw, const CT* g_tw) {
     for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = g_tw[i];
 }
 

@@ -12,39 +12,43 @@ __device__ __forceinline__ int select_bin(int j, int lout, int nf, int sel) {
     return r < 0 ? r + nf : r;
 }
 
-template <int NF>
+template <int NF, class CT>
 constexpr int rows_block() {
-    return (16 * FFTPlan<NF>::T) < 256 ? 16 * FFTPlan<NF>::T : 256;
+    constexpr int R = sizeof(CT) == 16 ? 8 : 16;  // lines per workgroup (LDS)
+    return (R * FFTPlan<NF>::T) < 256 ? R * FFTPlan<NF>::T : 256;
 }
 
-template <int NF, int DIR>
-__global__ void __launch_bounds__(256) rows_kernel(LinePass p, const float2* __restrict__ tw) {
+template <int NF, int DIR, class CT>
+__global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restrict__ tw) {
     if (p.skip && *p.skip) return;
+    using RT = real_t<CT>;
     constexpr int TP = FFTPlan<NF>::T;
-    constexpr int BLK = rows_block<NF>();
+    constexpr int BLK = rows_block<NF, CT>();
     constexpr int F = BLK / TP;
     constexpr int LD = lds_line<NF>() + 1;
-    __shared__ float2 s_tw[NF];
-    __shared__ float2 s[F * LD];
+    __shared__ CT s_tw[NF];
+    __shared__ CT s[F * LD];
+    const CT* in = static_cast<const CT*>(p.in);
+    CT* out = static_cast<CT*>(p.out);
     load_twiddles<NF>(s_tw, tw);
     const int64_t l0 = (int64_t)blockIdx.x * F;
     if (p.isel == SEL_FIRST && !p.win) {
         for (int e = threadIdx.x; e < F * NF; e += BLK) {
             const int l = e / NF, i = e - l * NF;
             const int64_t line = l0 + l;
-            float2 v = make_float2(0.f, 0.f);
-            if (line < p.nlines && i < p.lin) v = p.in[line * p.in_pitch + i];
+            CT v = cz<CT>();
+            if (line < p.nlines && i < p.lin) v = in[line * p.in_pitch + i];
             s[l * LD + lds_pad(i)] = v;
         }
     } else {
-        for (int e = threadIdx.x; e < F * NF; e += BLK) s[(e / NF) * LD + lds_pad(e % NF)] = make_float2(0.f, 0.f);
+        for (int e = threadIdx.x; e < F * NF; e += BLK) s[(e / NF) * LD + lds_pad(e % NF)] = cz<CT>();
         __syncthreads();
         for (int e = threadIdx.x; e < F * p.lin; e += BLK) {
             const int l = e / p.lin, i = e - l * p.lin;
             const int64_t line = l0 + l;
             if (line < p.nlines) {
-                float2 v = p.in[line * p.in_pitch + i];
-                if (p.win) v = cscale(v, p.win[i]);
+                CT v = in[line * p.in_pitch + i];
+                if (p.win) v = cscale(v, (RT)p.win[i]);
                 s[l * LD + lds_pad(select_bin(i, p.lin, NF, p.isel))] = v;
             }
         }
@@ -56,27 +60,30 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const float2* __r
         const int l = e / p.lout, j = e - l * p.lout;
         const int64_t line = l0 + l;
         if (line >= p.nlines) continue;
-        float2 v = s[l * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
-        if (p.w) v = cscale(v, p.w[j]);
-        p.out[line * p.out_pitch + j] = v;
+        CT v = s[l * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
+        if (p.w) v = cscale(v, (RT)p.w[j]);
+        out[line * p.out_pitch + j] = v;
     }
 }
 
-template <int NF>
+template <int NF, class CT>
 constexpr int cols_wc() {
-    return NF <= 512 ? 16 : 8;
+    return (NF <= 512 ? 16 : 8) / (sizeof(CT) == 16 ? 2 : 1);
 }
 
-template <int NF, int DIR>
-__global__ void __launch_bounds__(256) cols_kernel(LinePass p, const float2* __restrict__ tw) {
+template <int NF, int DIR, class CT>
+__global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restrict__ tw) {
     if (p.skip && *p.skip) return;
+    using RT = real_t<CT>;
     constexpr int TP = FFTPlan<NF>::T;
-    constexpr int WC = cols_wc<NF>();
+    constexpr int WC = cols_wc<NF, CT>();
     constexpr int BLK = (WC * TP) < 256 ? WC * TP : 256;
     constexpr int F = BLK / TP;
     constexpr int LD = lds_line<NF>() + 1;
-    __shared__ float2 s_tw[NF];
-    __shared__ float2 s[WC * LD];
+    __shared__ CT s_tw[NF];
+    __shared__ CT s[WC * LD];
+    const CT* in = static_cast<const CT*>(p.in);
+    CT* out = static_cast<CT*>(p.out);
     load_twiddles<NF>(s_tw, tw);
     const int64_t col0 = (int64_t)blockIdx.x * WC;
     const int64_t b = blockIdx.y;
@@ -84,19 +91,19 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const float2* __r
         for (int e = threadIdx.x; e < WC * NF; e += BLK) {
             const int wc = e % WC, i = e / WC;
             const int64_t col = col0 + wc;
-            float2 v = make_float2(0.f, 0.f);
-            if (col < p.ncols && i < p.lin) v = p.in[b * p.in_bstride + i * p.in_stride + col];
+            CT v = cz<CT>();
+            if (col < p.ncols && i < p.lin) v = in[b * p.in_bstride + i * p.in_stride + col];
             s[wc * LD + lds_pad(i)] = v;
         }
     } else {
-        for (int e = threadIdx.x; e < WC * NF; e += BLK) s[(e % WC) * LD + lds_pad(e / WC)] = make_float2(0.f, 0.f);
+        for (int e = threadIdx.x; e < WC * NF; e += BLK) s[(e % WC) * LD + lds_pad(e / WC)] = cz<CT>();
         __syncthreads();
         for (int e = threadIdx.x; e < WC * p.lin; e += BLK) {
             const int wc = e % WC, i = e / WC;
             const int64_t col = col0 + wc;
             if (col < p.ncols) {
-                float2 v = p.in[b * p.in_bstride + i * p.in_stride + col];
-                if (p.win) v = cscale(v, p.win[i]);
+                CT v = in[b * p.in_bstride + i * p.in_stride + col];
+                if (p.win) v = cscale(v, (RT)p.win[i]);
                 s[wc * LD + lds_pad(select_bin(i, p.lin, NF, p.isel))] = v;
             }
         }
@@ -109,10 +116,12 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const float2* __r
         const int wc = e % WC, j = e / WC;
         const int64_t col = col0 + wc;
         if (col >= p.ncols) continue;
-        float2 v = s[wc * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
-        if (p.w) v = cscale(v, p.w[j]);
-        if (!p.lam) {
-            p.out[b * p.out_bstride + j * p.out_stride + col] = v;
+        CT v = s[wc * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
+        if (p.w) v = cscale(v, (RT)p.w[j]);
+        if constexpr (sizeof(CT) == 16) {
+            out[b * p.out_bstride + j * p.out_stride + col] = v;
+        } else if (!p.lam) {
+            out[b * p.out_bstride + j * p.out_stride + col] = v;
         } else {
             int64_t off, mi;
             if (!p.lam3d) {
@@ -129,30 +138,41 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const float2* __r
     }
 }
 
-void run_rows(const LinePass& p, msr_ctx* ctx, hipStream_t st) {
-    const float2* tw = twiddle_table(ctx, p.nf);
+template <class CT>
+static void run_rows_t(const LinePass& p, const CT* tw, hipStream_t st) {
     bool ok = dispatch_fft_len(p.nf, [&]<int N>() {
-        constexpr int BLK = rows_block<N>();
+        constexpr int BLK = rows_block<N, CT>();
         constexpr int F = BLK / FFTPlan<N>::T;
         dim3 grid((unsigned)ceil_div(p.nlines, F));
-        if (p.dir < 0) hipLaunchKernelGGL((rows_kernel<N, -1>), grid, dim3(BLK), 0, st, p, tw);
-        else hipLaunchKernelGGL((rows_kernel<N, 1>), grid, dim3(BLK), 0, st, p, tw);
+        if (p.dir < 0) hipLaunchKernelGGL((rows_kernel<N, -1, CT>), grid, dim3(BLK), 0, st, p, tw);
+        else hipLaunchKernelGGL((rows_kernel<N, 1, CT>), grid, dim3(BLK), 0, st, p, tw);
     });
     MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "no FFT kernel for length %d", p.nf);
     MSR_CHECK_LAUNCH();
 }
 
-void run_cols(const LinePass& p, msr_ctx* ctx, hipStream_t st) {
-    const float2* tw = twiddle_table(ctx, p.nf);
+template <class CT>
+static void run_cols_t(const LinePass& p, const CT* tw, hipStream_t st) {
     bool ok = dispatch_fft_len(p.nf, [&]<int N>() {
-        constexpr int WC = cols_wc<N>();
+        constexpr int WC = cols_wc<N, CT>();
         constexpr int BLK = (WC * FFTPlan<N>::T) < 256 ? WC * FFTPlan<N>::T : 256;
         dim3 grid((unsigned)ceil_div(p.ncols, WC), (unsigned)p.nb);
-        if (p.dir < 0) hipLaunchKernelGGL((cols_kernel<N, -1>), grid, dim3(BLK), 0, st, p, tw);
-        else hipLaunchKernelGGL((cols_kernel<N, 1>), grid, dim3(BLK), 0, st, p, tw);
+        if (p.dir < 0) hipLaunchKernelGGL((cols_kernel<N, -1, CT>), grid, dim3(BLK), 0, st, p, tw);
+        else hipLaunchKernelGGL((cols_kernel<N, 1, CT>), grid, dim3(BLK), 0, st, p, tw);
     });
     MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "no FFT kernel for length %d", p.nf);
     MSR_CHECK_LAUNCH();
+}
+
+void run_rows(const LinePass& p, msr_ctx* ctx, hipStream_t st) {
+    if (p.f64) run_rows_t<double2>(p, twiddle_table64(ctx, p.nf), st);
+    else run_rows_t<float2>(p, twiddle_table(ctx, p.nf), st);
+}
+
+void run_cols(const LinePass& p, msr_ctx* ctx, hipStream_t st) {
+    MSR_REQUIRE(!(p.f64 && p.lam), MSR_ERR_ERROR, "the dense-kernel store is ComplexF32 only");
+    if (p.f64) run_cols_t<double2>(p, twiddle_table64(ctx, p.nf), st);
+    else run_cols_t<float2>(p, twiddle_table(ctx, p.nf), st);
 }
 
 }  // namespace msr

@@ -17,8 +17,9 @@ namespace msr {
 enum { SEL_FIRST = 0, SEL_CENTER = 1, SEL_FFTORDER = 2 };
 
 struct LinePass {
-    const float2* in = nullptr;
-    float2* out = nullptr;
+    const void* in = nullptr;   // ComplexF32, or ComplexF64 when f64
+    void* out = nullptr;
+    int f64 = 0;                // arithmetic / data precision (the dense-kernel store is FP32 only)
     int nf = 0;          // FFT length
     int dir = -1;        // -1 forward, +1 inverse
     int lin = 0, lout = 0, sel = SEL_FIRST;

@@ -13,7 +13,8 @@
 
 struct msr_ctx {
     int device = 0;
-    std::map<int, float2*> twiddles;  // N -> device table exp(-2 pi i m/N)
+    std::map<int, float2*> twiddles;     // N -> device table exp(-2 pi i m/N), ComplexF32
+    std::map<int, double2*> twiddles64;  // the same in ComplexF64
     std::map<std::tuple<int, int, int>, float*> deapod;  // (n_os, n_out, sel) -> 1/phihat
     std::mutex lock;  // guards the two lazily filled tables (calls may come from several threads)
     ~msr_ctx();
@@ -46,6 +47,7 @@ struct DevBuf {
 };
 
 const float2* twiddle_table(msr_ctx* ctx, int n);
+const double2* twiddle_table64(msr_ctx* ctx, int n);
 void ensure_device(const msr_ctx* ctx);
 
 // Tile sort of a trajectory's points for one oversampled grid (spread.hip).
@@ -97,16 +99,17 @@ struct msr_normalop {
     msr::DevBuf T;                    // FFT intermediate
     msr::DevBuf T2;                   // second intermediate (3D)
     msr::DevBuf partial;              // (G, K, V)
+    size_t scratch_es = 0;            // element size T / T2 / partial are allocated for
     int G = 1, Gb = 1;
     // CG state (allocated on first use)
     msr::DevBuf cg_r, cg_u, cg_c, cg_scal, cg_part, cg_x64, cg_u32;
-    int cg_fp64 = -1;                 // vector precision of the last started CG
+    int cg_fp64 = -1;                 // precision (MSR_PREC_*) of the last started CG
     float2* cg_x_user = nullptr;      // its caller-side FP32 solution
     hipGraphExec_t graph = nullptr;
     std::vector<const void*> graph_key;
     int graph_maxiter = -1;
     double graph_tol[2] = {0.0, 0.0};
-    int graph_fp64 = -1;
+    int graph_fp64 = -1;              // precision of the captured CG
     hipStream_t graph_stream = nullptr;
     ~msr_normalop();
 };

@@ -84,21 +84,43 @@ static int env_int(const char* name, int dflt) {
 }
 
 void op_alloc_scratch(msr_normalop* op) {
-    const int64_t nx = op->N[0], ny = op->N[1];
+    const int64_t ny = op->N[1];
     if (op->d == 2) {
-        const int tiles = tiles2d((int)ny);
+        const int tiles = tiles2d((int)ny, 0);
         int G = (int)ceil_div(512, (int64_t)tiles * op->K);
         G = std::max(1, std::min(G, op->C));
         op->G = env_int("MSR_G", G);
         op->G = std::max(1, std::min(op->G, op->C));
-        const int64_t colblocks = 2 * nx;
+        const int64_t colblocks = 2 * op->N[0];
         int Gb = colblocks >= 1024 ? 1 : 2;
         op->Gb = std::max(1, std::min(env_int("MSR_GB", Gb), op->C));
-        op->T.alloc(sizeof(float2) * (size_t)op->C * op->K * (2 * nx) * ny);
-        op->partial.alloc(sizeof(float2) * (size_t)op->G * op->K * op->V);
     } else {
-        alloc_scratch_3d(op);
+        op->G = 1;
+        op->Gb = 1;
     }
+    op_ensure_scratch(op, false);
+}
+
+// Intermediates for the apply precision (grow-only: ComplexF64 needs twice the
+// bytes).  Re-allocation changes the addresses, which the graph keys include.
+void op_ensure_scratch(msr_normalop* op, bool f64) {
+    // the twiddle tables of the apply are device allocations too: create them
+    // here, outside any stream capture
+    for (int a = 0; a < op->d; ++a) {
+        if (f64) (void)twiddle_table64(op->ctx, (int)(2 * op->N[a]));
+        else (void)twiddle_table(op->ctx, (int)(2 * op->N[a]));
+    }
+    const size_t es = f64 ? sizeof(double2) : sizeof(float2);
+    if (op->scratch_es >= es) return;
+    const int64_t nx = op->N[0], ny = op->N[1], nz = op->N[2];
+    if (op->d == 2) {
+        op->T.alloc(es * (size_t)op->C * op->K * (2 * nx) * ny);
+    } else {
+        op->T.alloc(es * (size_t)op->C * op->K * nz * ny * 2 * nx);
+        op->T2.alloc(es * (size_t)op->C * op->K * nz * 2 * ny * 2 * nx);
+    }
+    op->partial.alloc(es * (size_t)op->G * op->K * op->V);
+    op->scratch_es = es;
 }
 
 static void check_shape(int d, const int64_t* N) {
@@ -195,54 +217,70 @@ msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real)
 }
 
 // ------------------------------------------------------------------ applies
-Apply2DArgs args2d(msr_normalop* op) {
+Apply2DArgs args2d(msr_normalop* op, bool f64) {
     Apply2DArgs a;
     a.nx = (int)op->N[0];
     a.ny = (int)op->N[1];
     a.K = op->K;
     a.C = op->C;
+    a.f64 = f64 ? 1 : 0;
     a.cmaps = op->identity_cmaps ? nullptr : op->cmaps.as<float2>();
     a.lam = op->lam.p;
     a.lam_real = op->lam_real;
-    a.T = op->T.as<float2>();
-    a.partial = op->partial.as<float2>();
+    a.T = op->T.p;
+    a.partial = op->partial.p;
     a.G = op->G;
     a.Gb = op->Gb;
-    a.tw_x = twiddle_table(op->ctx, 2 * a.nx);
-    a.tw_y = twiddle_table(op->ctx, 2 * a.ny);
+    if (f64) {
+        a.tw_x = twiddle_table64(op->ctx, 2 * a.nx);
+        a.tw_y = twiddle_table64(op->ctx, 2 * a.ny);
+    } else {
+        a.tw_x = twiddle_table(op->ctx, 2 * a.nx);
+        a.tw_y = twiddle_table(op->ctx, 2 * a.ny);
+    }
     a.b = nullptr;
     a.skip = nullptr;
     return a;
 }
 
-float op_scale(const msr_normalop* op) { return (float)(1.0 / (double)op_grid_cells(op)); }
+double op_scale(const msr_normalop* op) { return 1.0 / (double)op_grid_cells(op); }
 
-// partial[g] <- unscaled group sums of A^H A b
-void op_passes(msr_normalop* op, const float2* b, hipStream_t st, const int* skip) {
+// partial[g] <- unscaled group sums of A^H A b (b and partial of the apply type)
+void op_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st, const int* skip) {
+    MSR_REQUIRE(op->scratch_es >= (f64 ? sizeof(double2) : sizeof(float2)), MSR_ERR_ERROR,
+                "apply scratch not allocated for this precision");
     if (op->d == 2) {
-        Apply2DArgs a = args2d(op);
+        Apply2DArgs a = args2d(op, f64);
         a.b = b;
         a.skip = skip;
         launch_apply2d_passes(a, st);
     } else {
-        launch_apply3d_passes(op, b, st, skip);
+        launch_apply3d_passes(op, b, f64, st, skip);
     }
 }
 
 void op_apply(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
               bool beta_zero, hipStream_t st) {
-    op_passes(op, b, st, nullptr);
-    launch_epilogue(op->partial.as<float2>(), op->G, op->V * op->K, x, alpha, beta, beta_zero,
-                    op_scale(op), st, nullptr);
+    op_passes(op, b, false, st, nullptr);
+    launch_epilogue<float2>(op->partial.as<float2>(), op->G, op->V * op->K, x, alpha, beta,
+                            beta_zero, op_scale(op), st, nullptr);
+}
+
+void op_apply_f64(msr_normalop* op, double2* x, const double2* b, double2 alpha, double2 beta,
+                  bool beta_zero, hipStream_t st) {
+    op_ensure_scratch(op, true);
+    op_passes(op, b, true, st, nullptr);
+    launch_epilogue<double2>(op->partial.as<double2>(), op->G, op->V * op->K, x, alpha, beta,
+                             beta_zero, op_scale(op), st, nullptr);
 }
 
 // The reference GPU order: one coil at a time, kernel re-read per coil.
 void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
                       bool beta_zero, hipStream_t st) {
-    launch_scale(x, op->V * op->K, beta, beta_zero, st);
+    launch_scale<float2>(x, op->V * op->K, beta, beta_zero, st);
     for (int c = 0; c < op->C; ++c) {
         if (op->d == 2) {
-            Apply2DArgs a = args2d(op);
+            Apply2DArgs a = args2d(op, false);
             a.C = 1;
             a.G = 1;
             a.Gb = 1;
@@ -252,8 +290,8 @@ void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha
         } else {
             launch_apply3d_passes_coil(op, b, c, st);
         }
-        launch_epilogue(op->partial.as<float2>(), 1, op->V * op->K, x, alpha,
-                        make_float2(1.f, 0.f), false, op_scale(op), st, nullptr);
+        launch_epilogue<float2>(op->partial.as<float2>(), 1, op->V * op->K, x, alpha,
+                                make_float2(1.f, 0.f), false, op_scale(op), st, nullptr);
     }
 }
 
@@ -278,58 +316,87 @@ void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hip
 }
 
 // ------------------------------------------------------------------ CG
-void op_cg_buffers(msr_normalop* op, bool fp64) {
+// Buffers of a CG in precision prec (MSR_PREC_*): r, u (and the FP64 x) in the
+// vector type, c in the apply type, and an FP32 copy of u when the vectors are
+// FP64 but the apply is FP32.  Also sizes the apply scratch.
+void op_cg_buffers(msr_normalop* op, int prec) {
     const size_t n = (size_t)op->V * op->K;
-    const size_t es = fp64 ? sizeof(double2) : sizeof(float2);
+    const bool v64 = prec != MSR_PREC_F32, a64 = prec == MSR_PREC_F64;
+    const size_t es = v64 ? sizeof(double2) : sizeof(float2);
     op->cg_r.alloc(es * n);
     op->cg_u.alloc(es * n);
-    op->cg_c.alloc(sizeof(float2) * n);
-    if (fp64) {
-        op->cg_x64.alloc(sizeof(double2) * n);
-        op->cg_u32.alloc(sizeof(float2) * n);
-    }
+    op->cg_c.alloc((a64 ? sizeof(double2) : sizeof(float2)) * n);
+    if (v64) op->cg_x64.alloc(sizeof(double2) * n);
+    if (prec == MSR_PREC_MIXED) op->cg_u32.alloc(sizeof(float2) * n);
     op->cg_scal.alloc(sizeof(CgScalars));
     op->cg_part.alloc(sizeof(double) * 2 * RED_BLOCKS);
+    op_ensure_scratch(op, a64);
 }
 
-template <class VT>
-CgVecs<VT> op_cg_vecs(msr_normalop* op, float2* x) {
-    CgVecs<VT> v;
+template <class VT, class AT>
+CgVecs<VT, AT> op_cg_vecs(msr_normalop* op, float2* x) {
+    CgVecs<VT, AT> v;
     v.r = op->cg_r.as<VT>();
     v.u = op->cg_u.as<VT>();
-    v.c = op->cg_c.as<float2>();
-    if constexpr (std::is_same<VT, double2>::value) {
-        v.x = op->cg_x64.as<double2>();
-        v.u32 = op->cg_u32.as<float2>();
-    } else {
-        v.x = x;
-        v.u32 = op->cg_u.as<float2>();
-    }
+    v.c = op->cg_c.as<AT>();
+    if constexpr (std::is_same<VT, double2>::value) v.x = op->cg_x64.as<double2>();
+    else v.x = x;
+    if constexpr (std::is_same<VT, AT>::value) v.ua = v.u;
+    else v.ua = op->cg_u32.as<AT>();
     return v;
 }
 
-template CgVecs<float2> op_cg_vecs<float2>(msr_normalop*, float2*);
-template CgVecs<double2> op_cg_vecs<double2>(msr_normalop*, float2*);
+template CgVecs<float2, float2> op_cg_vecs<float2, float2>(msr_normalop*, float2*);
+template CgVecs<double2, float2> op_cg_vecs<double2, float2>(msr_normalop*, float2*);
+template CgVecs<double2, double2> op_cg_vecs<double2, double2>(msr_normalop*, float2*);
+
+// One key per captured CG graph: everything the captured kernels hold.
+std::vector<const void*> op_graph_key(const msr_normalop* op) {
+    std::vector<const void*> key;
+    for (const DevBuf* d : {&op->cg_r, &op->cg_u, &op->cg_c, &op->cg_scal, &op->cg_part,
+                            &op->cg_x64, &op->cg_u32, &op->T, &op->T2, &op->partial})
+        key.push_back(d->p);
+    return key;
+}
+
+__global__ void f32_to_f64_kernel(const float2* __restrict__ a, double2* __restrict__ b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = make_double2(a[i].x, a[i].y);
+}
 
 // one CG iteration with device scalars (all kernels skip once done is set)
-template <class VT>
-static void cg_iteration(msr_normalop* op, const CgVecs<VT>& v, bool percoil, hipStream_t st) {
+template <class VT, class AT>
+static void cg_iteration(msr_normalop* op, const CgVecs<VT, AT>& v, hipStream_t st) {
+    constexpr bool a64 = std::is_same<AT, double2>::value;
     const int64_t n = op->V * op->K;
     auto* S = op->cg_scal.as<CgScalars>();
     double* part = op->cg_part.as<double>();
     launch_cg_u(v, n, S, st);
-    if (percoil) {
-        op_apply_percoil(op, v.c, v.u32, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true, st);
-        launch_cg_reduce_dot(v.c, 1, n, v, 1.0f, part, S, st);
-    } else {
-        op_passes(op, v.u32, st, &S->done);
-        launch_cg_reduce_dot(op->partial.as<float2>(), op->G, n, v, op_scale(op), part, S, st);
-    }
+    op_passes(op, v.ua, a64, st, &S->done);
+    launch_cg_reduce_dot(op->partial.as<AT>(), op->G, n, v, op_scale(op), part, S, st);
     launch_cg_xr(v, n, part, S, st);
 }
 
-template <class VT>
-static void cg_prologue(msr_normalop* op, const CgVecs<VT>& v, float2* x, const float2* b,
+// c = A x0 in the apply precision
+template <class VT, class AT>
+static void cg_apply_x0(msr_normalop* op, const CgVecs<VT, AT>& v, const float2* x,
+                        hipStream_t st) {
+    if constexpr (std::is_same<AT, double2>::value) {
+        const int64_t n = op->V * op->K;
+        int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);
+        hipLaunchKernelGGL(f32_to_f64_kernel, dim3(grid), dim3(256), 0, st, x, v.ua, n);
+        MSR_CHECK_LAUNCH();
+        op_passes(op, v.ua, true, st, nullptr);
+        launch_epilogue<double2>(op->partial.as<double2>(), op->G, n, v.c, make_double2(1.0, 0.0),
+                                 make_double2(0.0, 0.0), true, op_scale(op), st, nullptr);
+    } else {
+        op_apply(op, v.c, x, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true, st);
+    }
+}
+
+template <class VT, class AT>
+static void cg_prologue(msr_normalop* op, const CgVecs<VT, AT>& v, float2* x, const float2* b,
                         int maxiter, double reltol, double abstol, bool initially_zero,
                         hipStream_t st) {
     const int64_t n = op->V * op->K;
@@ -338,38 +405,34 @@ static void cg_prologue(msr_normalop* op, const CgVecs<VT>& v, float2* x, const 
         launch_cg_start(b, x, v, n, true, op->cg_part.as<double>(), S, reltol, abstol, maxiter,
                         false, st);
     } else {
-        op_apply(op, v.c, x, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true, st);
+        cg_apply_x0(op, v, x, st);
         launch_cg_start(b, x, v, n, false, op->cg_part.as<double>(), S, reltol, abstol, maxiter,
                         true, st);
     }
 }
 
-template <class VT>
-static void cg_epilogue(msr_normalop* op, const CgVecs<VT>& v, float2* x, hipStream_t st) {
-    if constexpr (std::is_same<VT, double2>::value)
-        launch_to_f32(v.x, x, op->V * op->K, st);
+template <class VT, class AT>
+static void cg_epilogue(msr_normalop* op, const CgVecs<VT, AT>& v, float2* x, hipStream_t st) {
+    if constexpr (std::is_same<VT, double2>::value) launch_to_f32(v.x, x, op->V * op->K, st);
 }
 
-template <class VT>
+template <class VT, class AT>
 static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
-                   double abstol, bool initially_zero, int mode, hipStream_t st) {
-    const CgVecs<VT> v = op_cg_vecs<VT>(op, x);
+                   double abstol, bool initially_zero, int mode, int prec, hipStream_t st) {
+    const CgVecs<VT, AT> v = op_cg_vecs<VT, AT>(op, x);
     auto* S = op->cg_scal.as<CgScalars>();
     if (mode == MSR_CG_DEVICE) {
         cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
-        for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, false, st);
+        for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, st);
         cg_epilogue(op, v, x, st);
     } else if (mode == MSR_CG_GRAPH) {
-        const int fp = std::is_same<VT, double2>::value ? 1 : 0;
-        // the captured kernels hold the CG buffer addresses: they are part of
-        // the key (another call with the other vector precision re-allocates them)
-        std::vector<const void*> key = {x, b, (const void*)(intptr_t)(initially_zero ? 1 : 2), st};
-        for (const DevBuf* d : {&op->cg_r, &op->cg_u, &op->cg_c, &op->cg_scal, &op->cg_part,
-                                &op->cg_x64, &op->cg_u32})
-            key.push_back(d->p);
+        // the captured kernels hold the buffer addresses: they are part of the
+        // key (another call may re-allocate them for another precision)
+        std::vector<const void*> key = op_graph_key(op);
+        key.insert(key.end(), {x, b, (const void*)(intptr_t)(initially_zero ? 1 : 2), st});
         const bool hit = op->graph && op->graph_key == key && op->graph_maxiter == maxiter &&
                          op->graph_tol[0] == reltol && op->graph_tol[1] == abstol &&
-                         op->graph_fp64 == fp;
+                         op->graph_fp64 == prec;
         if (!hit) {
             if (op->graph) {
                 MSR_HIP(hipStreamSynchronize(op->graph_stream));  // a replay may still run
@@ -382,7 +445,7 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
             MSR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
                 cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
-                for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, false, st);
+                for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, st);
                 cg_epilogue(op, v, x, st);
             } catch (...) {
                 (void)hipStreamEndCapture(st, &g);
@@ -400,40 +463,52 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
             op->graph_maxiter = maxiter;
             op->graph_tol[0] = reltol;
             op->graph_tol[1] = abstol;
-            op->graph_fp64 = fp;
+            op->graph_fp64 = prec;
         }
         MSR_HIP(hipGraphLaunch(op->graph, st));
     } else if (mode == MSR_CG_HOST) {
         // IterativeSolvers-style: the step length and the residual norm are
         // read back to the host every iteration; applies run coil by coil.
-        const int64_t n = op->V * op->K;
-        double* part = op->cg_part.as<double>();
-        cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
-        CgScalars hs;
-        for (int it = 0; it < maxiter; ++it) {
-            MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
-            MSR_HIP(hipStreamSynchronize(st));
-            if (hs.done) break;
-            launch_cg_u(v, n, S, st);
-            op_apply_percoil(op, v.c, v.u32, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true,
-                             st);
-            launch_cg_reduce_dot(v.c, 1, n, v, 1.0f, part, S, st);
-            MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
-            MSR_HIP(hipStreamSynchronize(st));
-            launch_cg_xr(v, n, part, S, st);
+        if constexpr (std::is_same<AT, float2>::value) {
+            const int64_t n = op->V * op->K;
+            double* part = op->cg_part.as<double>();
+            cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
+            CgScalars hs;
+            for (int it = 0; it < maxiter; ++it) {
+                MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+                MSR_HIP(hipStreamSynchronize(st));
+                if (hs.done) break;
+                launch_cg_u(v, n, S, st);
+                op_apply_percoil(op, v.c, v.ua, make_float2(1.f, 0.f), make_float2(0.f, 0.f),
+                                 true, st);
+                launch_cg_reduce_dot(v.c, 1, n, v, 1.0, part, S, st);
+                MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+                MSR_HIP(hipStreamSynchronize(st));
+                launch_cg_xr(v, n, part, S, st);
+            }
+            cg_epilogue(op, v, x, st);
+        } else {
+            fail(MSR_ERR_UNSUPPORTED, "MSR_CG_HOST runs the reference-order FP32 apply; "
+                                      "use MSR_PREC_F32 or MSR_PREC_MIXED");
         }
-        cg_epilogue(op, v, x, st);
     } else {
         fail(MSR_ERR_ARGUMENT, "unknown CG mode %d", mode);
     }
+    (void)prec;
 }
 
 void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
-           double abstol, bool initially_zero, int mode, bool fp64, int* iters_out,
+           double abstol, bool initially_zero, int mode, int prec, int* iters_out,
            double* res_out, hipStream_t st) {
-    op_cg_buffers(op, fp64);
-    if (fp64) cg_run<double2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, st);
-    else cg_run<float2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, st);
+    MSR_REQUIRE(prec == MSR_PREC_F32 || prec == MSR_PREC_MIXED || prec == MSR_PREC_F64,
+                MSR_ERR_ARGUMENT, "unknown CG precision %d", prec);
+    op_cg_buffers(op, prec);
+    if (prec == MSR_PREC_F64)
+        cg_run<double2, double2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, prec, st);
+    else if (prec == MSR_PREC_MIXED)
+        cg_run<double2, float2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, prec, st);
+    else
+        cg_run<float2, float2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, prec, st);
     if (iters_out || res_out) {
         CgScalars hs;
         MSR_HIP(hipMemcpyAsync(&hs, op->cg_scal.p, sizeof(CgScalars), hipMemcpyDeviceToHost, st));

@@ -2,6 +2,8 @@
 #pragma once
 
 #include "apply2d.hpp"
+#include <vector>
+
 #include "internal.hpp"
 #include "vecops.hpp"
 
@@ -14,19 +16,23 @@ void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, 
 void op_alloc_scratch(msr_normalop* op);
 void op_apply(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
               bool beta_zero, hipStream_t st);
+void op_apply_f64(msr_normalop* op, double2* x, const double2* b, double2 alpha, double2 beta,
+                  bool beta_zero, hipStream_t st);
 void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
                       bool beta_zero, hipStream_t st);
 void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hipStream_t st);
 void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
-           double abstol, bool initially_zero, int mode, bool fp64, int* iters_out,
+           double abstol, bool initially_zero, int mode, int prec, int* iters_out,
            double* res_out, hipStream_t st);
 
 // pieces used by the batch engine
-Apply2DArgs args2d(msr_normalop* op);                 // b = skip = nullptr
-void op_passes(msr_normalop* op, const float2* b, hipStream_t st, const int* skip);
-float op_scale(const msr_normalop* op);               // 1 / Toeplitz grid cells
-void op_cg_buffers(msr_normalop* op, bool fp64);      // allocates; not during capture
-template <class VT>
-CgVecs<VT> op_cg_vecs(msr_normalop* op, float2* x);   // x: caller's FP32 solution
+Apply2DArgs args2d(msr_normalop* op, bool f64);        // b = skip = nullptr
+void op_ensure_scratch(msr_normalop* op, bool f64);    // grow-only; not during capture
+void op_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st, const int* skip);
+double op_scale(const msr_normalop* op);               // 1 / Toeplitz grid cells
+void op_cg_buffers(msr_normalop* op, int prec);        // allocates; not during capture
+template <class VT, class AT>
+CgVecs<VT, AT> op_cg_vecs(msr_normalop* op, float2* x);  // x: caller's FP32 solution
+std::vector<const void*> op_graph_key(const msr_normalop* op);  // buffer addresses
 
 }  // namespace msr

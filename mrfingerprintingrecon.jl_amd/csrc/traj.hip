@@ -11,6 +11,7 @@
 
 msr_ctx::~msr_ctx() {
     for (auto& kv : twiddles) (void)hipFree(kv.second);
+    for (auto& kv : twiddles64) (void)hipFree(kv.second);
     for (auto& kv : deapod) (void)hipFree(kv.second);
 }
 
@@ -18,11 +19,9 @@ namespace msr {
 
 void ensure_device(const msr_ctx* ctx) { MSR_HIP(hipSetDevice(ctx->device)); }
 
-const float2* twiddle_table(msr_ctx* ctx, int n) {
-    std::lock_guard<std::mutex> guard(ctx->lock);
-    auto it = ctx->twiddles.find(n);
-    if (it != ctx->twiddles.end()) return it->second;
-    std::vector<float2> h(n);
+// exp(-2 pi i m / n), m < n, in double; exact at multiples of pi/2
+static std::vector<double2> twiddles_f64(int n) {
+    std::vector<double2> h(n);
     for (int m = 0; m < n; ++m) {
         // exp(-2 pi i m / n) in double, rounded once; exact at multiples of pi/2
         const int64_t q4 = 4LL * m;
@@ -36,12 +35,34 @@ const float2* twiddle_table(msr_ctx* ctx, int n) {
             c = std::cos(a);
             s = std::sin(a);
         }
-        h[m] = make_float2((float)c, (float)s);
+        h[m] = make_double2(c, s);
     }
+    return h;
+}
+
+const float2* twiddle_table(msr_ctx* ctx, int n) {
+    std::lock_guard<std::mutex> guard(ctx->lock);
+    auto it = ctx->twiddles.find(n);
+    if (it != ctx->twiddles.end()) return it->second;
+    const std::vector<double2> h64 = twiddles_f64(n);
+    std::vector<float2> h(n);
+    for (int m = 0; m < n; ++m) h[m] = make_float2((float)h64[m].x, (float)h64[m].y);  // rounded once
     float2* d = nullptr;
     MSR_HIP(hipMalloc(&d, sizeof(float2) * n));
     MSR_HIP(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
     ctx->twiddles[n] = d;
+    return d;
+}
+
+const double2* twiddle_table64(msr_ctx* ctx, int n) {
+    std::lock_guard<std::mutex> guard(ctx->lock);
+    auto it = ctx->twiddles64.find(n);
+    if (it != ctx->twiddles64.end()) return it->second;
+    const std::vector<double2> h = twiddles_f64(n);
+    double2* d = nullptr;
+    MSR_HIP(hipMalloc(&d, sizeof(double2) * n));
+    MSR_HIP(hipMemcpy(d, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    ctx->twiddles64[n] = d;
     return d;
 }
 

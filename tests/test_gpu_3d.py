@@ -92,9 +92,10 @@ def test_kernel3d_and_backprojection3d(torch, msr):
 def test_recon3d_solve_matches_fp64(torch, msr):
     """The kmask-truncated Toeplitz operator is indefinite (its smallest
     eigenvalue is about -2e-3 of the largest on this 16^3 koosh-ball case,
-    computed with ARPACK on the FP64 oracle), so CG amplifies the FP32 rounding
-    of each apply more the longer it runs: solve parity is asserted at 1e-4 for
-    8 iterations and bounded at 1e-3 (and printed) for 15."""
+    computed with ARPACK on the FP64 oracle), so CG amplifies the rounding of
+    each apply more the longer it runs: an FP32 apply reaches ~2e-4 after 15
+    iterations.  The default FP64 apply holds the solve to 1e-7 at 8 and 15
+    iterations; the FP32-apply ("mixed") result is printed."""
     N = (16, 16, 16)
     trj, U, cm, x = kooshball_case(N, 2, 4, 10, 8, seed=2)
     data = toeplitz.forward(x.astype(np.complex128), trj, U, cm.astype(np.complex128)).astype(np.complex64)
@@ -106,8 +107,11 @@ def test_recon3d_solve_matches_fp64(torch, msr):
     op = toeplitz.NormalOp(N, lam_d, km_d, cm.astype(np.complex128))
     bd = b.cpu().numpy().astype(np.complex128).reshape(-1)
     for mode in ("device", "graph"):
-        for iters, tol in ((8, 1e-4), (15, 1e-3)):
+        for iters in (8, 15):
             xr = msr.cg(A, b.reshape(-1), maxiter=iters, reltol=0.0, mode=mode).cpu().numpy()
             ref = ocg.cg(op, bd, maxiter=iters, reltol=0.0)
-            print(f"3d solve ({mode}, {iters} it) rel err {rel_err(xr, ref):.3e}", flush=True)
-            assert rel_err(xr, ref) < tol
+            xm = msr.cg(A, b.reshape(-1), maxiter=iters, reltol=0.0, mode=mode,
+                        precision="mixed").cpu().numpy()
+            print(f"3d solve ({mode}, {iters} it) rel err {rel_err(xr, ref):.3e} "
+                  f"(FP32 apply {rel_err(xm, ref):.3e})", flush=True)
+            assert rel_err(xr, ref) < 1e-7
