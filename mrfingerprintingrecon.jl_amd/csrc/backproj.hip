@@ -89,7 +89,7 @@ void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
     }
     float2* G = work.as<float2>();
     float2* H = reinterpret_cast<float2*>(reinterpret_cast<char*>(work.p) + gbytes);
-    spread(traj, plan, src, nch, G, H, st);
+    spread(traj, plan, src, nch, G, H, false, st);
 
     const bool combine = cmaps != nullptr;
     float2* img_out = combine ? G : (float2*)out;  // per-coil output is (C, K, N...)

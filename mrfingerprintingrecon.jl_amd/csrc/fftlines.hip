@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
     CT* out = static_cast<CT*>(p.out);
     load_twiddles<NF>(s_tw, tw);
     const int64_t l0 = (int64_t)blockIdx.x * F;
-    if (p.isel == SEL_FIRST && !p.win) {
+    if (p.isel == SEL_FIRST && !p.win && !p.win64) {
         for (int e = threadIdx.x; e < F * NF; e += BLK) {
             const int l = e / NF, i = e - l * NF;
             const int64_t line = l0 + l;
@@ -48,7 +48,8 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
             const int64_t line = l0 + l;
             if (line < p.nlines) {
                 CT v = in[line * p.in_pitch + i];
-                if (p.win) v = cscale(v, (RT)p.win[i]);
+                if (p.win64) v = cscale(v, (RT)p.win64[i]);
+                else if (p.win) v = cscale(v, (RT)p.win[i]);
                 s[l * LD + lds_pad(select_bin(i, p.lin, NF, p.isel))] = v;
             }
         }
@@ -61,7 +62,8 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
         const int64_t line = l0 + l;
         if (line >= p.nlines) continue;
         CT v = s[l * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
-        if (p.w) v = cscale(v, (RT)p.w[j]);
+        if (p.w64) v = cscale(v, (RT)p.w64[j]);
+        else if (p.w) v = cscale(v, (RT)p.w[j]);
         out[line * p.out_pitch + j] = v;
     }
 }
@@ -87,7 +89,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
     load_twiddles<NF>(s_tw, tw);
     const int64_t col0 = (int64_t)blockIdx.x * WC;
     const int64_t b = blockIdx.y;
-    if (p.isel == SEL_FIRST && !p.win) {
+    if (p.isel == SEL_FIRST && !p.win && !p.win64) {
         for (int e = threadIdx.x; e < WC * NF; e += BLK) {
             const int wc = e % WC, i = e / WC;
             const int64_t col = col0 + wc;
@@ -103,7 +105,8 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
             const int64_t col = col0 + wc;
             if (col < p.ncols) {
                 CT v = in[b * p.in_bstride + i * p.in_stride + col];
-                if (p.win) v = cscale(v, (RT)p.win[i]);
+                if (p.win64) v = cscale(v, (RT)p.win64[i]);
+                else if (p.win) v = cscale(v, (RT)p.win[i]);
                 s[wc * LD + lds_pad(select_bin(i, p.lin, NF, p.isel))] = v;
             }
         }
@@ -117,23 +120,25 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
         const int64_t col = col0 + wc;
         if (col >= p.ncols) continue;
         CT v = s[wc * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
-        if (p.w) v = cscale(v, (RT)p.w[j]);
-        if constexpr (sizeof(CT) == 16) {
-            out[b * p.out_bstride + j * p.out_stride + col] = v;
-        } else if (!p.lam) {
+        if (p.w64) v = cscale(v, (RT)p.w64[j]);
+        else if (p.w) v = cscale(v, (RT)p.w[j]);
+        if (!p.lam) {
             out[b * p.out_bstride + j * p.out_stride + col] = v;
         } else {
+            // dense Toeplitz kernel, rounded once to ComplexF32 / Float32
+            const int64_t pp = b + p.p0;
             int64_t off, mi;
             if (!p.lam3d) {
-                off = (col * p.P + b) * p.lout + j;
+                off = (col * p.P + pp) * p.lout + j;
                 mi = (int64_t)j * p.nx2 + col;
             } else {
-                off = (col * p.P + b) * p.lout + j;            // col = qy*nx2 + qx
+                off = (col * p.P + pp) * p.lout + j;           // col = qy*nx2 + qx
                 mi = (int64_t)j * p.ny2 * p.nx2 + col;         // j = qz
             }
             const bool on = p.mask[mi] != 0;
-            if (p.lam_real) reinterpret_cast<float*>(p.lam)[off] = on ? v.x : 0.f;
-            else reinterpret_cast<float2*>(p.lam)[off] = on ? v : make_float2(0.f, 0.f);
+            if (p.lam_real) reinterpret_cast<float*>(p.lam)[off] = on ? (float)v.x : 0.f;
+            else reinterpret_cast<float2*>(p.lam)[off] = on ? make_float2((float)v.x, (float)v.y)
+                                                           : make_float2(0.f, 0.f);
         }
     }
 }
@@ -170,7 +175,6 @@ void run_rows(const LinePass& p, msr_ctx* ctx, hipStream_t st) {
 }
 
 void run_cols(const LinePass& p, msr_ctx* ctx, hipStream_t st) {
-    MSR_REQUIRE(!(p.f64 && p.lam), MSR_ERR_ERROR, "the dense-kernel store is ComplexF32 only");
     if (p.f64) run_cols_t<double2>(p, twiddle_table64(ctx, p.nf), st);
     else run_cols_t<float2>(p, twiddle_table(ctx, p.nf), st);
 }

@@ -19,15 +19,17 @@ enum { SEL_FIRST = 0, SEL_CENTER = 1, SEL_FFTORDER = 2 };
 struct LinePass {
     const void* in = nullptr;   // ComplexF32, or ComplexF64 when f64
     void* out = nullptr;
-    int f64 = 0;                // arithmetic / data precision (the dense-kernel store is FP32 only)
+    int f64 = 0;                // arithmetic / data precision (the dense-kernel store rounds to FP32)
     int nf = 0;          // FFT length
     int dir = -1;        // -1 forward, +1 inverse
     int lin = 0, lout = 0, sel = SEL_FIRST;
     const float* w = nullptr;
+    const double* w64 = nullptr;  // FP64 output weights (used instead of w when set)
     // input placement: input i goes to bin select(i, lin) scaled by win[i]
     // (SEL_FIRST: bins 0..lin-1, the rest zero; SEL_CENTER: frequency i - lin/2)
     int isel = SEL_FIRST;
     const float* win = nullptr;
+    const double* win64 = nullptr;
     // rows: line l at in + l*in_pitch (contiguous), out + l*out_pitch
     int64_t nlines = 0, in_pitch = 0, out_pitch = 0;
     // cols: lines (b, col) with col < ncols adjacent in memory
@@ -38,6 +40,7 @@ struct LinePass {
     void* lam = nullptr;
     const uint8_t* mask = nullptr;
     int P = 0, lam_real = 0;
+    int p0 = 0;                   // packed pair of batch line b is p0 + b
     int64_t nx2 = 0;
     // 3D variant of the dense store: col = qy*nx2 + qx, j = qz,
     // lam[((qy*nx2 + qx)*P + p)*nz2 + qz], mask[(qz*ny2 + qy)*nx2 + qx]

@@ -16,6 +16,7 @@ struct msr_ctx {
     std::map<int, float2*> twiddles;     // N -> device table exp(-2 pi i m/N), ComplexF32
     std::map<int, double2*> twiddles64;  // the same in ComplexF64
     std::map<std::tuple<int, int, int>, float*> deapod;  // (n_os, n_out, sel) -> 1/phihat
+    std::map<std::tuple<int, int, int>, double*> deapod64;  // the same in FP64
     std::mutex lock;  // guards the two lazily filled tables (calls may come from several threads)
     ~msr_ctx();
 };

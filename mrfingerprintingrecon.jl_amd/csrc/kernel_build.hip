@@ -9,9 +9,10 @@
 //
 // Kernel: for each packed pair (j <= k) the PSF
 //   p_jk[r] = sum_m conj(U[t_m, j]) U[t_m, k] exp(+2 pi i k_m . r),  r on the 2N grid (FFT order)
-// is one channel of a type-1 NUFFT (all P channels spread together onto the
+// is one channel of a type-1 NUFFT (the P channels spread together onto the
 // sigma = 2 grid 4N, inverse FFTs with FFT-order crop and deapodisation), then
-// lambda_jk = FFT_2N(p_jk), stored densely with zeros off the kmask.  Real U
+// lambda_jk = FFT_2N(p_jk), stored densely with zeros off the kmask.  All of it
+// runs in FP64 and is rounded once when stored.  Real U
 // stores Re(lambda) (the reference's brfft path, src/NFFTNormalOp.jl:206-219,
 // up to the PSF's Nyquist planes).
 #include <algorithm>
@@ -188,128 +189,151 @@ void op_build_from_traj(msr_normalop* op, const msr_traj* traj, const void* U, i
     src.u_real = u_real;
     src.nt = traj->nt;
     src.K = op->K;
-    src.pair_j = pairs.as<int>();
-    src.pair_k = pairs.as<int>() + P;
 
+    // The kernel is computed in FP64 (FP64 spreading accumulation, weights,
+    // deapodisation and FFTs) and rounded once to ComplexF32 / Float32 when
+    // stored: FP32 arithmetic here perturbs the operator by ~1e-6, which the
+    // CG iterations of an indefinite truncated operator amplify beyond 1e-4.
+    // Pairs are processed in chunks so the FP64 grids stay within a budget.
+    const bool f64 = true;
+    const size_t es = f64 ? sizeof(double2) : sizeof(float2);
     const SpreadPlan& plan = spread_plan(traj, n4, st);
+    const size_t per_ch = es * (size_t)G4 + std::max(es * (size_t)(G4 / 2),
+                                                     spread_workspace_bytes(plan, 1, f64));
+    const size_t budget = (size_t)4 << 30;
+    const int pc = (int)std::max<size_t>(1, std::min<size_t>((size_t)P, budget / per_ch));
     DevBuf A, B;
-    A.alloc(sizeof(float2) * (size_t)P * G4);
-    B.alloc(std::max(sizeof(float2) * (size_t)P * (G4 / 2), spread_workspace_bytes(plan, P)));
-    spread(traj, plan, src, P, A.as<float2>(), B.as<float2>(), st);
+    A.alloc(es * (size_t)pc * G4);
+    B.alloc(std::max(es * (size_t)pc * (G4 / 2), spread_workspace_bytes(plan, pc, f64)));
 
-    const size_t es = op->lam_real ? sizeof(float) : sizeof(float2);
-    op->lam.alloc(es * (size_t)P * G2);
+    const size_t ls = op->lam_real ? sizeof(float) : sizeof(float2);
+    op->lam.alloc(ls * (size_t)P * G2);
 
-    // x: inverse FFT over 4Nx, FFT-order crop to 2Nx, deapodise
-    LinePass px;
-    px.in = A.as<float2>();
-    px.out = B.as<float2>();
-    px.nf = (int)n4[0];
-    px.dir = +1;
-    px.lin = (int)n4[0];
-    px.lout = (int)n2[0];
-    px.sel = SEL_FFTORDER;
-    px.w = deapod_table(ctx, (int)n4[0], (int)n2[0], SEL_FFTORDER);
-    px.nlines = (int64_t)P * n4[1] * n4[2];
-    px.in_pitch = n4[0];
-    px.out_pitch = n2[0];
-    run_rows(px, ctx, st);
-    // B: [p][z4][y4][x2]
-    LinePass py;
-    py.in = B.as<float2>();
-    py.out = A.as<float2>();
-    py.nf = (int)n4[1];
-    py.dir = +1;
-    py.lin = (int)n4[1];
-    py.lout = (int)n2[1];
-    py.sel = SEL_FFTORDER;
-    py.w = deapod_table(ctx, (int)n4[1], (int)n2[1], SEL_FFTORDER);
-    py.nb = (int64_t)P * n4[2];
-    py.ncols = n2[0];
-    py.in_bstride = n4[1] * n2[0];
-    py.in_stride = n2[0];
-    py.out_bstride = n2[1] * n2[0];
-    py.out_stride = n2[0];
-    run_cols(py, ctx, st);
-    // A: [p][z4][y2][x2]
-    float2* psf = A.as<float2>();
-    float2* other = B.as<float2>();
-    if (d == 3) {
-        LinePass pz;
-        pz.in = A.as<float2>();
-        pz.out = B.as<float2>();
-        pz.nf = (int)n4[2];
-        pz.dir = +1;
-        pz.lin = (int)n4[2];
-        pz.lout = (int)n2[2];
-        pz.sel = SEL_FFTORDER;
-        pz.w = deapod_table(ctx, (int)n4[2], (int)n2[2], SEL_FFTORDER);
-        pz.nb = P;
-        pz.ncols = n2[1] * n2[0];
-        pz.in_bstride = n4[2] * n2[1] * n2[0];
-        pz.in_stride = n2[1] * n2[0];
-        pz.out_bstride = G2;
-        pz.out_stride = n2[1] * n2[0];
-        run_cols(pz, ctx, st);
-        psf = B.as<float2>();
-        other = A.as<float2>();
-    }
-    // psf: [p][z2][y2][x2], FFT order.  Forward FFT over the 2N grid.
-    LinePass fx;
-    fx.in = psf;
-    fx.out = other;
-    fx.nf = (int)n2[0];
-    fx.dir = -1;
-    fx.lin = (int)n2[0];
-    fx.lout = (int)n2[0];
-    fx.sel = SEL_FIRST;
-    fx.nlines = (int64_t)P * n2[1] * n2[2];
-    fx.in_pitch = n2[0];
-    fx.out_pitch = n2[0];
-    run_rows(fx, ctx, st);
-    LinePass fy;
-    fy.in = other;
-    fy.nf = (int)n2[1];
-    fy.dir = -1;
-    fy.lin = (int)n2[1];
-    fy.lout = (int)n2[1];
-    fy.sel = SEL_FIRST;
-    fy.nb = (int64_t)P * n2[2];
-    fy.ncols = n2[0];
-    fy.in_bstride = n2[1] * n2[0];
-    fy.in_stride = n2[0];
-    if (d == 2) {
-        // final pass stores the dense kernel: lam[(qx*P + p)*ny2 + qy]
-        fy.lam = op->lam.p;
-        fy.mask = mask.as<uint8_t>();
-        fy.P = P;
-        fy.lam_real = op->lam_real;
-        fy.nx2 = n2[0];
-        run_cols(fy, ctx, st);
-    } else {
-        fy.out = psf;
-        fy.out_bstride = n2[1] * n2[0];
-        fy.out_stride = n2[0];
-        run_cols(fy, ctx, st);
-        LinePass fz;
-        fz.in = psf;
-        fz.nf = (int)n2[2];
-        fz.dir = -1;
-        fz.lin = (int)n2[2];
-        fz.lout = (int)n2[2];
-        fz.sel = SEL_FIRST;
-        fz.nb = P;
-        fz.ncols = n2[1] * n2[0];
-        fz.in_bstride = G2;
-        fz.in_stride = n2[1] * n2[0];
-        fz.lam = op->lam.p;
-        fz.mask = mask.as<uint8_t>();
-        fz.P = P;
-        fz.lam_real = op->lam_real;
-        fz.nx2 = n2[0];
-        fz.ny2 = n2[1];
-        fz.lam3d = 1;
-        run_cols(fz, ctx, st);
+    for (int p0 = 0; p0 < P; p0 += pc) {
+        const int np = std::min(pc, P - p0);
+        src.pair_j = pairs.as<int>() + p0;
+        src.pair_k = pairs.as<int>() + P + p0;
+        spread(traj, plan, src, np, A.p, B.p, f64, st);
+
+        // x: inverse FFT over 4Nx, FFT-order crop to 2Nx, deapodise
+        LinePass px;
+        px.in = A.p;
+        px.out = B.p;
+        px.f64 = f64;
+        px.nf = (int)n4[0];
+        px.dir = +1;
+        px.lin = (int)n4[0];
+        px.lout = (int)n2[0];
+        px.sel = SEL_FFTORDER;
+        px.w64 = deapod_table64(ctx, (int)n4[0], (int)n2[0], SEL_FFTORDER);
+        px.nlines = (int64_t)np * n4[1] * n4[2];
+        px.in_pitch = n4[0];
+        px.out_pitch = n2[0];
+        run_rows(px, ctx, st);
+        // B: [p][z4][y4][x2]
+        LinePass py;
+        py.in = B.p;
+        py.out = A.p;
+        py.f64 = f64;
+        py.nf = (int)n4[1];
+        py.dir = +1;
+        py.lin = (int)n4[1];
+        py.lout = (int)n2[1];
+        py.sel = SEL_FFTORDER;
+        py.w64 = deapod_table64(ctx, (int)n4[1], (int)n2[1], SEL_FFTORDER);
+        py.nb = (int64_t)np * n4[2];
+        py.ncols = n2[0];
+        py.in_bstride = n4[1] * n2[0];
+        py.in_stride = n2[0];
+        py.out_bstride = n2[1] * n2[0];
+        py.out_stride = n2[0];
+        run_cols(py, ctx, st);
+        // A: [p][z4][y2][x2]
+        void* psf = A.p;
+        void* other = B.p;
+        if (d == 3) {
+            LinePass pz;
+            pz.in = A.p;
+            pz.out = B.p;
+            pz.f64 = f64;
+            pz.nf = (int)n4[2];
+            pz.dir = +1;
+            pz.lin = (int)n4[2];
+            pz.lout = (int)n2[2];
+            pz.sel = SEL_FFTORDER;
+            pz.w64 = deapod_table64(ctx, (int)n4[2], (int)n2[2], SEL_FFTORDER);
+            pz.nb = np;
+            pz.ncols = n2[1] * n2[0];
+            pz.in_bstride = n4[2] * n2[1] * n2[0];
+            pz.in_stride = n2[1] * n2[0];
+            pz.out_bstride = G2;
+            pz.out_stride = n2[1] * n2[0];
+            run_cols(pz, ctx, st);
+            psf = B.p;
+            other = A.p;
+        }
+        // psf: [p][z2][y2][x2], FFT order.  Forward FFT over the 2N grid.
+        LinePass fx;
+        fx.in = psf;
+        fx.out = other;
+        fx.f64 = f64;
+        fx.nf = (int)n2[0];
+        fx.dir = -1;
+        fx.lin = (int)n2[0];
+        fx.lout = (int)n2[0];
+        fx.sel = SEL_FIRST;
+        fx.nlines = (int64_t)np * n2[1] * n2[2];
+        fx.in_pitch = n2[0];
+        fx.out_pitch = n2[0];
+        run_rows(fx, ctx, st);
+        LinePass fy;
+        fy.in = other;
+        fy.f64 = f64;
+        fy.nf = (int)n2[1];
+        fy.dir = -1;
+        fy.lin = (int)n2[1];
+        fy.lout = (int)n2[1];
+        fy.sel = SEL_FIRST;
+        fy.nb = (int64_t)np * n2[2];
+        fy.ncols = n2[0];
+        fy.in_bstride = n2[1] * n2[0];
+        fy.in_stride = n2[0];
+        if (d == 2) {
+            // final pass stores the dense kernel: lam[(qx*P + p)*ny2 + qy]
+            fy.lam = op->lam.p;
+            fy.mask = mask.as<uint8_t>();
+            fy.P = P;
+            fy.p0 = p0;
+            fy.lam_real = op->lam_real;
+            fy.nx2 = n2[0];
+            run_cols(fy, ctx, st);
+        } else {
+            fy.out = psf;
+            fy.out_bstride = n2[1] * n2[0];
+            fy.out_stride = n2[0];
+            run_cols(fy, ctx, st);
+            LinePass fz;
+            fz.in = psf;
+            fz.f64 = f64;
+            fz.nf = (int)n2[2];
+            fz.dir = -1;
+            fz.lin = (int)n2[2];
+            fz.lout = (int)n2[2];
+            fz.sel = SEL_FIRST;
+            fz.nb = np;
+            fz.ncols = n2[1] * n2[0];
+            fz.in_bstride = G2;
+            fz.in_stride = n2[1] * n2[0];
+            fz.lam = op->lam.p;
+            fz.mask = mask.as<uint8_t>();
+            fz.P = P;
+            fz.p0 = p0;
+            fz.lam_real = op->lam_real;
+            fz.nx2 = n2[0];
+            fz.ny2 = n2[1];
+            fz.lam3d = 1;
+            run_cols(fz, ctx, st);
+        }
     }
     MSR_HIP(hipStreamSynchronize(st));  // temporaries are freed on return
 }

@@ -8,6 +8,8 @@
 // per dim is accumulated in a (TB + 8)^d LDS block.
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -19,23 +21,29 @@ namespace msr {
 template <int D> struct TileShape;
 template <> struct TileShape<2> {
     static constexpr int X = 32, Y = 32, Z = 1;
-    static constexpr int CHG = 8;   // channels per workgroup (one wavefront each)
+    static constexpr int CHG = 8;   // channels per workgroup (one wavefront each), ComplexF32
 };
 template <> struct TileShape<3> {
     static constexpr int X = 16, Y = 8, Z = 8;
     static constexpr int CHG = 2;
 };
+// channels per workgroup for the accumulation type AT (LDS: ComplexF64 tiles
+// take twice the space)
+template <int D, class AT>
+constexpr int chg_of() { return sizeof(AT) == 16 ? TileShape<D>::CHG / 2 : TileShape<D>::CHG; }
 
-__device__ __forceinline__ float es_phi(float t) {
-    const float z = t * (2.0f / SPREAD_W);
-    const float a = 1.0f - z * z;
-    return a > 0.0f ? expf(SPREAD_BETA * (sqrtf(a) - 1.0f)) : 0.0f;
+template <class RT>
+__device__ __forceinline__ RT es_phi(RT t) {
+    const RT z = t * (RT)(2.0 / SPREAD_W);
+    const RT a = (RT)1 - z * z;
+    return a > (RT)0 ? exp((RT)SPREAD_BETA * (sqrt(a) - (RT)1)) : (RT)0;
 }
 
-__device__ __forceinline__ void point_cell(float k, int64_t n, int64_t& cell, float& frac) {
+template <class RT>
+__device__ __forceinline__ void point_cell(float k, int64_t n, int64_t& cell, RT& frac) {
     const double s = (double)k * (double)n;
     const double fl = floor(s);
-    frac = (float)(s - fl);
+    frac = (RT)(s - fl);
     int64_t c = (int64_t)fl % n;
     if (c < 0) c += n;
     cell = c;
@@ -56,7 +64,7 @@ __global__ void bin_count_kernel(const float* __restrict__ kc, int d, int64_t M,
         for (int a = 0; a < d; ++a) {
             int64_t c;
             float f;
-            point_cell(kc[j * d + a], g.n[a], c, f);
+            point_cell<float>(kc[j * d + a], g.n[a], c, f);
             b += (int)(c / g.tb[a]) * mul;
             mul *= g.nb[a];
         }
@@ -174,8 +182,11 @@ __device__ __forceinline__ float2 basis_value(const ValueSrc& s, int t, int k) {
     return reinterpret_cast<const float2*>(s.U)[t + (int64_t)k * s.nt];
 }
 
-__device__ __forceinline__ float2 channel_value(const ValueSrc& s, const float* kc, int d,
-                                                const int* frame, int64_t j, int ch) {
+// Value of channel ch at point j in the accumulation type AT (the product of
+// two ComplexF32 basis values is exact in ComplexF64).
+template <class AT>
+__device__ __forceinline__ AT channel_value(const ValueSrc& s, const float* kc, int d,
+                                            const int* frame, int64_t j, int ch) {
     const int t = frame[j];
     if (s.kind == 0) {
         const int c = ch / s.K, k = ch - c * s.K;
@@ -190,10 +201,10 @@ __device__ __forceinline__ float2 channel_value(const ValueSrc& s, const float* 
         } else if (s.dcf_mode == 3) {
             v = cscale(v, s.dcf[j]);
         }
-        return v;
+        return cto<AT>(v);
     }
-    const float2 a = basis_value(s, t, s.pair_j[ch]);
-    const float2 b = basis_value(s, t, s.pair_k[ch]);
+    const AT a = cto<AT>(basis_value(s, t, s.pair_j[ch]));
+    const AT b = cto<AT>(basis_value(s, t, s.pair_k[ch]));
     return cmulc(a, b);  // conj(U[t, j]) * U[t, k]
 }
 
@@ -206,27 +217,28 @@ __device__ __forceinline__ float2 channel_value(const ValueSrc& s, const float* 
 // the accumulation order is fixed and the result is bit-reproducible.
 constexpr int CHUNK = 64;
 
-template <int D>
+template <int D, class AT>
 __global__ void __launch_bounds__(512)
 spread_tiles_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
                     const int* __restrict__ perm, const int* __restrict__ offs, PlanGeom g,
-                    ValueSrc src, int nch, float2* __restrict__ slab, float2* __restrict__ grid) {
+                    ValueSrc src, int nch, AT* __restrict__ slab, AT* __restrict__ grid) {
     using TS = TileShape<D>;
+    using RT = real_t<AT>;
     constexpr int H = SPREAD_HALO;
     constexpr int LX = TS::X + 2 * H, LY = TS::Y + 2 * H, LZ = D == 3 ? TS::Z + 2 * H : 1;
     constexpr int CELLS = LX * LY * LZ;
-    constexpr int CHG = TS::CHG;
-    __shared__ float2 tile[CHG * CELLS];
-    __shared__ float s_w[CHUNK][D][8];
+    constexpr int CHG = chg_of<D, AT>();
+    __shared__ AT tile[CHG * CELLS];
+    __shared__ RT s_w[CHUNK][D][8];
     __shared__ int s_base[CHUNK][D];
-    __shared__ float2 s_val[CHG][CHUNK];
+    __shared__ AT s_val[CHG][CHUNK];
     const int bin = blockIdx.x;
     const int ch0 = blockIdx.y * CHG;
     const int bx = bin % g.nb[0];
     const int by = (bin / g.nb[0]) % g.nb[1];
     const int bz = D == 3 ? bin / (g.nb[0] * g.nb[1]) : 0;
     const int64_t t0[3] = {(int64_t)bx * TS::X, (int64_t)by * TS::Y, (int64_t)bz * TS::Z};
-    for (int e = threadIdx.x; e < CHG * CELLS; e += blockDim.x) tile[e] = make_float2(0.f, 0.f);
+    for (int e = threadIdx.x; e < CHG * CELLS; e += blockDim.x) tile[e] = cz<AT>();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;  // wave == local channel
     const int ox = lane & 7, oy = lane >> 3;
@@ -240,34 +252,34 @@ spread_tiles_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
             if (pl < np) {
                 const int64_t j = perm[c0 + pl];
                 int64_t cell;
-                float f;
-                point_cell(kc[j * D + a], g.n[a], cell, f);
-                s_w[pl][a][o] = es_phi((float)(o - 3) - f);
+                RT f;
+                point_cell<RT>(kc[j * D + a], g.n[a], cell, f);
+                s_w[pl][a][o] = es_phi<RT>((RT)(o - 3) - f);
                 if (o == 0) s_base[pl][a] = (int)(cell - t0[a]) + H - 3;
             }
         }
         for (int e = threadIdx.x; e < CHUNK * CHG; e += blockDim.x) {
             const int ch = e / CHUNK, pl = e - ch * CHUNK;
             if (pl < np && ch0 + ch < nch)
-                s_val[ch][pl] = channel_value(src, kc, D, frame, perm[c0 + pl], ch0 + ch);
+                s_val[ch][pl] = channel_value<AT>(src, kc, D, frame, perm[c0 + pl], ch0 + ch);
         }
         __syncthreads();
         if (ch0 + wave < nch) {
-            float2* tw = tile + wave * CELLS;
+            AT* tw = tile + wave * CELLS;
             for (int pl = 0; pl < np; ++pl) {
-                const float2 v = s_val[wave][pl];
-                const float wxy = s_w[pl][0][ox] * s_w[pl][1][oy];
+                const AT v = s_val[wave][pl];
+                const RT wxy = s_w[pl][0][ox] * s_w[pl][1][oy];
                 const int lx = s_base[pl][0] + ox, ly = s_base[pl][1] + oy;
                 if (D == 2) {
-                    float2* cell = tw + ly * LX + lx;
+                    AT* cell = tw + ly * LX + lx;
                     cell->x += wxy * v.x;
                     cell->y += wxy * v.y;
                 } else {
                     const int lz0 = s_base[pl][D - 1];
 #pragma unroll
                     for (int oz = 0; oz < 8; ++oz) {
-                        const float w = wxy * s_w[pl][D - 1][oz];
-                        float2* cell = tw + ((lz0 + oz) * LY + ly) * LX + lx;
+                        const RT w = wxy * s_w[pl][D - 1][oz];
+                        AT* cell = tw + ((lz0 + oz) * LY + ly) * LX + lx;
                         cell->x += w * v.x;
                         cell->y += w * v.y;
                     }
@@ -287,8 +299,8 @@ spread_tiles_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
     } else {
         const int64_t G = g.n[0] * g.n[1] * g.n[2];
         for (int e = threadIdx.x; e < chg * CELLS; e += blockDim.x) {
-            const float2 v = tile[e];
-            if (v.x == 0.f && v.y == 0.f) continue;
+            const AT v = tile[e];
+            if (v.x == (RT)0 && v.y == (RT)0) continue;
             const int ch = e / CELLS;
             int r = e - ch * CELLS;
             const int lx = r % LX;
@@ -301,7 +313,7 @@ spread_tiles_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
             if (gy < 0) gy += g.n[1];
             int64_t gz = (t0[2] - H + lz) % g.n[2];
             if (gz < 0) gz += g.n[2];
-            float2* dst = grid + (int64_t)(ch0 + ch) * G + (gz * g.n[1] + gy) * g.n[0] + gx;
+            AT* dst = grid + (int64_t)(ch0 + ch) * G + (gz * g.n[1] + gy) * g.n[0] + gx;
             atomicAdd(&dst->x, v.x);
             atomicAdd(&dst->y, v.y);
         }
@@ -336,8 +348,9 @@ __device__ __forceinline__ int covering(int64_t gc, int64_t n, int tb, int nb, i
     return cnt;
 }
 
-__global__ void gather_tiles_2d(const float2* __restrict__ slab, PlanGeom g, int nch,
-                                float2* __restrict__ grid) {
+template <class AT>
+__global__ void gather_tiles_2d(const AT* __restrict__ slab, PlanGeom g, int nch,
+                                AT* __restrict__ grid) {
     constexpr int H = SPREAD_HALO;
     constexpr int LX = TileShape<2>::X + 2 * H, LY = TileShape<2>::Y + 2 * H;
     constexpr int CELLS = LX * LY;
@@ -351,11 +364,11 @@ __global__ void gather_tiles_2d(const float2* __restrict__ slab, PlanGeom g, int
         int tx[6], lxs[6], ty[6], lys[6];
         const int nx = covering(gx, g.n[0], TileShape<2>::X, g.nb[0], tx, lxs);
         const int ny = covering(gy, g.n[1], TileShape<2>::Y, g.nb[1], ty, lys);
-        float2 acc = make_float2(0.f, 0.f);
-        const float2* sc = slab + (int64_t)ch * nbins * CELLS;
+        AT acc = cz<AT>();
+        const AT* sc = slab + (int64_t)ch * nbins * CELLS;
         for (int b = 0; b < ny; ++b)
             for (int a = 0; a < nx; ++a) {
-                const float2 v = sc[((int64_t)ty[b] * g.nb[0] + tx[a]) * CELLS + lys[b] * LX + lxs[a]];
+                const AT v = sc[((int64_t)ty[b] * g.nb[0] + tx[a]) * CELLS + lys[b] * LX + lxs[a]];
                 acc.x += v.x;
                 acc.y += v.y;
             }
@@ -363,16 +376,17 @@ __global__ void gather_tiles_2d(const float2* __restrict__ slab, PlanGeom g, int
     }
 }
 
-size_t spread_workspace_bytes(const SpreadPlan& plan, int nch) {
+size_t spread_workspace_bytes(const SpreadPlan& plan, int nch, bool f64) {
     if (plan.d != 2) return 0;
     constexpr int CELLS = (TileShape<2>::X + 2 * SPREAD_HALO) * (TileShape<2>::Y + 2 * SPREAD_HALO);
-    return sizeof(float2) * (size_t)nch * plan.nbins * CELLS;
+    return (f64 ? sizeof(double2) : sizeof(float2)) * (size_t)nch * plan.nbins * CELLS;
 }
 
 void fill_floats(void* p, int64_t n, float v, hipStream_t st);  // normalop.hip
 
-void spread(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src, int nch,
-            float2* grid, float2* slab, hipStream_t st) {
+template <class AT>
+static void spread_t(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src, int nch,
+                     AT* grid, AT* slab, hipStream_t st) {
     const int64_t G = plan.n[0] * plan.n[1] * plan.n[2];
     PlanGeom g;
     for (int a = 0; a < 3; ++a) {
@@ -382,26 +396,35 @@ void spread(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src, i
     }
     if (traj->d == 2) {
         MSR_REQUIRE(slab != nullptr, MSR_ERR_ERROR, "2D spreading needs a workspace");
-        dim3 grd((unsigned)plan.nbins, (unsigned)ceil_div(nch, TileShape<2>::CHG));
-        hipLaunchKernelGGL(spread_tiles_kernel<2>, grd, dim3(64 * TileShape<2>::CHG), 0, st,
+        constexpr int CHG = chg_of<2, AT>();
+        dim3 grd((unsigned)plan.nbins, (unsigned)ceil_div(nch, CHG));
+        hipLaunchKernelGGL((spread_tiles_kernel<2, AT>), grd, dim3(64 * CHG), 0, st,
                            traj->kc.as<float>(), traj->frame.as<int>(), plan.perm.as<int>(),
                            plan.offs.as<int>(), g, src, nch, slab, grid);
         MSR_CHECK_LAUNCH();
         int gr = (int)std::min<int64_t>((G * nch + 255) / 256, 65536);
-        hipLaunchKernelGGL(gather_tiles_2d, dim3(gr), dim3(256), 0, st, slab, g, nch, grid);
+        hipLaunchKernelGGL(gather_tiles_2d<AT>, dim3(gr), dim3(256), 0, st, slab, g, nch, grid);
     } else {
-        fill_floats(grid, 2 * G * nch, 0.f, st);
-        dim3 grd((unsigned)plan.nbins, (unsigned)ceil_div(nch, TileShape<3>::CHG));
-        hipLaunchKernelGGL(spread_tiles_kernel<3>, grd, dim3(64 * TileShape<3>::CHG), 0, st,
+        fill_floats(grid, (int64_t)(sizeof(AT) / sizeof(float)) * G * nch, 0.f, st);
+        constexpr int CHG = chg_of<3, AT>();
+        dim3 grd((unsigned)plan.nbins, (unsigned)ceil_div(nch, CHG));
+        hipLaunchKernelGGL((spread_tiles_kernel<3, AT>), grd, dim3(64 * CHG), 0, st,
                            traj->kc.as<float>(), traj->frame.as<int>(), plan.perm.as<int>(),
                            plan.offs.as<int>(), g, src, nch, slab, grid);
     }
     MSR_CHECK_LAUNCH();
 }
 
+void spread(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src, int nch,
+            void* grid, void* slab, bool f64, hipStream_t st) {
+    if (f64) spread_t<double2>(traj, plan, src, nch, (double2*)grid, (double2*)slab, st);
+    else spread_t<float2>(traj, plan, src, nch, (float2*)grid, (float2*)slab, st);
+}
+
 // ------------------------------------------------------------------ deapodisation
 // 1/phihat(r), phihat(r) = int phi(t) cos(2 pi r t / n_os) dt (midpoint rule, FP64)
-__global__ void deapod_kernel(int n_os, int n_out, int sel, float* __restrict__ out) {
+template <class OT>
+__global__ void deapod_kernel(int n_os, int n_out, int sel, OT* __restrict__ out) {
     const int j = blockIdx.x;
     const int r = sel == 1 ? j - n_out / 2 : (j < (n_out + 1) / 2 ? j : j - n_out);
     constexpr int NQ = 8192;
@@ -421,21 +444,31 @@ __global__ void deapod_kernel(int n_os, int n_out, int sel, float* __restrict__ 
         if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[j] = (float)(1.0 / (s[0] * h));
+    if (threadIdx.x == 0) out[j] = (OT)(1.0 / (s[0] * h));
+}
+
+template <class OT>
+static const OT* deapod_table_t(msr_ctx* ctx, std::map<std::tuple<int, int, int>, OT*>& cache,
+                                int n_os, int n_out, int sel) {
+    std::lock_guard<std::mutex> guard(ctx->lock);
+    auto key = std::make_tuple(n_os, n_out, sel);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    OT* d = nullptr;
+    MSR_HIP(hipMalloc(&d, sizeof(OT) * n_out));
+    hipLaunchKernelGGL(deapod_kernel<OT>, dim3(n_out), dim3(256), 0, nullptr, n_os, n_out, sel, d);
+    MSR_CHECK_LAUNCH();
+    MSR_HIP(hipDeviceSynchronize());
+    cache[key] = d;
+    return d;
 }
 
 const float* deapod_table(msr_ctx* ctx, int n_os, int n_out, int sel) {
-    std::lock_guard<std::mutex> guard(ctx->lock);
-    auto key = std::make_tuple(n_os, n_out, sel);
-    auto it = ctx->deapod.find(key);
-    if (it != ctx->deapod.end()) return it->second;
-    float* d = nullptr;
-    MSR_HIP(hipMalloc(&d, sizeof(float) * n_out));
-    hipLaunchKernelGGL(deapod_kernel, dim3(n_out), dim3(256), 0, nullptr, n_os, n_out, sel, d);
-    MSR_CHECK_LAUNCH();
-    MSR_HIP(hipDeviceSynchronize());
-    ctx->deapod[key] = d;
-    return d;
+    return deapod_table_t<float>(ctx, ctx->deapod, n_os, n_out, sel);
+}
+
+const double* deapod_table64(msr_ctx* ctx, int n_os, int n_out, int sel) {
+    return deapod_table_t<double>(ctx, ctx->deapod64, n_os, n_out, sel);
 }
 
 }  // namespace msr

@@ -44,12 +44,15 @@ struct ValueSrc {
 // channels.  2D needs a workspace of spread_workspace_bytes(plan, nch) (the
 // padded tiles, summed into the grid in a fixed order); 3D accumulates with
 // global atomics and takes none.
-size_t spread_workspace_bytes(const SpreadPlan& plan, int nch);
+// f64: accumulate, and store the grid and workspace, in ComplexF64 (the
+// values and kernel weights are then evaluated in FP64 too).
+size_t spread_workspace_bytes(const SpreadPlan& plan, int nch, bool f64 = false);
 void spread(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src, int nch,
-            float2* grid, float2* slab, hipStream_t st);
+            void* grid, void* slab, bool f64, hipStream_t st);
 
 // Device table 1/phihat(r) for the output frequencies of a type-1 NUFFT of
 // size n_out (sel: 1 centred r = j - n_out/2, 2 FFT order) on an n_os grid.
 const float* deapod_table(msr_ctx* ctx, int n_os, int n_out, int sel);
+const double* deapod_table64(msr_ctx* ctx, int n_os, int n_out, int sel);
 
 }  // namespace msr

@@ -13,6 +13,7 @@ msr_ctx::~msr_ctx() {
     for (auto& kv : twiddles) (void)hipFree(kv.second);
     for (auto& kv : twiddles64) (void)hipFree(kv.second);
     for (auto& kv : deapod) (void)hipFree(kv.second);
+    for (auto& kv : deapod64) (void)hipFree(kv.second);
 }
 
 namespace msr {

@@ -76,7 +76,7 @@ def test_kernel3d_and_backprojection3d(torch, msr):
     assert np.array_equal(A.prod.A.kmask_indcs.cpu().numpy(), km_ref + 1)
     lam = A.prod.A.Λ.cpu().numpy()
     print(f"3d kernel rel err {rel_err(lam, lam_ref):.3e}", flush=True)
-    assert rel_err(lam, lam_ref) < 1e-5
+    assert rel_err(lam, lam_ref) < 3e-7  # FP64 build, W = 8 truncation
     rng = np.random.default_rng(4)
     data = (rng.normal(size=(3,) + trj.shape[:2]) + 1j * rng.normal(size=(3,) + trj.shape[:2])).astype(np.complex64)
     for dcf in ("none", "radial_3D"):

@@ -15,6 +15,9 @@ from oracle import toeplitz
 pytestmark = pytest.mark.gpu
 
 NUFFT_TOL = 1e-5
+# the kernel is built in FP64 and rounded once to ComplexF32: what remains is
+# the W = 8 spreading kernel's truncation error (~1e-7) and the FP32 rounding
+KERNEL_TOL = 3e-7
 RECON_TOL = 1e-4
 
 
@@ -71,7 +74,7 @@ def test_kernel_and_kmask_match_oracle(torch, msr, N, K, nt, nspk, real, half, m
     lam = A.prod.A.Λ.cpu().numpy()
     assert lam.dtype == (np.float32 if real else np.complex64)
     print(f"kernel rel err {rel_err(lam, lam_ref):.3e}", flush=True)
-    assert rel_err(lam, lam_ref) < NUFFT_TOL
+    assert rel_err(lam, lam_ref) < KERNEL_TOL
 
 
 BP_CASES = [
