@@ -98,14 +98,29 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
 #pragma unroll
     for (int i = 0; i < EA; ++i) acc[i] = cz<CT>();
     const int g = tid / TP, t = tid - g * TP;
-    for (int c = c0; c < c1; ++c) {
+    // the tile's rows of one coil, NLE elements per thread, prefetched into
+    // registers one coil ahead
+    constexpr int NLE = (RT * NX2 + BLK - 1) / BLK;
+    CT pre[NLE];
+    auto prefetch = [&](int c) {
         const CT* Tk = static_cast<const CT*>(a.T) + ((size_t)c * K + k) * NX2 * ny;
-        for (int e = tid; e < RT * NX2; e += BLK) {
+#pragma unroll
+        for (int r = 0; r < NLE; ++r) {
+            const int e = tid + r * BLK;
             const int yl = e % RT, qx = e / RT;
             const int y = y0 + yl;
-            s_rows[yl * LD + lds_pad(qx)] = (y < ny) ? Tk[(size_t)qx * ny + y] : cz<CT>();
+            pre[r] = (e < RT * NX2 && y < ny) ? Tk[(size_t)qx * ny + y] : cz<CT>();
+        }
+    };
+    if (c0 < c1) prefetch(c0);
+    for (int c = c0; c < c1; ++c) {
+#pragma unroll
+        for (int r = 0; r < NLE; ++r) {
+            const int e = tid + r * BLK;
+            if (e < RT * NX2) s_rows[(e % RT) * LD + lds_pad(e / RT)] = pre[r];
         }
         __syncthreads();
+        if (c + 1 < c1) prefetch(c + 1);  // in flight during this coil's transforms
 #pragma unroll
         for (int r0 = 0; r0 < RT; r0 += F) fft_lds<NX2, +1>(s_rows + (r0 + g) * LD, s_tw, t);
         const float2* cm = a.cmaps ? a.cmaps + (size_t)c * nx * ny : nullptr;

@@ -51,12 +51,13 @@ struct Apply2DArgs {
     const int* skip;      // when non-null and *skip != 0 the passes do nothing
 };
 
-// rows per tile in passes A and C: 128-byte segments of the transposed
-// intermediate (16 ComplexF32 or 8 ComplexF64 rows)
+// rows per tile in passes A and C: 64-byte segments of the transposed
+// intermediate (8 ComplexF32 or 4 ComplexF64 rows), which keeps a 512-point
+// tile near 40 KB of LDS (three workgroups per CU)
 template <class CT>
-constexpr int rows_per_tile() { return 128 / (int)sizeof(CT); }
+constexpr int rows_per_tile() { return 64 / (int)sizeof(CT); }
 __host__ __device__ inline int tiles2d(int ny, int f64) {
-    const int rt = f64 ? 8 : 16;
+    const int rt = f64 ? 4 : 8;
     return (ny + rt - 1) / rt;
 }
 int pass_b_colblocks(int nx2, int ny2, int K, int f64);  // column blocks of pass B per coil group

@@ -71,17 +71,34 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
 
     const int g = tid / TP, t = tid - g * TP;
     const bool fft_thread = tid < FT;
-    for (int c = c0; c < c1; ++c) {
-        // load K columns per owned qx, zero-padded from ny to NY2
-        for (int el = tid; el < WC * K * NY2; el += BLK) {
+    // the K columns of one coil (zero-padded from ny to NY2), NLE elements per
+    // thread, prefetched into registers one coil ahead
+    constexpr int NLE = (WC * K * NY2 + BLK - 1) / BLK;
+    CT pre[NLE];
+    auto prefetch = [&](int c) {
+#pragma unroll
+        for (int r = 0; r < NLE; ++r) {
+            const int el = tid + r * BLK;
             const int line = el / NY2, i = el - line * NY2;
             const int wc = line / K, k = line - wc * K;
             const int qx = qx0 + wc;
-            CT v = cz<CT>();
-            if (i < ny && qx < nx2) v = T[(((size_t)c * K + k) * nx2 + qx) * ny + i];
-            s_buf[line * LL + lds_pad(i)] = v;
+            pre[r] = cz<CT>();
+            if (el < WC * K * NY2 && i < ny && qx < nx2)
+                pre[r] = T[(((size_t)c * K + k) * nx2 + qx) * ny + i];
+        }
+    };
+    if (c0 < c1) prefetch(c0);
+    for (int c = c0; c < c1; ++c) {
+#pragma unroll
+        for (int r = 0; r < NLE; ++r) {
+            const int el = tid + r * BLK;
+            if (el < WC * K * NY2) {
+                const int line = el / NY2, i = el - line * NY2;
+                s_buf[line * LL + lds_pad(i)] = pre[r];
+            }
         }
         __syncthreads();
+        if (c + 1 < c1) prefetch(c + 1);  // in flight during this coil's transforms
         fft_lds<NY2, -1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
         // per-voxel Hermitian (or real symmetric) K x K multiply
 #pragma unroll

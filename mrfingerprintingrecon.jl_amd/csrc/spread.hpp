@@ -50,6 +50,15 @@ size_t spread_workspace_bytes(const SpreadPlan& plan, int nch, bool f64 = false)
 void spread(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src, int nch,
             void* grid, void* slab, bool f64, hipStream_t st);
 
+// 2D ComplexF32 backprojection spreading onto output-owned tiles
+// (spread_bp.hip): builds the plan's point streams and pair lists on first
+// use; the workspace holds the partial tiles of split tiles and the coil data
+// in sorted point order.  grid (nch, n1, n0) is overwritten.
+size_t spread_bp_workspace_bytes(const msr_traj* traj, const SpreadPlan& plan, int nch, int ncoils,
+                                 hipStream_t st);
+void spread_bp(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src, int nch,
+               float2* grid, float2* work, hipStream_t st);
+
 // Device table 1/phihat(r) for the output frequencies of a type-1 NUFFT of
 // size n_out (sel: 1 centred r = j - n_out/2, 2 FFT order) on an n_os grid.
 const float* deapod_table(msr_ctx* ctx, int n_os, int n_out, int sel);
