@@ -13,6 +13,16 @@ constexpr int rowpass_block() {
                                                          : 256;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
+// each XCD with its own L2.  Row tiles of 64 bytes per line half-fill a
+// 128-byte line of the transposed intermediate, so neighbouring tiles should
+// run on one XCD: blocks b and b + 8 get consecutive logical indices (tile
+// fastest), when the block count divides by 8.
+__device__ __forceinline__ int64_t xcd_order(int64_t lb, int64_t nblk) {
+    if (nblk % 8 != 0) return lb;
+    return (lb % 8) * (nblk / 8) + lb / 8;
+}
+
 // ----------------------------------------------------------------- pass A
 // block lb = (tile, k, c), tile fastest
 template <int NX2, class CT>
@@ -27,6 +37,7 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     __shared__ CT s_rows[RT * LD];
     const int nx = a.nx, ny = a.ny, K = a.K;
     const int tiles = tiles2d(ny, sizeof(CT) == 16);
+    lb = xcd_order(lb, (int64_t)tiles * K * a.C);
     const int tile = (int)(lb % tiles);
     const int k = (int)((lb / tiles) % K);
     const int c = (int)(lb / ((int64_t)tiles * K));
@@ -87,6 +98,7 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     __shared__ CT s_rows[RT * LD];
     const int nx = a.nx, ny = a.ny, K = a.K, C = a.C, G = a.G;
     const int tiles = tiles2d(ny, sizeof(CT) == 16);
+    lb = xcd_order(lb, (int64_t)tiles * K * G);
     const int tile = (int)(lb % tiles);
     const int k = (int)((lb / tiles) % K);
     const int grp = (int)(lb / ((int64_t)tiles * K));
