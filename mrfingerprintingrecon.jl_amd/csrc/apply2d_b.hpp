@@ -7,10 +7,47 @@
 #include "fft.hpp"
 
 namespace msr {
+#ifndef PB_WAVES
+#define PB_WAVES 4
+#endif
+#ifndef PB_LAMG
+#define PB_LAMG false
+#endif
+#ifndef PB_PREFETCH
+#define PB_PREFETCH false
+#endif
 
 // 0-based LAPACK 'U' packed index of (j, k), j <= k
 // (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:239, 1-based j + k(k-1)/2).
 __host__ __device__ constexpr int packed_index(int j, int k) { return j + k * (k + 1) / 2; }
+
+// acc[j] += sum_k Lambda_jk in[k] for the packed Hermitian (or real
+// symmetric) K x K kernel of one voxel, lr[packed_index(j, k)] for j <= k and
+// Lambda_kj = conj(Lambda_jk).  Each packed entry is promoted once, on use, to
+// the arithmetic type, so the registers hold the Float32 table only.
+template <int K, bool LREAL, class CT, class LT>
+__device__ __forceinline__ void kernel_multiply(const LT* lr, const CT* in, CT* acc) {
+    using RT = real_t<CT>;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int j = 0; j <= k; ++j) {
+            const LT lf = sizeof(CT) == 16 ? opaque(lr[packed_index(j, k)]) : lr[packed_index(j, k)];
+            if constexpr (LREAL) {
+                const RT l = (RT)lf;
+                acc[j].x = fma(l, in[k].x, acc[j].x);
+                acc[j].y = fma(l, in[k].y, acc[j].y);
+                if (j != k) {
+                    acc[k].x = fma(l, in[j].x, acc[k].x);
+                    acc[k].y = fma(l, in[j].y, acc[k].y);
+                }
+            } else {
+                const CT l = cto<CT>(lf);
+                cfma(acc[j], l, in[k]);
+                if (j != k) cfmac(acc[k], l, in[j]);
+            }
+        }
+}
 
 // Workgroup shape: WC grid columns, each with K lines of NY2 (FFT by
 // TP = FFTPlan<NY2>::T threads per line), and at least one thread per voxel of
@@ -53,53 +90,69 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     const int c0 = (int)((int64_t)C * grp / Gb), c1 = (int)((int64_t)C * (grp + 1) / Gb);
     load_twiddles<NY2>(s_tw, static_cast<const CT*>(a.tw_y));
 
-    // kernel values of the voxels this thread owns, kept for all coils
-    LT lr[EV][P];
-#pragma unroll
-    for (int e = 0; e < EV; ++e) {
+    // kernel values of the voxels this thread owns: kept in registers for all
+    // coils, or (LAMG) read again from global memory for every coil
+    constexpr bool LAMG = PB_LAMG && sizeof(CT) == 16;
+    auto load_lam = [&](LT* dst, int e) {
         const int v = tid + e * BLK;
         const int wc = v / NY2, qy = v - wc * NY2;
         const int qx = qx0 + wc;
         const bool ok = (v < NV) && (qx < nx2);
+        // branch-free: an out-of-range voxel reads column 0 and keeps zero
+        const LT* src = lam + (ok ? (size_t)qx * P * NY2 + qy : 0);
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            if (ok) lr[e][p] = lam[((size_t)qx * P + p) * NY2 + qy];
-            else if constexpr (LREAL) lr[e][p] = 0.f;
-            else lr[e][p] = make_float2(0.f, 0.f);
-        }
-    }
-
-    const int g = tid / TP, t = tid - g * TP;
-    const bool fft_thread = tid < FT;
-    // the K columns of one coil (zero-padded from ny to NY2), NLE elements per
-    // thread, prefetched into registers one coil ahead
-    constexpr int NLE = (WC * K * NY2 + BLK - 1) / BLK;
-    CT pre[NLE];
-    auto prefetch = [&](int c) {
-#pragma unroll
-        for (int r = 0; r < NLE; ++r) {
-            const int el = tid + r * BLK;
-            const int line = el / NY2, i = el - line * NY2;
-            const int wc = line / K, k = line - wc * K;
-            const int qx = qx0 + wc;
-            pre[r] = cz<CT>();
-            if (el < WC * K * NY2 && i < ny && qx < nx2)
-                pre[r] = T[(((size_t)c * K + k) * nx2 + qx) * ny + i];
+            const LT x = src[(size_t)p * NY2];
+            if constexpr (LREAL) dst[p] = ok ? x : 0.f;
+            else dst[p] = ok ? x : make_float2(0.f, 0.f);
         }
     };
-    if (c0 < c1) prefetch(c0);
-    for (int c = c0; c < c1; ++c) {
+    LT lr[LAMG ? 1 : EV][P];
+    if constexpr (!LAMG) {
 #pragma unroll
-        for (int r = 0; r < NLE; ++r) {
-            const int el = tid + r * BLK;
-            if (el < WC * K * NY2) {
-                const int line = el / NY2, i = el - line * NY2;
-                s_buf[line * LL + lds_pad(i)] = pre[r];
+        for (int e = 0; e < EV; ++e) load_lam(lr[e], e);
+    }
+
+    // FFT thread (g, t): line g = wc * K + k of the WG, t < TP.  Its line is
+    // held in registers in the transform's first-stage layout, element
+    // i = t + p TP + r NY2/R0 at reg[p R0 + r]; the elements of the upper half
+    // (i >= ny, the zero padding) are the constant zero.
+    constexpr int NR = NY2 / TP;
+    constexpr bool PREFETCH = PB_PREFETCH;
+    constexpr int R0 = FFTPlan<NY2>::R[0], NB0 = NY2 / R0;
+    const int g = tid / TP, t = tid - g * TP;
+    const bool fft_thread = tid < FT;
+    const int gl = fft_thread ? g : 0;
+    const int wcl = gl / K, kl = gl - wcl * K;
+    const bool col_ok = fft_thread && (qx0 + wcl) < nx2;
+    CT* line = s_buf + gl * LL;
+    CT pre[NR];  // the next coil's line, loaded one coil ahead
+    auto prefetch = [&](int c) {
+        // branch-free: a thread without a line reads line 0 of the WG and keeps zero
+        const CT* src = T + (((size_t)c * K + kl) * nx2 + (col_ok ? qx0 + wcl : qx0)) * ny;
+#pragma unroll
+        for (int p = 0; p < NR / R0; ++p)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                const int i = t + p * TP + r * NB0;
+                if (r * NB0 < NY2 / 2) {
+                    const CT x = src[i < NY2 / 2 ? i : 0];
+                    pre[p * R0 + r] = (i < NY2 / 2 && col_ok) ? x : cz<CT>();
+                } else {
+                    pre[p * R0 + r] = cz<CT>();
+                }
             }
-        }
+    };
+    if (PREFETCH && c0 < c1) prefetch(c0);
+    __syncthreads();  // twiddles
+    for (int c = c0; c < c1; ++c) {
+        CT xr[NR];
+        if constexpr (!PREFETCH) prefetch(c);
+#pragma unroll
+        for (int e = 0; e < NR; ++e) xr[e] = pre[e];
+        if (PREFETCH && c + 1 < c1) prefetch(c + 1);  // in flight during this coil's transforms
+        if (fft_thread) fft_wave<NY2, -1, true, false>(line, s_tw, t, true, xr);
         __syncthreads();
-        if (c + 1 < c1) prefetch(c + 1);  // in flight during this coil's transforms
-        fft_lds<NY2, -1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
         // per-voxel Hermitian (or real symmetric) K x K multiply
 #pragma unroll
         for (int e = 0; e < EV; ++e) {
@@ -107,47 +160,46 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
             if (v < NV) {
                 const int wc = v / NY2, qy = v - wc * NY2;
                 CT* col = s_buf + (size_t)wc * K * LL + lds_pad(qy);
-                CT in[K];
+                CT in[K], acc[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) in[k] = col[k * LL];
-#pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    CT acc = cz<CT>();
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        if constexpr (LREAL) {
-                            const RT l = (RT)lr[e][j <= k ? packed_index(j, k) : packed_index(k, j)];
-                            acc.x = fma(l, in[k].x, acc.x);
-                            acc.y = fma(l, in[k].y, acc.y);
-                        } else {
-                            if (j <= k) cfma(acc, cto<CT>(lr[e][packed_index(j, k)]), in[k]);
-                            else cfmac(acc, cto<CT>(lr[e][packed_index(k, j)]), in[k]);
-                        }
-                    }
-                    col[j * LL] = acc;
+                for (int k = 0; k < K; ++k) {
+                    in[k] = col[k * LL];
+                    acc[k] = cz<CT>();
                 }
+                if constexpr (LAMG) {
+                    LT lg[P];
+                    load_lam(lg, e);
+                    kernel_multiply<K, LREAL, CT>(lg, in, acc);
+                } else {
+                    kernel_multiply<K, LREAL, CT>(lr[e], in, acc);
+                }
+#pragma unroll
+                for (int j = 0; j < K; ++j) col[j * LL] = acc[j];
             }
         }
         __syncthreads();
-        fft_lds<NY2, +1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
-        for (int el = tid; el < WC * K * NY2; el += BLK) {
-            const int line = el / NY2, i = el - line * NY2;
-            const int wc = line / K, k = line - wc * K;
-            const int qx = qx0 + wc;
-            if (i < ny && qx < nx2)
-                T[(((size_t)c * K + k) * nx2 + qx) * ny + i] = s_buf[line * LL + lds_pad(i)];
+        if (fft_thread) {
+            // the last stage writes the line to LDS (a register-resident last
+            // stage costs more registers than the LDS round trip); the crop
+            // stores elements i < ny, lanes on consecutive i
+            fft_wave<NY2, +1, false, false>(line, s_tw, t, true, xr);
+            CT* dst = T + (((size_t)c * K + kl) * nx2 + qx0 + wcl) * ny;
+#pragma unroll
+            for (int m = 0; m < NR / 2; ++m) {
+                const int i = t + m * TP;
+                if (col_ok) dst[i] = line[lds_pad(i)];
+            }
         }
-        __syncthreads();
     }
 }
 
 template <int NY2, int K, bool LREAL, class CT>
-__global__ void __launch_bounds__(512) pass_b_2d(Apply2DArgs a) {
+__global__ void __launch_bounds__(512, PB_WAVES) pass_b_2d(Apply2DArgs a) {
     pass_b_body<NY2, K, LREAL, CT>(a, blockIdx.x);
 }
 
 template <int NY2, int K, bool LREAL, class CT>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512, PB_WAVES)
 pass_b_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pre, int nj) {
     const int j = batch_job(pre, nj, blockIdx.x);
     pass_b_body<NY2, K, LREAL, CT>(jobs[j], (int64_t)blockIdx.x - pre[j]);

@@ -116,6 +116,15 @@ __host__ __device__ __forceinline__ CT cto(float2 a) { return CTraits<CT>::make(
 template <class CT>
 __host__ __device__ __forceinline__ CT cto(double2 a) { return CTraits<CT>::make(a.x, a.y); }
 
+// Opaque copy of a register value: work on the copy cannot be hoisted out of
+// the enclosing loop (keeps a register-resident Float32 table from being
+// widened to Float64 once for all iterations, which doubles its registers).
+__device__ __forceinline__ float opaque(float v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+__device__ __forceinline__ float2 opaque(float2 v) { return make_float2(opaque(v.x), opaque(v.y)); }
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace msr

@@ -5,13 +5,16 @@
 // the twiddles W_{NS*R}^{k r} (k = j mod NS, NS = product of earlier radices),
 // runs an R-point DFT in registers and writes the R results to positions
 // (j - k) R + k + r NS, so the output is in natural order without a
-// bit-reversal pass (Stockham autosort).  Stages are separated by workgroup
-// barriers; every thread of the workgroup must call the transform.
+// bit-reversal pass (Stockham autosort).  A line's threads lie in one
+// wavefront, so stages are ordered by wavefront-level fences; the first
+// stage may take its inputs from registers and the last leave its outputs
+// there (fft_wave), saving the LDS round trips next to global memory.
 //
 // Sign: DIR = -1 forward (exp(-2 pi i ...)), DIR = +1 inverse, unnormalised.
-// Twiddles come from a table tw[m] = exp(-2 pi i m / N) computed on the host in
-// double precision (stored as ComplexF32 or ComplexF64 for the arithmetic type)
-// and staged into LDS by the caller.  The arithmetic type is the element type
+// Twiddles come from a stage-ordered table (entry NS - R0 + (r - 1) NS + k =
+// exp(-2 pi i k r / (NS R)) for the stage with radix R after NS) computed on
+// the host in double precision (stored as ComplexF32 or ComplexF64 for the
+// arithmetic type) and staged into LDS by the caller.  The arithmetic type is the element type
 // of the line: float2 or double2.
 #pragma once
 
@@ -20,8 +23,9 @@
 namespace msr {
 
 // ----------------------------------------------------------- radix plans
-// R[]: radices in stage order; T: threads per line (divides N/R for the
-// stages where that matters; other stages use a guarded loop).
+// R[]: radices in stage order; T: threads per line.  T divides 64 (a line
+// lies in one wavefront) and N/R of every stage (every thread does the same
+// number of butterflies, so a thread holds N/T values of its line).
 template <int N> struct FFTPlan;
 #define MSR_PLAN(NN, TT, ...)                                   \
     template <> struct FFTPlan<NN> {                            \
@@ -38,10 +42,10 @@ MSR_PLAN(64, 8, 8, 8)
 MSR_PLAN(80, 2, 8, 5, 2)
 MSR_PLAN(96, 4, 8, 4, 3)
 MSR_PLAN(128, 16, 8, 4, 4)
-MSR_PLAN(160, 8, 8, 4, 5)
+MSR_PLAN(160, 8, 4, 4, 5, 2)
 MSR_PLAN(192, 8, 8, 8, 3)
 MSR_PLAN(256, 32, 8, 8, 4)
-MSR_PLAN(320, 16, 8, 8, 5)
+MSR_PLAN(320, 16, 4, 4, 4, 5)
 MSR_PLAN(384, 16, 8, 8, 3, 2)
 MSR_PLAN(512, 64, 8, 8, 8)
 MSR_PLAN(640, 16, 8, 8, 5, 2)
@@ -197,60 +201,129 @@ template <int N>
 constexpr int lds_line() { return N + N / 8; }
 
 // ----------------------------------------------------------- stages
-template <int N, int R, int NS, int T, int DIR, class CT>
-__device__ __forceinline__ void fft_stage(CT* buf, const CT* tw, int t, bool active) {
+// The T threads of a line always lie in one wavefront (every plan's T divides
+// 64), so the stages of a line are ordered by wavefront-level fences instead
+// of workgroup barriers: the LDS operations of a wavefront complete in issue
+// order, and the fences keep the compiler from moving LDS accesses across.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// True when every stage of the plan gives each thread the same number of
+// butterflies (N / R divisible by T): then a thread holds N / T values and the
+// first stage's inputs / last stage's outputs can stay in registers.
+template <int N>
+constexpr bool fft_plan_exact() {
+    for (int s = 0; s < FFTPlan<N>::NST; ++s)
+        if ((N / FFTPlan<N>::R[s]) % FFTPlan<N>::T != 0) return false;
+    return true;
+}
+
+// One stage.  MODE bit 0: the inputs come from reg (reg[p R + r] = x[j + r N/R]
+// for j = t + p T) instead of LDS; bit 1: the outputs go to reg (for the last
+// stage reg[p R + r] = X[j + r N/R]) instead of LDS.
+template <int N, int R, int NS, int T, int DIR, int MODE, class CT>
+__device__ __forceinline__ void fft_stage(CT* buf, const CT* tw, int t, bool active, CT* reg) {
     constexpr int NB = N / R;
     constexpr int PER = (NB + T - 1) / T;
     constexpr bool EXACT = (NB % T) == 0;
+    constexpr bool IN_REG = (MODE & 1) != 0, OUT_REG = (MODE & 2) != 0;
+    static_assert(EXACT || MODE == 0, "register stages need an exact plan");
     CT v[PER][R];
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         const int j = t + p * T;
         if (active && (EXACT || j < NB)) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[p][r] = buf[lds_pad(j + r * NB)];
+            for (int r = 0; r < R; ++r) {
+                if constexpr (IN_REG) v[p][r] = reg[p * R + r];
+                else v[p][r] = buf[lds_pad(j + r * NB)];
+            }
         }
     }
-    __syncthreads();
+    // every lane's reads of buf (this stage's, or the caller's before a
+    // register-input stage) come before any lane's writes
+    wave_lds_sync();
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         const int j = t + p * T;
         if (active && (EXACT || j < NB)) {
             const int k = j % NS;
             if constexpr (NS > 1) {
+                // stage-ordered table: W_{NS R}^{k r} at OFF + (r - 1) NS + k
+                constexpr int OFF = NS - FFTPlan<N>::R[0];
+                if constexpr (sizeof(CT) == 16 && R == 8) {
+                    // ComplexF64: two table twiddles, the others by products
+                    // (rounding <= 3 ulp), fewer LDS reads and registers
+                    CT w1 = tw[OFF + k], w4 = tw[OFF + 3 * NS + k];
+                    if constexpr (DIR > 0) {
+                        w1.y = -w1.y;
+                        w4.y = -w4.y;
+                    }
+                    const CT w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+                    v[p][1] = cmul(v[p][1], w1);
+                    v[p][2] = cmul(v[p][2], w2);
+                    v[p][3] = cmul(v[p][3], w3);
+                    v[p][4] = cmul(v[p][4], w4);
+                    v[p][5] = cmul(v[p][5], cmul(w4, w1));
+                    v[p][6] = cmul(v[p][6], cmul(w4, w2));
+                    v[p][7] = cmul(v[p][7], cmul(w4, w3));
+                } else {
 #pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    CT w = tw[(k * r) * (N / (NS * R))];
-                    if constexpr (DIR > 0) w.y = -w.y;
-                    v[p][r] = cmul(v[p][r], w);
+                    for (int r = 1; r < R; ++r) {
+                        CT w = tw[OFF + (r - 1) * NS + k];
+                        if constexpr (DIR > 0) w.y = -w.y;
+                        v[p][r] = cmul(v[p][r], w);
+                    }
                 }
             }
             Butterfly<R, DIR>::run(v[p]);
             const int base = (j - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[lds_pad(base + r * NS)] = v[p][r];
+            for (int r = 0; r < R; ++r) {
+                if constexpr (OUT_REG) reg[p * R + r] = v[p][r];
+                else buf[lds_pad(base + r * NS)] = v[p][r];
+            }
         }
     }
-    __syncthreads();
+    if constexpr (!OUT_REG) wave_lds_sync();  // visible to the next stage's reads
 }
 
-template <int N, int T, int DIR, int S, int NS, class CT>
-__device__ __forceinline__ void fft_stages(CT* buf, const CT* tw, int t, bool active) {
-    if constexpr (S < FFTPlan<N>::NST) {
+template <int N, int T, int DIR, int S, int NS, int MODE, class CT>
+__device__ __forceinline__ void fft_stages(CT* buf, const CT* tw, int t, bool active, CT* reg) {
+    constexpr int NST = FFTPlan<N>::NST;
+    if constexpr (S < NST) {
         constexpr int R = FFTPlan<N>::R[S];
-        fft_stage<N, R, NS, T, DIR, CT>(buf, tw, t, active);
-        fft_stages<N, T, DIR, S + 1, NS * R, CT>(buf, tw, t, active);
+        constexpr int M = (S == 0 ? (MODE & 1) : 0) | (S == NST - 1 ? (MODE & 2) : 0);
+        fft_stage<N, R, NS, T, DIR, M, CT>(buf, tw, t, active, reg);
+        fft_stages<N, T, DIR, S + 1, NS * R, MODE, CT>(buf, tw, t, active, reg);
     }
 }
 
 // In-place N-point FFT of a line (element i at buf[lds_pad(i)]) by the
 // T = FFTPlan<N>::T threads with local index t.  Requires a barrier before the
 // call (data in LDS); ends with a barrier.  All threads of the workgroup must
-// call it; threads with active = false only take part in the barriers (buf is
-// not touched), for workgroups with more threads than line work.
+// call it; threads with active = false only take part in the final barrier
+// (buf is not touched), for workgroups with more threads than line work.
 template <int N, int DIR, class CT>
 __device__ __forceinline__ void fft_lds(CT* buf, const CT* tw, int t, bool active = true) {
-    fft_stages<N, FFTPlan<N>::T, DIR, 0, 1, CT>(buf, tw, t, active);
+    fft_stages<N, FFTPlan<N>::T, DIR, 0, 1, 0, CT>(buf, tw, t, active, nullptr);
+    __syncthreads();
+}
+
+// The same transform with no workgroup barrier, for an exact plan, the line's
+// buf used only by its own wavefront.  IN_REG: the input is in reg in the
+// first stage's layout, reg[p R0 + r] = x[t + p T + r N/R0] (otherwise in buf,
+// ordered by the caller); OUT_REG: the output is left in reg in the last
+// stage's layout, reg[p RL + r] = X[t + p T + r N/RL] (otherwise in buf in
+// natural order, visible to this wavefront; other wavefronts need a barrier).
+// reg holds N / T values.
+template <int N, int DIR, bool IN_REG, bool OUT_REG, class CT>
+__device__ __forceinline__ void fft_wave(CT* buf, const CT* tw, int t, bool active, CT* reg) {
+    static_assert(fft_plan_exact<N>(), "fft_wave needs an exact plan");
+    fft_stages<N, FFTPlan<N>::T, DIR, 0, 1, (IN_REG ? 1 : 0) | (OUT_REG ? 2 : 0), CT>(buf, tw, t, active, reg);
 }
 
 // Copy an N-entry twiddle table from global memory into LDS (no barrier).

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <vector>
 
+#include "fft.hpp"
 #include "internal.hpp"
 
 msr_ctx::~msr_ctx() {
@@ -41,11 +42,34 @@ static std::vector<double2> twiddles_f64(int n) {
     return h;
 }
 
+// Stage-ordered twiddles of the length-n plan (the layout fft_stage reads):
+// for stage s >= 1, with NS the product of the earlier radices and R its
+// radix, entry (NS - R0) + (r - 1) NS + k = exp(-2 pi i k r / (NS R)) for
+// k < NS, 1 <= r < R.  Consecutive k are adjacent, so the threads of a stage
+// read it without LDS bank conflicts.  n - R0 entries, padded to n.
+static std::vector<double2> stage_twiddles_f64(int n) {
+    const std::vector<double2> w = twiddles_f64(n);
+    std::vector<double2> h(n, make_double2(0.0, 0.0));
+    const bool ok = dispatch_fft_len(n, [&]<int N>() {
+        using Pl = FFTPlan<N>;
+        int ns = Pl::R[0];
+        for (int s = 1; s < Pl::NST; ++s) {
+            const int R = Pl::R[s];
+            for (int r = 1; r < R; ++r)
+                for (int k = 0; k < ns; ++k)
+                    h[(ns - Pl::R[0]) + (r - 1) * ns + k] = w[(int64_t)k * r * (N / (ns * R))];
+            ns *= R;
+        }
+    });
+    MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "no FFT plan for length %d", n);
+    return h;
+}
+
 const float2* twiddle_table(msr_ctx* ctx, int n) {
     std::lock_guard<std::mutex> guard(ctx->lock);
     auto it = ctx->twiddles.find(n);
     if (it != ctx->twiddles.end()) return it->second;
-    const std::vector<double2> h64 = twiddles_f64(n);
+    const std::vector<double2> h64 = stage_twiddles_f64(n);
     std::vector<float2> h(n);
     for (int m = 0; m < n; ++m) h[m] = make_float2((float)h64[m].x, (float)h64[m].y);  // rounded once
     float2* d = nullptr;
@@ -59,7 +83,7 @@ const double2* twiddle_table64(msr_ctx* ctx, int n) {
     std::lock_guard<std::mutex> guard(ctx->lock);
     auto it = ctx->twiddles64.find(n);
     if (it != ctx->twiddles64.end()) return it->second;
-    const std::vector<double2> h = twiddles_f64(n);
+    const std::vector<double2> h = stage_twiddles_f64(n);
     double2* d = nullptr;
     MSR_HIP(hipMalloc(&d, sizeof(double2) * n));
     MSR_HIP(hipMemcpy(d, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
