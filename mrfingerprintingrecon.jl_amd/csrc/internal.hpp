@@ -61,17 +61,17 @@ struct SpreadPlan {
     DevBuf perm;                // (M) int32: points sorted by tile
     DevBuf offs;                // (nbins + 1) int32
     // 2D output-tile streams of the ComplexF32 spreading (built on first
-    // use): per sorted point its kernel weights, cell, frame and data index;
-    // per tile the sorted points whose support reaches it (pair list), cut
-    // into bounded work items
+    // use): per sorted point its cell, frame and data index; per tile the
+    // sorted points whose support reaches it (pair list) with their kernel
+    // weights over the tile's columns and rows, cut into bounded work items
     bool streams_ready = false;
     int64_t npairs = 0, nitems = 0, nsplit = 0, nslots = 0;
-    DevBuf wts;                 // (M, 16) float: wx[0..7], wy[0..7]
+    DevBuf pw;                  // (npairs, 32) float: weights over the tile's 16 columns, 16 rows
     DevBuf cell;                // (M) int2: floor cell
     DevBuf frame_s;             // (M) int32
     DevBuf jsel_s;              // (M) int64
     DevBuf kc_s;                // (M) float2
-    DevBuf pairs;               // (npairs) int32 sorted point index, grouped by tile
+    DevBuf pairs;               // (npairs) int32: sorted point index | wave mask << 28, grouped by tile
     DevBuf items;               // (nitems) int4 {tile, q0, q1, slab slot or -1}
     DevBuf split_tiles;         // (nsplit) int2 {tile, first item}; items of a split tile are consecutive
     DevBuf split_nitems;        // (nsplit) int32

@@ -1,23 +1,28 @@
 // 2D ComplexF32 spreading of the backprojection (C*K value channels; the
 // reference runs one type-1 NUFFT per coil and coefficient).
 //
-// Output-owned tiles.  The grid is cut into 16 x 16 tiles; every tile has the
-// list of the (tile-sorted) points whose W = 8 support reaches it, in
-// increasing point order, cut into work items of at most ITEM_PAIRS points.
-// One wavefront per (work item, group of BP_CH channels): it walks the item's
-// points and lane (ox, oy) adds w_x[ox] w_y[oy] v to its cell, when that cell
-// lies in the tile, in each of the group's channel tiles (plain LDS
-// read-modify-writes in program order: the accumulation order is fixed and the
-// result bit-reproducible; the BP_CH updates per point go to different tiles
-// and overlap in the LDS pipeline).  A tile with one work item writes its
-// cells straight into the grid; the items of a split tile (the dense centre
-// of a radial trajectory) write partial tiles that a second kernel sums in
-// item order.  Points are staged 64 at a time in LDS (lane = point).
+// Output-owned blocks gathered in registers.  The grid is cut into 8 x 8
+// blocks; every block has the list of the (tile-sorted) points whose W = 8
+// support reaches it, in increasing point order, cut into work items of at
+// most ITEM_PAIRS points.  One wavefront per (work item, group of CG
+// channels), one lane per cell of the block: a lane keeps its cell's CG
+// channel sums in registers and walks the item's points, adding w_x w_y v.
+// A point's 2 CG value floats are loaded 16 per VGPR, the same 16 in each of
+// the four 16-lane rows, and each FMA takes value j from lane j of its row
+// through DPP row_newbcast, so a point costs 2 CG FMAs per cell and no other
+// arithmetic.  Its kernel weights over the block's columns and rows were laid
+// out per (point, block) pair when the streams were built.  All loads are
+// vector loads issued a few points ahead of their use (they complete in
+// order).  Each cell's sum runs over the
+// points in a fixed order (bit-reproducible).  A block with one work item
+// writes its cells straight into the grid; the items of a split block (the
+// dense centre of a radial trajectory) write partial blocks that a second
+// kernel sums in item order.
 //
-// Built once per trajectory and grid: each sorted point's kernel weights,
-// cell, frame and data index, and the pair lists (stable radix sort of
-// (tile, point) pairs by tile).  Per call: the coil data in sorted point order
-// with the density compensation applied.
+// Built once per trajectory and grid: each sorted point's cell, frame and
+// data index, the pair lists (stable radix sort of (block, point) pairs by
+// block) and the pair weights.  Per call: the channel values of every sorted
+// point, conj(U[t, k]) y_c dcf, CG channels per point and group.
 #include <algorithm>
 #include <vector>
 
@@ -27,11 +32,10 @@
 
 namespace msr {
 
-constexpr int ITEM_PAIRS = 4096;
-constexpr int BP_TB = 16;          // tile (cells per dimension)
-constexpr int BP_PITCH = 24;       // LDS row pitch: the 8 x 8 lanes of an update hit 64 banks
-constexpr int BP_CH = 4;           // channels per wavefront (one wavefront per workgroup)
-constexpr int BP_CHUNK = 64;
+constexpr int ITEM_PAIRS = 1024;
+constexpr int BP_TB = 8;           // block (cells per dimension): one lane per cell
+constexpr size_t BP_VALS_BUDGET = (size_t)1 << 30;  // bytes of channel values per pass
+constexpr int BP_CGMAX = 16;       // largest channel group: VGPR sums, two points' values in SGPRs
 
 void fill_floats(void* p, int64_t n, float v, hipStream_t st);  // normalop.hip
 
@@ -63,7 +67,7 @@ __device__ __forceinline__ int support_tiles(int64_t c, int64_t n, int* t) {
 
 __global__ void bp_points_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
                                  const int64_t* __restrict__ sel, const int* __restrict__ perm,
-                                 int64_t M, StreamGeom g, float* __restrict__ wts,
+                                 int64_t M, StreamGeom g, float2* __restrict__ frac,
                                  int2* __restrict__ cell, int* __restrict__ frame_s,
                                  int64_t* __restrict__ jsel_s, float2* __restrict__ kc_s,
                                  int* __restrict__ npairs) {
@@ -71,15 +75,16 @@ __global__ void bp_points_kernel(const float* __restrict__ kc, const int* __rest
          s += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = perm[s];
         int64_t c[2];
+        float f[2];
         for (int a = 0; a < 2; ++a) {
             const double x = (double)kc[j * 2 + a] * (double)g.n[a];
             const double fl = floor(x);
-            const float f = (float)(x - fl);
+            f[a] = (float)(x - fl);
             int64_t cc = (int64_t)fl % g.n[a];
             if (cc < 0) cc += g.n[a];
             c[a] = cc;
-            for (int o = 0; o < 8; ++o) wts[s * 16 + a * 8 + o] = es_phi_f((float)(o - 3) - f);
         }
+        frac[s] = make_float2(f[0], f[1]);
         cell[s] = make_int2((int)c[0], (int)c[1]);
         frame_s[s] = frame[j];
         jsel_s[s] = sel ? sel[j] : j;
@@ -114,11 +119,43 @@ __global__ void bp_tile_count_kernel(const int* __restrict__ keys, int64_t n, in
         atomicAdd(&counts[keys[i]], 1);
 }
 
+// Per pair q (point s in block t): the kernel weights of s over the block's 8
+// columns (pw[q][0..7]) and 8 rows (pw[q][8..15]), zero off the support and
+// off the grid.
+__global__ void bp_pair_weights_kernel(const int* __restrict__ tile_of, const float2* __restrict__ frac,
+                                       const int2* __restrict__ cell, const int* __restrict__ pairs,
+                                       int64_t npairs, StreamGeom g, float* __restrict__ pw) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < npairs;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int tile = tile_of[q];
+        const int s = pairs[q];
+        const int2 c = cell[s];
+        const float2 f = frac[s];
+        const int t0[2] = {(tile % g.nb[0]) * BP_TB, (tile / g.nb[0]) * BP_TB};
+        const int cc[2] = {c.x, c.y};
+        const float ff[2] = {f.x, f.y};
+        for (int a = 0; a < 2; ++a) {
+            const int n = (int)g.n[a];
+            for (int l = 0; l < BP_TB; ++l) {
+                float w = 0.f;
+                const int x = t0[a] + l;
+                if (x < n) {
+                    int o = x - (cc[a] - 3);  // in (-n, n + 3): one wrap each way
+                    o += o < 0 ? n : 0;
+                    o -= o >= n ? n : 0;
+                    if (o < 8) w = es_phi_f((float)(o - 3) - ff[a]);
+                }
+                pw[q * 16 + a * BP_TB + l] = w;
+            }
+        }
+    }
+}
+
 void spread_streams(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
     if (plan.streams_ready) return;
     MSR_REQUIRE(plan.d == 2, MSR_ERR_ERROR, "output-tile streams are 2D");
     const int64_t M = traj->M;
-    MSR_REQUIRE(M < ((int64_t)1 << 28), MSR_ERR_UNSUPPORTED, "more than 2^28 samples");
+    MSR_REQUIRE(M < ((int64_t)1 << 30), MSR_ERR_UNSUPPORTED, "more than 2^30 samples");
     StreamGeom g;
     for (int a = 0; a < 2; ++a) {
         g.n[a] = plan.n[a];
@@ -126,7 +163,8 @@ void spread_streams(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
     }
     const int64_t ntiles = (int64_t)g.nb[0] * g.nb[1];
     const int64_t Mp = std::max<int64_t>(M, 1);
-    plan.wts.alloc(sizeof(float) * 16 * Mp);
+    DevBuf frac;
+    frac.alloc(sizeof(float2) * Mp);
     plan.cell.alloc(sizeof(int2) * Mp);
     plan.frame_s.alloc(sizeof(int) * Mp);
     plan.jsel_s.alloc(sizeof(int64_t) * Mp);
@@ -138,7 +176,7 @@ void spread_streams(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
     if (M > 0) {
         hipLaunchKernelGGL(bp_points_kernel, dim3(grid), dim3(256), 0, st, traj->kc.as<float>(),
                            traj->frame.as<int>(), traj->sel.p ? traj->sel.as<int64_t>() : nullptr,
-                           plan.perm.as<int>(), M, g, plan.wts.as<float>(), plan.cell.as<int2>(),
+                           plan.perm.as<int>(), M, g, frac.as<float2>(), plan.cell.as<int2>(),
                            plan.frame_s.as<int>(), plan.jsel_s.as<int64_t>(),
                            plan.kc_s.as<float2>(), npp.as<int>());
         MSR_CHECK_LAUNCH();
@@ -176,6 +214,15 @@ void spread_streams(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
         MSR_HIP(hipcub::DeviceRadixSort::SortPairs(temp.p, sb, keys.as<int>(), keys_sorted.as<int>(),
                                                    vals.as<int>(), plan.pairs.as<int>(), total, 0,
                                                    bits, st));
+    }
+    // pair weights (keys_sorted: the block of each pair)
+    plan.pw.alloc(sizeof(float) * 16 * P);
+    if (total > 0) {
+        const int g2 = (int)std::min<int64_t>((total + 255) / 256, 8192);
+        hipLaunchKernelGGL(bp_pair_weights_kernel, dim3(g2), dim3(256), 0, st, keys_sorted.as<int>(),
+                           frac.as<float2>(), plan.cell.as<int2>(), plan.pairs.as<int>(),
+                           (int64_t)total, g, plan.pw.as<float>());
+        MSR_CHECK_LAUNCH();
     }
     // per-tile pair counts -> work items (host)
     counts.alloc(sizeof(int) * ntiles);
@@ -225,147 +272,180 @@ void spread_streams(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
 }
 
 // ------------------------------------------------------------------ per call
-// ys[c][s] = y[jsel(s), c] * dcf(s): coil data in sorted point order
-__global__ void bp_sorted_data_kernel(ValueSrc src, const int64_t* __restrict__ jsel_s,
-                                      const float2* __restrict__ kc_s, const int* __restrict__ perm,
-                                      int64_t M, int C, float2* __restrict__ ys) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < M * C;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int c = (int)(e / M);
-        const int64_t s = e - (int64_t)c * M;
-        float2 v = src.data[jsel_s[s] + (int64_t)c * src.nsel];
-        if (src.dcf_mode == 1 || src.dcf_mode == 2) {
-            const float2 q = kc_s[s];
-            const float r2 = q.x * q.x + q.y * q.y;
-            v = cscale(v, src.dcf_mode == 1 ? sqrtf(r2) : r2);
-        } else if (src.dcf_mode == 3) {
-            v = cscale(v, src.dcf[perm[s]]);
-        }
-        ys[e] = v;
-    }
-}
-
 __device__ __forceinline__ float2 basis_val(const ValueSrc& s, int t, int k) {
     if (!s.U) return make_float2(t == k ? 1.f : 0.f, 0.f);
     if (s.u_real) return make_float2(reinterpret_cast<const float*>(s.U)[t + (int64_t)k * s.nt], 0.f);
     return reinterpret_cast<const float2*>(s.U)[t + (int64_t)k * s.nt];
 }
 
-struct BpTileArgs {
+// vals[gl][s][j] = conj(U[t_s, k]) y[jsel(s), c] dcf(s) for channel
+// ch = (g0 + gl) CG + j = c K + k (zero beyond the last channel): the channel
+// values of sorted point s, CG per group; one thread per (point, group),
+// blockIdx.y = gl, each coil sample read once per group.
+template <int CG>
+__global__ void bp_values_kernel(ValueSrc src, const int64_t* __restrict__ jsel_s,
+                                 const float2* __restrict__ kc_s, const int* __restrict__ perm,
+                                 const int* __restrict__ frame_s, int64_t M, int nch, int g0,
+                                 float2* __restrict__ vals) {
+    const int gl = blockIdx.y;
+    const int chb = (g0 + gl) * CG;
+    const int K = src.K;
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < M;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t js = jsel_s[s];
+        const int t = frame_s[s];
+        float dcf = 1.f;
+        if (src.dcf_mode == 1 || src.dcf_mode == 2) {
+            const float2 q = kc_s[s];
+            const float r2 = q.x * q.x + q.y * q.y;
+            dcf = src.dcf_mode == 1 ? sqrtf(r2) : r2;
+        } else if (src.dcf_mode == 3) {
+            dcf = src.dcf[perm[s]];
+        }
+        float2 out[CG];
+        int c = chb / K, k = chb - c * K;
+        float2 y = make_float2(0.f, 0.f);
+        int cy = -1;
+#pragma unroll
+        for (int j = 0; j < CG; ++j) {
+            out[j] = make_float2(0.f, 0.f);
+            if (chb + j < nch) {
+                if (c != cy) {
+                    y = cscale(src.data[js + (int64_t)c * src.nsel], dcf);
+                    cy = c;
+                }
+                out[j] = cmulc(basis_val(src, t, k), y);
+            }
+            if (++k == K) {
+                k = 0;
+                ++c;
+            }
+        }
+        float4* dst = reinterpret_cast<float4*>(vals + ((int64_t)gl * M + s) * CG);
+#pragma unroll
+        for (int j = 0; j < CG / 2; ++j)
+            dst[j] = make_float4(out[2 * j].x, out[2 * j].y, out[2 * j + 1].x, out[2 * j + 1].y);
+    }
+}
+
+struct BpGatherArgs {
     const int4* items;
     const int* pairs;
-    const float* wts;
-    const int2* cell;
-    const int* frame_s;
-    const float2* ys;     // (C, M)
+    const float* pw;
+    const float2* vals;   // (ngp, M, CG)
     int64_t M;
-    int nch;
+    int nch, g0;
     StreamGeom g;
     float2* grid;         // (nch, n1, n0)
     float2* slab;         // (slots, nch, TB*TB)
 };
 
-// One wavefront per workgroup, BP_CH channels per wavefront: for every point
-// the BP_CH read-modify-writes go to different tiles, so they overlap in the
-// LDS pipeline; the order per tile is the program order (deterministic).
-__global__ void __launch_bounds__(64) bp_tile_kernel(BpTileArgs a, ValueSrc src) {
-    __shared__ float tre[BP_CH][BP_TB * BP_PITCH];
-    __shared__ float tim[BP_CH][BP_TB * BP_PITCH];
-    __shared__ float2 sv[BP_CH][BP_CHUNK];
-    __shared__ int2 scell[BP_CHUNK];
-    __shared__ float sw[BP_CHUNK][16];
-    const int4 it = a.items[blockIdx.x];
-    const int tile = it.x, q0 = it.y, q1 = it.z, slot = it.w;
-    const int ch0 = blockIdx.y * BP_CH;
-    const int n0 = (int)a.g.n[0], n1 = (int)a.g.n[1];
-    const int t0x = (tile % a.g.nb[0]) * BP_TB, t0y = (tile / a.g.nb[0]) * BP_TB;
-    const int ex = min(BP_TB, n0 - t0x);
-    const int ey = min(BP_TB, n1 - t0y);
-    const int lane = threadIdx.x;
-    for (int e = lane; e < BP_CH * BP_TB * BP_PITCH; e += 64) {
-        (&tre[0][0])[e] = 0.f;
-        (&tim[0][0])[e] = 0.f;
-    }
-    const int ox = lane & 7, oy = lane >> 3;
-    int cc[BP_CH], kk[BP_CH];
-#pragma unroll
-    for (int h = 0; h < BP_CH; ++h) {
-        const int ch = min(ch0 + h, a.nch - 1);
-        cc[h] = ch / src.K;
-        kk[h] = ch - cc[h] * src.K;
-    }
-    const int nchunk = (q1 - q0 + BP_CHUNK - 1) / BP_CHUNK;
-    for (int n = 0; n < nchunk; ++n) {
-        // stage the chunk: lane = point
-        const int q = q0 + n * BP_CHUNK + lane;
-        const int np = min(BP_CHUNK, q1 - q0 - n * BP_CHUNK);
-        if (lane < np) {
-            const int s = a.pairs[q];
-            const int t = a.frame_s[s];
-#pragma unroll
-            for (int h = 0; h < BP_CH; ++h)
-                sv[h][lane] = cmulc(basis_val(src, t, kk[h]), a.ys[(int64_t)cc[h] * a.M + s]);
-            scell[lane] = a.cell[s];
-            const float4* w4 = reinterpret_cast<const float4*>(a.wts + (int64_t)s * 16);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) reinterpret_cast<float4*>(sw[lane])[r] = w4[r];
-        }
-        __syncthreads();
-        // software pipeline: the staged data of point p + 1 is read while
-        // point p's read-modify-writes are in flight
-        int2 cn = scell[0];
-        float wxn = sw[0][ox], wyn = sw[0][8 + oy];
-        float2 vn[BP_CH];
-#pragma unroll
-        for (int h = 0; h < BP_CH; ++h) vn[h] = sv[h][0];
-        for (int p = 0; p < np; ++p) {
-            const int2 c = cn;
-            const float w = wxn * wyn;
-            float2 v[BP_CH];
-#pragma unroll
-            for (int h = 0; h < BP_CH; ++h) v[h] = vn[h];
-            if (p + 1 < np) {
-                cn = scell[p + 1];
-                wxn = sw[p + 1][ox];
-                wyn = sw[p + 1][8 + oy];
-#pragma unroll
-                for (int h = 0; h < BP_CH; ++h) vn[h] = sv[h][p + 1];
-            }
-            int ax = c.x - 3 + ox;
-            ax += ax < 0 ? n0 : 0;
-            ax -= ax >= n0 ? n0 : 0;
-            int ay = c.y - 3 + oy;
-            ay += ay < 0 ? n1 : 0;
-            ay -= ay >= n1 ? n1 : 0;
-            const int rx = ax - t0x, ry = ay - t0y;
-            if (rx >= 0 && rx < ex && ry >= 0 && ry < ey) {
-                const int o = ry * BP_PITCH + rx;
-#pragma unroll
-                for (int h = 0; h < BP_CH; ++h) {
-                    tre[h][o] += w * v[h].x;
-                    tim[h][o] += w * v[h].y;
-                }
-            }
-        }
-        __syncthreads();
-    }
-    // output: the tile's cells into the grid, or a partial tile into the slab
-    const int chg = min(BP_CH, a.nch - ch0);
-    const int64_t G = a.g.n[0] * a.g.n[1];
-    for (int e = lane; e < chg * BP_TB * BP_TB; e += 64) {
-        const int k = e / (BP_TB * BP_TB), r = e - k * (BP_TB * BP_TB);
-        const int ry = r / BP_TB, rx = r - ry * BP_TB;
-        const float2 v = make_float2(tre[k][ry * BP_PITCH + rx], tim[k][ry * BP_PITCH + rx]);
-        if (slot < 0) {
-            if (rx < ex && ry < ey)
-                a.grid[(int64_t)(ch0 + k) * G + (int64_t)(t0y + ry) * a.g.n[0] + t0x + rx] = v;
-        } else {
-            a.slab[((int64_t)slot * a.nch + ch0 + k) * (BP_TB * BP_TB) + r] = v;
-        }
-    }
+// acc[J0 + j] += v_j w for j < 16, where v_j is lane j of the 16-lane row of v
+// (the point's values, the same 16 floats in each of the four rows): DPP
+// row_newbcast hands lane j of each row to the whole row, so the 16 FMAs need
+// no other instruction.  s_nop 1: two wait states before a DPP read of a VGPR
+// that a VALU instruction may have just written.
+template <int J0, int NA>
+__device__ __forceinline__ void fma_bcast16(float (&acc)[NA], float v, float w) {
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %16, %17 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %1, %16, %17 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %2, %16, %17 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %3, %16, %17 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %4, %16, %17 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %5, %16, %17 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %6, %16, %17 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %7, %16, %17 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %8, %16, %17 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %9, %16, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %10, %16, %17 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %11, %16, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %12, %16, %17 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %13, %16, %17 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %14, %16, %17 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %15, %16, %17 row_newbcast:15 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc[J0 + 0]), "+v"(acc[J0 + 1]), "+v"(acc[J0 + 2]), "+v"(acc[J0 + 3]),
+          "+v"(acc[J0 + 4]), "+v"(acc[J0 + 5]), "+v"(acc[J0 + 6]), "+v"(acc[J0 + 7]),
+          "+v"(acc[J0 + 8]), "+v"(acc[J0 + 9]), "+v"(acc[J0 + 10]), "+v"(acc[J0 + 11]),
+          "+v"(acc[J0 + 12]), "+v"(acc[J0 + 13]), "+v"(acc[J0 + 14]), "+v"(acc[J0 + 15])
+        : "v"(v), "v"(w));
 }
 
-// split tiles: grid cells = sum of the tile's partial tiles in item order
+constexpr int BP_DEPTH = 4;  // points whose loads are in flight per wavefront
+
+// One wavefront per (work item, channel group), lane (lx, ly) on the cell
+// (lx, ly) of the 8 x 8 block.  The loads of a point (its 2 CG value floats,
+// 16 per VGPR replicated over the four rows, and the lane's two weights) are
+// issued BP_DEPTH points ahead of its use, the pair word (point index) BP_DEPTH
+// points before that; vector loads complete in order, so each use waits only
+// for loads issued BP_DEPTH points earlier.
+template <int CG>
+__global__ void __launch_bounds__(64) bp_gather_kernel(BpGatherArgs a) {
+    constexpr int NV = 2 * CG / 16;  // value VGPRs per point
+    constexpr int D = BP_DEPTH;
+    const int4 it = a.items[blockIdx.x];
+    const int tile = it.x, q0 = it.y, q1 = it.z, slot = it.w;
+    const int gl = blockIdx.y;
+    const int ch0 = (a.g0 + gl) * CG;
+    const int n0 = (int)a.g.n[0], n1 = (int)a.g.n[1];
+    const int t0x = (tile % a.g.nb[0]) * BP_TB, t0y = (tile / a.g.nb[0]) * BP_TB;
+    const int lane = threadIdx.x;
+    const int lx = lane & 7, ly = lane >> 3;
+    const float* __restrict__ vf = reinterpret_cast<const float*>(a.vals + (size_t)gl * a.M * CG) + (lane & 15);
+    const float* __restrict__ pwx = a.pw + lx;
+    const float* __restrict__ pwy = a.pw + BP_TB + ly;
+    const int* __restrict__ pq = a.pairs;
+    float acc[2 * CG];
+#pragma unroll
+    for (int j = 0; j < 2 * CG; ++j) acc[j] = 0.f;
+    const int n = q1 - q0;
+    const int last = q1 - 1;
+    int wd[D];            // pair words of points p + D .. p + 2D - 1
+    float vv[D][NV];      // values of points p .. p + D - 1
+    float wx[D], wy[D];   // weights of points p .. p + D - 1
+    auto issue = [&](int i, int pt, int word) {  // loads of point q0 + pt into stage i
+        const int q = min(q0 + pt, last);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) vv[i][v] = vf[(int64_t)word * (2 * CG) + 16 * v];
+        wx[i] = pwx[(int64_t)q * 16];
+        wy[i] = pwy[(int64_t)q * 16];
+    };
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        const int w0 = pq[min(q0 + i, last)];
+        issue(i, i, w0);
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) wd[i] = pq[min(q0 + D + i, last)];
+    for (int p = 0; p < n; p += D) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            if (p + i < n) {
+                const float w = wx[i] * wy[i];
+                fma_bcast16<0>(acc, vv[i][0], w);
+                if constexpr (NV > 1) fma_bcast16<16>(acc, vv[i][NV - 1], w);
+            }
+            issue(i, p + i + D, wd[i]);
+            wd[i] = pq[min(q0 + p + i + 2 * D, last)];
+        }
+    }
+    const int ex = min(BP_TB, n0 - t0x), ey = min(BP_TB, n1 - t0y);
+    if (lx >= ex || ly >= ey) return;
+    const int64_t G = a.g.n[0] * a.g.n[1];
+#pragma unroll
+    for (int j = 0; j < CG; ++j) {
+        if (ch0 + j >= a.nch) break;
+        const float2 r = make_float2(acc[2 * j], acc[2 * j + 1]);
+        if (slot < 0)
+            a.grid[(int64_t)(ch0 + j) * G + (int64_t)(t0y + ly) * n0 + t0x + lx] = r;
+        else
+            a.slab[((int64_t)slot * a.nch + ch0 + j) * (BP_TB * BP_TB) + lane] = r;
+    }
+    (void)n1;
+}
+
+// split blocks: grid cells = sum of the block's partial blocks in item order
 __global__ void bp_split_reduce_kernel(const int2* __restrict__ split, const int* __restrict__ nit,
                                        const int4* __restrict__ items, const float2* __restrict__ slab,
                                        int nch, StreamGeom g, float2* __restrict__ grid) {
@@ -389,11 +469,30 @@ __global__ void bp_split_reduce_kernel(const int2* __restrict__ split, const int
     }
 }
 
+// channel group size: the fewest groups of at most BP_CGMAX channels, each
+// rounded up to an instantiated size
+static int bp_group_size(int nch) {
+    const int ng = (int)ceil_div(nch, BP_CGMAX);
+    const int need = (int)ceil_div(nch, ng);
+    return need <= 8 ? 8 : BP_CGMAX;
+}
+
+// channel groups per pass: all of them (one launch with enough wavefronts to
+// fill the GPU), unless their values exceed BP_VALS_BUDGET
+static int bp_groups_per_pass(int64_t M, int CG, int ng) {
+    const size_t per = sizeof(float2) * (size_t)std::max<int64_t>(M, 1) * CG;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)ng, BP_VALS_BUDGET / per));
+}
+
 size_t spread_bp_workspace_bytes(const msr_traj* traj, const SpreadPlan& plan, int nch, int ncoils,
                                  hipStream_t st) {
+    (void)ncoils;
     spread_streams(traj, const_cast<SpreadPlan&>(plan), st);
+    const int CG = bp_group_size(nch);
+    const int ng = (int)ceil_div(nch, CG);
+    const int gp = bp_groups_per_pass(traj->M, CG, ng);
     return sizeof(float2) * ((size_t)plan.nslots * nch * BP_TB * BP_TB +
-                             (size_t)ncoils * std::max<int64_t>(traj->M, 1));
+                             (size_t)gp * std::max<int64_t>(traj->M, 1) * CG);
 }
 
 void spread_bp(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src, int nch,
@@ -401,7 +500,6 @@ void spread_bp(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src
     MSR_REQUIRE(plan.streams_ready, MSR_ERR_ERROR, "point streams not built");
     MSR_REQUIRE(src.kind == 0 && nch % src.K == 0, MSR_ERR_ERROR,
                 "stream spreading takes backprojection values");
-    const int C = nch / src.K;
     const int64_t M = traj->M;
     const int64_t G = plan.n[0] * plan.n[1];
     StreamGeom g;
@@ -412,19 +510,31 @@ void spread_bp(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src
     // tiles no point reaches stay zero
     fill_floats(grid, 2 * G * nch, 0.f, st);
     if (plan.nitems == 0) return;
-    float2* slab = work;
-    float2* ys = work + (size_t)plan.nslots * nch * BP_TB * BP_TB;
-    const int gr = (int)std::min<int64_t>((M * C + 255) / 256, 16384);
-    hipLaunchKernelGGL(bp_sorted_data_kernel, dim3(gr), dim3(256), 0, st, src,
-                       plan.jsel_s.as<int64_t>(), plan.kc_s.as<float2>(), plan.perm.as<int>(), M,
-                       C, ys);
-    MSR_CHECK_LAUNCH();
-    BpTileArgs a{plan.items.as<int4>(), plan.pairs.as<int>(), plan.wts.as<float>(),
-                 plan.cell.as<int2>(), plan.frame_s.as<int>(), ys, M, nch, g, grid, slab};
     MSR_REQUIRE(plan.nitems < ((int64_t)1 << 31), MSR_ERR_UNSUPPORTED, "too many work items");
-    dim3 grd((unsigned)plan.nitems, (unsigned)ceil_div(nch, BP_CH));
-    hipLaunchKernelGGL(bp_tile_kernel, grd, dim3(64), 0, st, a, src);
-    MSR_CHECK_LAUNCH();
+    const int CG = bp_group_size(nch);
+    const int ng = (int)ceil_div(nch, CG);
+    const int gp = bp_groups_per_pass(M, CG, ng);
+    float2* slab = work;
+    float2* vals = work + (size_t)plan.nslots * nch * BP_TB * BP_TB;
+    for (int g0 = 0; g0 < ng; g0 += gp) {
+        const int ngp = std::min(gp, ng - g0);
+        const int gr = (int)std::min<int64_t>(ceil_div(M, 256), 8192);
+        if (CG == 8)
+            hipLaunchKernelGGL(bp_values_kernel<8>, dim3(gr, ngp), dim3(256), 0, st, src,
+                               plan.jsel_s.as<int64_t>(), plan.kc_s.as<float2>(), plan.perm.as<int>(),
+                               plan.frame_s.as<int>(), M, nch, g0, vals);
+        else
+            hipLaunchKernelGGL(bp_values_kernel<BP_CGMAX>, dim3(gr, ngp), dim3(256), 0, st, src,
+                               plan.jsel_s.as<int64_t>(), plan.kc_s.as<float2>(), plan.perm.as<int>(),
+                               plan.frame_s.as<int>(), M, nch, g0, vals);
+        MSR_CHECK_LAUNCH();
+        BpGatherArgs a{plan.items.as<int4>(), plan.pairs.as<int>(), plan.pw.as<float>(), vals, M,
+                       nch, g0, g, grid, slab};
+        dim3 grd((unsigned)plan.nitems, (unsigned)ngp);
+        if (CG == 8) hipLaunchKernelGGL(bp_gather_kernel<8>, grd, dim3(64), 0, st, a);
+        else hipLaunchKernelGGL(bp_gather_kernel<BP_CGMAX>, grd, dim3(64), 0, st, a);
+        MSR_CHECK_LAUNCH();
+    }
     if (plan.nsplit > 0) {
         hipLaunchKernelGGL(bp_split_reduce_kernel, dim3((unsigned)plan.nsplit, (unsigned)nch),
                            dim3(256), 0, st, plan.split_tiles.as<int2>(),

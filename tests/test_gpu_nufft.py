@@ -82,6 +82,11 @@ BP_CASES = [
     ((32, 16), 2, 3, 6, 3, "radial_2D", True),
     ((24, 40), 4, 2, 5, 4, "radial_3D", False),
     ((48, 48), 5, 6, 10, 2, "none", False),
+    # 64 x 64 grid, 16k samples: the four centre tiles hold more than 4096
+    # (point, tile) pairs and are split into work items summed afterwards;
+    # 30 and 60 channels make two groups of 16 and three of 25 (padded)
+    ((32, 32), 5, 6, 64, 4, "radial_2D", False),
+    ((32, 32), 5, 12, 64, 4, "none", True),
 ]
 
 
