@@ -9,9 +9,11 @@ plus 25 CG iterations with the Toeplitz normal operator; the kernel, coil maps
 and sorted trajectory stay resident on the device between solves.
 
 Prints one JSON line with solves/s, p50/p99 latency, the plain
-one-recon-at-a-time GPU path (reference GPU order: per-coil applies that re-read
-the kernel, IterativeSolvers-style host-synchronised CG with ComplexF32
-vectors) timed beside it, the HBM roofline of the Toeplitz apply, the CPU
+one-recon-at-a-time GPU path (the reference's GPU call sequence on this
+library's kernels: one single-channel type-1 NUFFT per coefficient and coil for
+the backprojection, per-coil applies that re-read the kernel,
+IterativeSolvers-style host-synchronised CG with ComplexF32 vectors) timed
+beside it, the HBM roofline of the Toeplitz apply, the CPU
 baseline (FP32 port of the reference CPU path) and the relative error of the
 solve against an FP64 oracle.
 """
@@ -168,7 +170,7 @@ def main():
                 precision=PRECISION)
 
     def solve_plain():
-        msr.calculate_backprojection(data_d, traj, cml, U=U_d, out=b)
+        msr.calculate_backprojection(data_d, traj, cml, U=U_d, out=b, channelwise=True)
         msr.cg_(x, A, bflat, maxiter=CG_ITERS, reltol=0.0, initially_zero=True, mode="host",
                 precision="f32")
 
@@ -259,6 +261,8 @@ def main():
                    "samples_per_frame": NR, "cg_iters": CG_ITERS},
         "latency_ms": {"p50": round(percentile(lat, 50), 4), "p99": round(percentile(lat, 99), 4)},
         "plain_gpu": {"value": round(plain_sps, 3), "unit": "solves/s",
+                      "path": "reference GPU call sequence: K*C single-channel NUFFT backprojection, "
+                              "per-coil applies re-reading the kernel, host-synchronised ComplexF32 CG",
                       "ms_per_solve": round(1e3 * pwall / args.plain_steps, 3),
                       "p50_ms": round(percentile(plat, 50), 3),
                       "speedup": round(solves_per_s / plain_sps, 3),

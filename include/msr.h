@@ -154,6 +154,16 @@ int msr_backprojection(msr_ctx *ctx, const msr_traj *traj, int d, const int64_t 
                        const void *const *cmaps, int dcf_mode, const float *dcf, void *out,
                        void *stream);
 
+/* msr_backprojection with coil maps, computed in the reference's call
+ * sequence (ext/MRISubspaceReconCUDAExt/BackProjection.jl:56-66): one
+ * single-channel type-1 NUFFT per (coefficient, coil), each spreading every
+ * sample, accumulated with conj(S_c).  Same result; it exists to time the
+ * reference's sequence on this library's kernels.  U and cmaps are required. */
+int msr_backprojection_channelwise(msr_ctx *ctx, const msr_traj *traj, int d,
+                                   const int64_t *img_shape, const void *data, int ncoils, int K,
+                                   const void *U, int u_real, const void *const *cmaps,
+                                   int dcf_mode, const float *dcf, void *out, void *stream);
+
 /* --------------------------------------------------------- forward model */
 /* The tests' data simulation (test/reconstruct_radial.jl:62-74: per coil and
  * frame, NonuniformFFTs exec_type2! of S_c * sum_k U[t,k] x_k), i.e. the

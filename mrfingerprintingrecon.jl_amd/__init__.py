@@ -306,13 +306,17 @@ def _test_dimension(data, trj_nt, trj_nsamp, trj_d, U_nt, img_shape, ncoil):
 
 
 def calculate_backprojection(data, trj, cmaps_or_img_shape, U=None, sample_mask=None,
-                             density_compensation="none", verbose=False, out=None):
+                             density_compensation="none", verbose=False, out=None,
+                             channelwise=False):
     """calculate_backprojection(data, trj, cmaps; U, sample_mask,
     density_compensation) -> (K, *N[::-1]) coil-combined
     (ext/MRISubspaceReconCUDAExt/BackProjection.jl:36-71), or with img_shape
     -> (C, K, *N[::-1]) per coil (:1-34).  4D data/trj wrapper :74-79.
     ``trj`` may also be a trajectory handle from ``plan_trajectory`` (reused
-    sample selection and tile sort); ``out`` an output tensor to write into."""
+    sample selection and tile sort); ``out`` an output tensor to write into.
+    ``channelwise=True`` runs the reference's call sequence (one
+    single-channel NUFFT per coefficient and coil, :56-66) for timing
+    comparisons; it needs U and coil maps."""
     torch = _torch()
     data = _dev_tensor(data, "data", torch.complex64)
     if data.dim() == 4:  # (C, Nt, Ncyc, Nr)
@@ -364,7 +368,10 @@ def calculate_backprojection(data, trj, cmaps_or_img_shape, U=None, sample_mask=
           or not out.is_contiguous() or out.device != data.device):
         raise ArgumentError(f"out must be a contiguous complex64 tensor of {shape}")
     carr = None if per_coil else (ctypes.c_void_p * len(cl))(*[c.data_ptr() for c in cl])
-    _lib.call("msr_backprojection", _ctx(data.device.index), traj.h, len(img_shape),
+    if channelwise and (per_coil or uptr is None):
+        raise ArgumentError("the channel-wise backprojection needs U and coil maps")
+    _lib.call("msr_backprojection_channelwise" if channelwise else "msr_backprojection",
+              _ctx(data.device.index), traj.h, len(img_shape),
               _i64arr(img_shape), _ptr(data), C, K, uptr, real, carr, mode, dcf_ptr, _ptr(out),
               _stream_ptr(data.device))
     return out

@@ -16,7 +16,11 @@ void op_build_from_traj(msr_normalop* op, const msr_traj* traj, const void* U, i
 // backproj.hip
 void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, const void* data,
                     int ncoils, int K, const void* U, int u_real, const void* const* cmaps,
-                    int dcf_mode, const float* dcf, void* out, hipStream_t st);
+                    int dcf_mode, const float* dcf, void* out, hipStream_t st, bool accumulate = false);
+void backprojection_channelwise(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
+                                const void* data, int ncoils, int K, const void* U, int u_real,
+                                const void* const* cmaps, int dcf_mode, const float* dcf, void* out,
+                                hipStream_t st);
 
 // forward.hip
 void forward_model(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, const void* x,
@@ -235,6 +239,22 @@ int msr_backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t*
         ensure_device(ctx);
         backprojection(ctx, traj, d, img_shape, data, ncoils, K, U, u_real, cmaps, dcf_mode, dcf,
                        out, (hipStream_t)stream);
+    });
+}
+
+int msr_backprojection_channelwise(msr_ctx* ctx, const msr_traj* traj, int d,
+                                   const int64_t* img_shape, const void* data, int ncoils, int K,
+                                   const void* U, int u_real, const void* const* cmaps, int dcf_mode,
+                                   const float* dcf, void* out, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(traj);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(data);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        backprojection_channelwise(ctx, traj, d, img_shape, data, ncoils, K, U, u_real, cmaps,
+                                   dcf_mode, dcf, out, (hipStream_t)stream);
     });
 }
 

@@ -36,7 +36,7 @@ __global__ void combine_kernel(const float2* __restrict__ img, CmapPtrs maps, in
 
 void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, const void* data,
                     int ncoils, int K, const void* U, int u_real, const void* const* cmaps,
-                    int dcf_mode, const float* dcf, void* out, hipStream_t st) {
+                    int dcf_mode, const float* dcf, void* out, hipStream_t st, bool accumulate) {
     MSR_REQUIRE(traj->d == d, MSR_ERR_ARGUMENT,
                 "`cmaps` contains %d image dimensions, yet the 1st dimension of trj is of "
                 "length %d", d, traj->d);
@@ -176,8 +176,34 @@ void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
                 mp.p[c] = (const float2*)cmaps[c0 + c];
             }
             hipLaunchKernelGGL(combine_kernel, dim3(grid), dim3(256), 0, st, img_out, mp, c0, nc, K,
-                               V, c0 > 0 ? 1 : 0, (float2*)out);
+                               V, (c0 > 0 || accumulate) ? 1 : 0, (float2*)out);
             MSR_CHECK_LAUNCH();
+        }
+    }
+}
+
+// The reference's call sequence (ext/MRISubspaceReconCUDAExt/BackProjection.jl:
+// 56-66): one single-channel type-1 NUFFT per (coefficient, coil), each
+// spreading every sample, its image accumulated into b_k with conj(S_c).  The
+// result equals backprojection(); this form times the reference's sequence on
+// this library's kernels (the baseline the fused form is compared with).
+void backprojection_channelwise(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
+                                const void* data, int ncoils, int K, const void* U, int u_real,
+                                const void* const* cmaps, int dcf_mode, const float* dcf, void* out,
+                                hipStream_t st) {
+    MSR_REQUIRE(U != nullptr, MSR_ERR_ARGUMENT, "the channel-wise backprojection needs a basis U");
+    MSR_REQUIRE(cmaps != nullptr, MSR_ERR_ARGUMENT, "the channel-wise backprojection needs coil maps");
+    MSR_REQUIRE(ncoils >= 1 && K >= 1, MSR_ERR_ARGUMENT, "ncoils = %d, K = %d", ncoils, K);
+    int64_t V = 1;
+    for (int a = 0; a < d; ++a) V *= N[a];
+    const int64_t nsel = traj->nsamp * traj->nt;
+    for (int k = 0; k < K; ++k) {
+        const void* Uk = u_real ? (const void*)((const float*)U + (int64_t)k * traj->nt)
+                                : (const void*)((const float2*)U + (int64_t)k * traj->nt);
+        for (int c = 0; c < ncoils; ++c) {
+            const void* data_c = (const float2*)data + (int64_t)c * nsel;
+            backprojection(ctx, traj, d, N, data_c, 1, 1, Uk, u_real, cmaps + c, dcf_mode, dcf,
+                           (float2*)out + (int64_t)k * V, st, c > 0);
         }
     }
 }

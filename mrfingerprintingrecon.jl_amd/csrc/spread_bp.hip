@@ -35,7 +35,7 @@ namespace msr {
 constexpr int ITEM_PAIRS = 1024;
 constexpr int BP_TB = 8;           // block (cells per dimension): one lane per cell
 constexpr size_t BP_VALS_BUDGET = (size_t)1 << 30;  // bytes of channel values per pass
-constexpr int BP_CGMAX = 16;       // largest channel group: VGPR sums, two points' values in SGPRs
+constexpr int BP_CGMAX = 16;       // largest channel group (2 CG value floats in two VGPRs)
 
 void fill_floats(void* p, int64_t n, float v, hipStream_t st);  // normalop.hip
 
@@ -321,10 +321,15 @@ __global__ void bp_values_kernel(ValueSrc src, const int64_t* __restrict__ jsel_
                 ++c;
             }
         }
-        float4* dst = reinterpret_cast<float4*>(vals + ((int64_t)gl * M + s) * CG);
+        if constexpr (CG % 2 == 0) {
+            float4* dst = reinterpret_cast<float4*>(vals + ((int64_t)gl * M + s) * CG);
 #pragma unroll
-        for (int j = 0; j < CG / 2; ++j)
-            dst[j] = make_float4(out[2 * j].x, out[2 * j].y, out[2 * j + 1].x, out[2 * j + 1].y);
+            for (int j = 0; j < CG / 2; ++j)
+                dst[j] = make_float4(out[2 * j].x, out[2 * j].y, out[2 * j + 1].x, out[2 * j + 1].y);
+        } else {
+#pragma unroll
+            for (int j = 0; j < CG; ++j) vals[((int64_t)gl * M + s) * CG + j] = out[j];
+        }
     }
 }
 
@@ -372,6 +377,16 @@ __device__ __forceinline__ void fma_bcast16(float (&acc)[NA], float v, float w) 
         : "v"(v), "v"(w));
 }
 
+// The same for a single channel: acc[0..1] += (lane 0, lane 1 of the row) w.
+__device__ __forceinline__ void fma_bcast2(float (&acc)[2], float v, float w) {
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %2, %3 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %1, %2, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc[0]), "+v"(acc[1])
+        : "v"(v), "v"(w));
+}
+
 constexpr int BP_DEPTH = 4;  // points whose loads are in flight per wavefront
 
 // One wavefront per (work item, channel group), lane (lx, ly) on the cell
@@ -382,7 +397,8 @@ constexpr int BP_DEPTH = 4;  // points whose loads are in flight per wavefront
 // for loads issued BP_DEPTH points earlier.
 template <int CG>
 __global__ void __launch_bounds__(64) bp_gather_kernel(BpGatherArgs a) {
-    constexpr int NV = 2 * CG / 16;  // value VGPRs per point
+    constexpr int NV = CG == 1 ? 1 : 2 * CG / 16;  // value VGPRs per point
+    constexpr int LV = CG == 1 ? 2 : 16;          // value floats per VGPR (per 16-lane row)
     constexpr int D = BP_DEPTH;
     const int4 it = a.items[blockIdx.x];
     const int tile = it.x, q0 = it.y, q1 = it.z, slot = it.w;
@@ -392,7 +408,8 @@ __global__ void __launch_bounds__(64) bp_gather_kernel(BpGatherArgs a) {
     const int t0x = (tile % a.g.nb[0]) * BP_TB, t0y = (tile / a.g.nb[0]) * BP_TB;
     const int lane = threadIdx.x;
     const int lx = lane & 7, ly = lane >> 3;
-    const float* __restrict__ vf = reinterpret_cast<const float*>(a.vals + (size_t)gl * a.M * CG) + (lane & 15);
+    const float* __restrict__ vf =
+        reinterpret_cast<const float*>(a.vals + (size_t)gl * a.M * CG) + (lane & 15) % LV;
     const float* __restrict__ pwx = a.pw + lx;
     const float* __restrict__ pwy = a.pw + BP_TB + ly;
     const int* __restrict__ pq = a.pairs;
@@ -423,8 +440,12 @@ __global__ void __launch_bounds__(64) bp_gather_kernel(BpGatherArgs a) {
         for (int i = 0; i < D; ++i) {
             if (p + i < n) {
                 const float w = wx[i] * wy[i];
-                fma_bcast16<0>(acc, vv[i][0], w);
-                if constexpr (NV > 1) fma_bcast16<16>(acc, vv[i][NV - 1], w);
+                if constexpr (CG == 1) {
+                    fma_bcast2(acc, vv[i][0], w);
+                } else {
+                    fma_bcast16<0>(acc, vv[i][0], w);
+                    if constexpr (NV > 1) fma_bcast16<16>(acc, vv[i][NV - 1], w);
+                }
             }
             issue(i, p + i + D, wd[i]);
             wd[i] = pq[min(q0 + p + i + 2 * D, last)];
@@ -470,11 +491,11 @@ __global__ void bp_split_reduce_kernel(const int2* __restrict__ split, const int
 }
 
 // channel group size: the fewest groups of at most BP_CGMAX channels, each
-// rounded up to an instantiated size
+// rounded up to an instantiated size (1, 8 or 16)
 static int bp_group_size(int nch) {
     const int ng = (int)ceil_div(nch, BP_CGMAX);
     const int need = (int)ceil_div(nch, ng);
-    return need <= 8 ? 8 : BP_CGMAX;
+    return need == 1 ? 1 : (need <= 8 ? 8 : BP_CGMAX);
 }
 
 // channel groups per pass: all of them (one launch with enough wavefronts to
@@ -519,19 +540,20 @@ void spread_bp(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src
     for (int g0 = 0; g0 < ng; g0 += gp) {
         const int ngp = std::min(gp, ng - g0);
         const int gr = (int)std::min<int64_t>(ceil_div(M, 256), 8192);
-        if (CG == 8)
-            hipLaunchKernelGGL(bp_values_kernel<8>, dim3(gr, ngp), dim3(256), 0, st, src,
-                               plan.jsel_s.as<int64_t>(), plan.kc_s.as<float2>(), plan.perm.as<int>(),
-                               plan.frame_s.as<int>(), M, nch, g0, vals);
-        else
-            hipLaunchKernelGGL(bp_values_kernel<BP_CGMAX>, dim3(gr, ngp), dim3(256), 0, st, src,
-                               plan.jsel_s.as<int64_t>(), plan.kc_s.as<float2>(), plan.perm.as<int>(),
-                               plan.frame_s.as<int>(), M, nch, g0, vals);
+#define MSR_BP_VALUES(CGV)                                                                         \
+    hipLaunchKernelGGL(bp_values_kernel<CGV>, dim3(gr, ngp), dim3(256), 0, st, src,                \
+                       plan.jsel_s.as<int64_t>(), plan.kc_s.as<float2>(), plan.perm.as<int>(),     \
+                       plan.frame_s.as<int>(), M, nch, g0, vals)
+        if (CG == 1) MSR_BP_VALUES(1);
+        else if (CG == 8) MSR_BP_VALUES(8);
+        else MSR_BP_VALUES(BP_CGMAX);
+#undef MSR_BP_VALUES
         MSR_CHECK_LAUNCH();
         BpGatherArgs a{plan.items.as<int4>(), plan.pairs.as<int>(), plan.pw.as<float>(), vals, M,
                        nch, g0, g, grid, slab};
         dim3 grd((unsigned)plan.nitems, (unsigned)ngp);
-        if (CG == 8) hipLaunchKernelGGL(bp_gather_kernel<8>, grd, dim3(64), 0, st, a);
+        if (CG == 1) hipLaunchKernelGGL(bp_gather_kernel<1>, grd, dim3(64), 0, st, a);
+        else if (CG == 8) hipLaunchKernelGGL(bp_gather_kernel<8>, grd, dim3(64), 0, st, a);
         else hipLaunchKernelGGL(bp_gather_kernel<BP_CGMAX>, grd, dim3(64), 0, st, a);
         MSR_CHECK_LAUNCH();
     }
