@@ -233,6 +233,51 @@ int msr_batch_cg_finish(msr_batch *batch, void *stream);
  * each, any may be NULL); synchronises the stream. */
 int msr_batch_cg_status(msr_batch *batch, int *iters, double *residual, int *done, void *stream);
 
+/* ------------------------------------------------------ shared kernels */
+/* A further operator on `proto`'s Toeplitz kernel and kmask (held until the
+ * last operator using them is destroyed) with its own coil maps (ncoils
+ * device pointers, or 0/NULL for S = 1), apply scratch and CG state, so that
+ * several solves of one protocol (trajectory and basis) run at once, e.g.
+ * in a batch or a server.  No counterpart in the reference (its operator owns
+ * its kernel); callers that serve many reconstructions of one protocol. */
+int msr_normalop_create_shared(msr_ctx *ctx, const msr_normalop *proto, int ncoils,
+                               const void *const *cmaps, void *stream, msr_normalop **out);
+/* Device bytes of an operator's own state for a solve at `precision`
+ * (coil maps, apply scratch, CG vectors) and of its (shareable) kernel. */
+int msr_normalop_footprint(const msr_normalop *op, int precision, size_t *own_bytes,
+                           size_t *kernel_bytes);
+
+/* --------------------------------------------------------------- server */
+/* Continuous batching of CG solves (the serving loop for streams of
+ * reconstructions).  No counterpart in the reference.  A job is an operator
+ * with its b and x (ComplexF32 device vectors), maxiter and tolerances; it
+ * waits in a FIFO queue until it fits: msr_server_step admits queued jobs while
+ * the device state of the jobs in flight (msr_normalop_footprint) stays within
+ * `budget_bytes` and there are at most `max_active` of them, starts their CG
+ * at `precision`, runs `niter` lockstep CG iterations over every job in
+ * flight (msr_batch_cg_iterate) and retires the jobs whose CG has stopped
+ * (solution written to x).  A job's result is the one its operator gives in
+ * msr_cg alone.  One job per operator at a time; the operators must outlive
+ * their jobs.  Not re-entrant. */
+#define MSR_JOB_QUEUED 0
+#define MSR_JOB_RUNNING 1
+#define MSR_JOB_DONE 2
+typedef struct msr_server msr_server;
+int msr_server_create(msr_ctx *ctx, size_t budget_bytes, int max_active, int precision,
+                      msr_server **out);
+int msr_server_destroy(msr_server *server);
+int msr_server_submit(msr_server *server, msr_normalop *op, void *x, const void *b, int maxiter,
+                      double reltol, double abstol, int64_t *job_id);
+/* n_active / n_queued (may be NULL): jobs in flight / waiting after the step;
+ * synchronises the stream. */
+int msr_server_step(msr_server *server, int niter, int *n_active, int *n_queued, void *stream);
+/* state: MSR_JOB_*; times_ms (3 entries, may be NULL): time queued, time in
+ * flight, total (host clock, to the step that retired the job). */
+int msr_server_poll(msr_server *server, int64_t job_id, int *state, int *iters, double *residual,
+                    double *times_ms);
+/* Drops a finished job's record. */
+int msr_server_forget(msr_server *server, int64_t job_id);
+
 #ifdef __cplusplus
 }
 #endif

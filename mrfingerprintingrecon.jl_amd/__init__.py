@@ -286,6 +286,30 @@ def NFFTNormalOp(img_shape, a, b, cmaps=None, sample_mask=None, verbose=False):
     return LinearOperator(_NFFTNormalOp(h, img_shape, traj.device, cl))
 
 
+def NFFTNormalOp_shared(A, cmaps=None):
+    """A further normal operator on A's Toeplitz kernel (shared device memory)
+    with its own coil maps, scratch and CG state (msr_normalop_create_shared):
+    several reconstructions of one protocol can then be solved at once."""
+    torch = _torch()
+    src = _op_state(A)
+    cl = [_dev_tensor(c, "cmaps", torch.complex64) for c in _cmaps_list(cmaps)]
+    carr = (ctypes.c_void_p * max(1, len(cl)))(*[c.data_ptr() for c in cl]) if cl else None
+    h = ctypes.c_void_p()
+    _lib.call("msr_normalop_create_shared", _ctx(src.device.index), src.h, len(cl), carr,
+              _stream_ptr(src.device), ctypes.byref(h))
+    return LinearOperator(_NFFTNormalOp(h, src.shape, src.device, cl))
+
+
+def footprint(A, precision="f64"):
+    """(own bytes, kernel bytes) of an operator's device state for a solve at
+    `precision` (msr_normalop_footprint)."""
+    own = ctypes.c_size_t()
+    ker = ctypes.c_size_t()
+    _lib.call("msr_normalop_footprint", _op_state(A).h, _prec(precision), ctypes.byref(own),
+              ctypes.byref(ker))
+    return own.value, ker.value
+
+
 def _test_dimension(data, trj_nt, trj_nsamp, trj_d, U_nt, img_shape, ncoil):
     """test_dimension (src/BackProjection.jl:183-202), on reversed shapes."""
     C, nt, ns = data.shape
@@ -572,3 +596,77 @@ class NFFTNormalOpBatch:
 
 
 __all__ += ["NFFTNormalOpBatch"]
+
+
+class NFFTServer:
+    """Continuous batching of CG solves (msr_server_*): jobs wait in a FIFO
+    queue, are admitted while the device state of the jobs in flight fits
+    ``budget_bytes`` (at most ``max_active`` of them), run in lockstep batched
+    CG iterations and retire when their CG stops.  Each job's result is the
+    one ``cg_`` gives for its operator alone (same precision and stopping)."""
+
+    QUEUED, RUNNING, DONE = 0, 1, 2
+
+    def __init__(self, device=None, budget_bytes=None, max_active=64, precision="f64"):
+        torch = _torch()
+        dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        if budget_bytes is None:
+            free, _total = torch.cuda.mem_get_info(dev)
+            budget_bytes = int(free * 0.8)
+        self.device = dev
+        self.precision = precision
+        self._keep = {}
+        h = ctypes.c_void_p()
+        _lib.call("msr_server_create", _ctx(dev.index), ctypes.c_size_t(int(budget_bytes)),
+                  int(max_active), _prec(precision), ctypes.byref(h))
+        self.h = h
+
+    def submit(self, A, x, b, maxiter, reltol=0.0, abstol=0.0):
+        """Queue the solve of A x = b (x, b: contiguous complex64 device vectors
+        of length V*K); returns the job id."""
+        op = _op_state(A)
+        n = math.prod(op.shape) * op.Ncoeff
+        for t, name in ((x, "x"), (b, "b")):
+            if t.numel() != n or not t.is_contiguous() or t.device != self.device:
+                raise ArgumentError(f"{name}: expected a contiguous device vector of length {n}")
+        jid = ctypes.c_int64()
+        _lib.call("msr_server_submit", self.h, op.h, _ptr(x), _ptr(b), int(maxiter), float(reltol),
+                  float(abstol), ctypes.byref(jid))
+        self._keep[jid.value] = (A, x, b)
+        return jid.value
+
+    def step(self, niter=5, stream=None):
+        """Admit, run niter batched CG iterations, retire; returns
+        (jobs in flight, jobs queued)."""
+        na = ctypes.c_int()
+        nq = ctypes.c_int()
+        st = _stream_ptr(self.device) if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        _lib.call("msr_server_step", self.h, int(niter), ctypes.byref(na), ctypes.byref(nq), st)
+        return na.value, nq.value
+
+    def poll(self, job):
+        """{'state', 'iters', 'residual', 'queued_ms', 'running_ms', 'total_ms'}."""
+        state = ctypes.c_int()
+        its = ctypes.c_int()
+        res = ctypes.c_double()
+        t = (ctypes.c_double * 3)()
+        _lib.call("msr_server_poll", self.h, int(job), ctypes.byref(state), ctypes.byref(its),
+                  ctypes.byref(res), t)
+        return {"state": state.value, "iters": its.value, "residual": res.value,
+                "queued_ms": t[0], "running_ms": t[1], "total_ms": t[2]}
+
+    def forget(self, job):
+        _lib.call("msr_server_forget", self.h, int(job))
+        self._keep.pop(job, None)
+
+    def run(self, niter=5):
+        """Steps until no job is in flight or queued."""
+        while True:
+            na, nq = self.step(niter)
+            if na == 0 and nq == 0:
+                return
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib.load().msr_server_destroy(self.h)
+            self.h = None

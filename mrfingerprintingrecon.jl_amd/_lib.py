@@ -69,6 +69,14 @@ SIGNATURES = {
     "msr_forward": (_I, [_P, _P, _I, _P, _P, _I, _P, _I, _I, _P, _P, _P]),
     "msr_cg": (_I, [_P, _P, _P, _I, _D, _D, _I, _I, _I, _P, _P, _P]),
     "msr_normalop_apply_f64": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),
+    "msr_normalop_create_shared": (_I, [_P, _P, _I, _P, _P, _P]),
+    "msr_normalop_footprint": (_I, [_P, _I, _P, _P]),
+    "msr_server_create": (_I, [_P, ctypes.c_size_t, _I, _I, _P]),
+    "msr_server_destroy": (_I, [_P]),
+    "msr_server_submit": (_I, [_P, _P, _P, _P, _I, _D, _D, _P]),
+    "msr_server_step": (_I, [_P, _I, _P, _P, _P]),
+    "msr_server_poll": (_I, [_P, ctypes.c_int64, _P, _P, _P, _P]),
+    "msr_server_forget": (_I, [_P, ctypes.c_int64]),
     "msr_batch_create": (_I, [_P, _I, _P, _P]),
     "msr_batch_destroy": (_I, [_P]),
     "msr_batch_apply": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),

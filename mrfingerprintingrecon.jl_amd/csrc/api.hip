@@ -5,6 +5,7 @@
 
 #include "batch.hpp"
 #include "normalop.hpp"
+#include "server.hpp"
 
 namespace msr {
 // traj.hip
@@ -26,6 +27,14 @@ void backprojection_channelwise(msr_ctx* ctx, const msr_traj* traj, int d, const
 void forward_model(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, const void* x,
                    int K, const void* U, int u_real, int ncoils, const void* const* cmaps, void* out,
                    hipStream_t st);
+
+// server.hip
+msr_server* server_create(msr_ctx* ctx, size_t budget, int max_active, int prec);
+int64_t server_submit(msr_server* s, msr_normalop* op, float2* x, const float2* b, int maxiter,
+                      double reltol, double abstol);
+void server_step(msr_server* s, int niter, int* n_active, int* n_queued, hipStream_t st);
+void server_poll(msr_server* s, int64_t id, int* state, int* iters, double* res, double* times);
+void server_forget(msr_server* s, int64_t id);
 
 static thread_local std::string g_err;
 void set_last_error(const std::string& s) { g_err = s; }
@@ -380,3 +389,87 @@ int msr_batch_cg_status(msr_batch* batch, int* iters, double* residual, int* don
 }
 
 }  // extern "C"
+
+int msr_normalop_create_shared(msr_ctx* ctx, const msr_normalop* proto, int ncoils,
+                               const void* const* cmaps, void* stream, msr_normalop** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(proto);
+        MSR_NOTNULL(out);
+        MSR_REQUIRE(proto->ctx == ctx, MSR_ERR_ARGUMENT, "the prototype belongs to another context");
+        ensure_device(ctx);
+        hipStream_t st = (hipStream_t)stream;
+        std::unique_ptr<msr_normalop> op(op_new(ctx, proto->d, proto->N, proto->K, proto->lam_real));
+        op->nkm = proto->nkm;
+        op->lam = proto->lam;
+        op->kmask = proto->kmask;
+        op_set_cmaps(op.get(), ncoils, cmaps, st);
+        op_alloc_scratch(op.get());
+        MSR_HIP(hipStreamSynchronize(st));
+        *out = op.release();
+    });
+}
+
+int msr_normalop_footprint(const msr_normalop* op, int precision, size_t* own_bytes,
+                           size_t* kernel_bytes) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        MSR_REQUIRE(precision >= MSR_PREC_F32 && precision <= MSR_PREC_F64, MSR_ERR_ARGUMENT,
+                    "precision = %d", precision);
+        if (own_bytes) *own_bytes = op_footprint(op, precision);
+        if (kernel_bytes) *kernel_bytes = op_kernel_bytes(op);
+    });
+}
+
+int msr_server_create(msr_ctx* ctx, size_t budget_bytes, int max_active, int precision,
+                      msr_server** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(out);
+        *out = server_create(ctx, budget_bytes, max_active, precision);
+    });
+}
+
+int msr_server_destroy(msr_server* server) {
+    return guarded([&] {
+        if (!server) return;
+        ensure_device(server->ctx);
+        delete server;
+    });
+}
+
+int msr_server_submit(msr_server* server, msr_normalop* op, void* x, const void* b, int maxiter,
+                      double reltol, double abstol, int64_t* job_id) {
+    return guarded([&] {
+        MSR_NOTNULL(server);
+        MSR_NOTNULL(op);
+        MSR_NOTNULL(x);
+        MSR_NOTNULL(b);
+        const int64_t id = server_submit(server, op, (float2*)x, (const float2*)b, maxiter, reltol,
+                                         abstol);
+        if (job_id) *job_id = id;
+    });
+}
+
+int msr_server_step(msr_server* server, int niter, int* n_active, int* n_queued, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(server);
+        ensure_device(server->ctx);
+        server_step(server, niter, n_active, n_queued, (hipStream_t)stream);
+    });
+}
+
+int msr_server_poll(msr_server* server, int64_t job_id, int* state, int* iters, double* residual,
+                    double* times_ms) {
+    return guarded([&] {
+        MSR_NOTNULL(server);
+        server_poll(server, job_id, state, iters, residual, times_ms);
+    });
+}
+
+int msr_server_forget(msr_server* server, int64_t job_id) {
+    return guarded([&] {
+        MSR_NOTNULL(server);
+        server_forget(server, job_id);
+    });
+}

@@ -212,7 +212,7 @@ static Apply3DArgs args3d(msr_normalop* op, bool f64) {
     a.C = op->C;
     a.f64 = f64 ? 1 : 0;
     a.cmaps = op->identity_cmaps ? nullptr : op->cmaps.as<float2>();
-    a.lam = op->lam.p;
+    a.lam = op->lam->p;
     a.lam_real = op->lam_real;
     a.T1 = op->T.p;
     a.T2 = op->T2.p;

@@ -110,8 +110,8 @@ struct msr_normalop {
     bool identity_cmaps = true;
     int64_t nkm = 0;
     msr::DevBuf cmaps;                // (C, V) complex
-    msr::DevBuf lam;                  // dense kernel, layout per dimension (see normalop.hip)
-    msr::DevBuf kmask;                // (nkm) int64, 1-based
+    std::shared_ptr<msr::DevBuf> lam = std::make_shared<msr::DevBuf>();    // dense kernel (shared by msr_normalop_create_shared)
+    std::shared_ptr<msr::DevBuf> kmask = std::make_shared<msr::DevBuf>();  // (nkm) int64, 1-based
     msr::DevBuf T;                    // FFT intermediate
     msr::DevBuf T2;                   // second intermediate (3D)
     msr::DevBuf partial;              // (G, K, V)

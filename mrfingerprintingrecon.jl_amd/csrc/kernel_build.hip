@@ -168,7 +168,7 @@ void op_build_from_traj(msr_normalop* op, const msr_traj* traj, const void* U, i
     }
     // kmask
     DevBuf mask;
-    build_kmask(traj, n2, mask, op->kmask, op->nkm, st);
+    build_kmask(traj, n2, mask, *op->kmask, op->nkm, st);
 
     // pair tables (packed order)
     std::vector<int> pj(P), pk(P);
@@ -207,7 +207,7 @@ void op_build_from_traj(msr_normalop* op, const msr_traj* traj, const void* U, i
     B.alloc(std::max(es * (size_t)pc * (G4 / 2), spread_workspace_bytes(plan, pc, f64)));
 
     const size_t ls = op->lam_real ? sizeof(float) : sizeof(float2);
-    op->lam.alloc(ls * (size_t)P * G2);
+    op->lam->alloc(ls * (size_t)P * G2);
 
     for (int p0 = 0; p0 < P; p0 += pc) {
         const int np = std::min(pc, P - p0);
@@ -300,7 +300,7 @@ void op_build_from_traj(msr_normalop* op, const msr_traj* traj, const void* U, i
         fy.in_stride = n2[0];
         if (d == 2) {
             // final pass stores the dense kernel: lam[(qx*P + p)*ny2 + qy]
-            fy.lam = op->lam.p;
+            fy.lam = op->lam->p;
             fy.mask = mask.as<uint8_t>();
             fy.P = P;
             fy.p0 = p0;
@@ -324,7 +324,7 @@ void op_build_from_traj(msr_normalop* op, const msr_traj* traj, const void* U, i
             fz.ncols = n2[1] * n2[0];
             fz.in_bstride = G2;
             fz.in_stride = n2[1] * n2[0];
-            fz.lam = op->lam.p;
+            fz.lam = op->lam->p;
             fz.mask = mask.as<uint8_t>();
             fz.P = P;
             fz.p0 = p0;

@@ -171,12 +171,12 @@ void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, 
                     hipStream_t st) {
     const int64_t G = op_grid_cells(op);
     const size_t es = op->lam_real ? sizeof(float) : sizeof(float2);
-    op->lam.alloc(es * (size_t)op->P * G);
-    fill_floats(op->lam.p, (int64_t)(op->lam.bytes / sizeof(float)), 0.f, st);
+    op->lam->alloc(es * (size_t)op->P * G);
+    fill_floats(op->lam->p, (int64_t)(op->lam->bytes / sizeof(float)), 0.f, st);
     op->nkm = nkm;
-    op->kmask.alloc(sizeof(int64_t) * std::max<int64_t>(nkm, 1));
+    op->kmask->alloc(sizeof(int64_t) * std::max<int64_t>(nkm, 1));
     if (nkm > 0)
-        MSR_HIP(hipMemcpyAsync(op->kmask.p, kmask, sizeof(int64_t) * nkm, hipMemcpyDeviceToDevice, st));
+        MSR_HIP(hipMemcpyAsync(op->kmask->p, kmask, sizeof(int64_t) * nkm, hipMemcpyDeviceToDevice, st));
     DevBuf bad;
     bad.alloc(sizeof(int));
     fill_floats(bad.p, 1, 0.f, st);
@@ -185,12 +185,12 @@ void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, 
         GridDims g = op_grid(op);
         if (op->lam_real)
             hipLaunchKernelGGL(scatter_kernel<float>, dim3(grid), dim3(256), 0, st,
-                               (const float*)lambda, op->kmask.as<int64_t>(), nkm, op->P, op->d, g,
-                               op->lam.as<float>(), bad.as<int>());
+                               (const float*)lambda, op->kmask->as<int64_t>(), nkm, op->P, op->d, g,
+                               op->lam->as<float>(), bad.as<int>());
         else
             hipLaunchKernelGGL(scatter_kernel<float2>, dim3(grid), dim3(256), 0, st,
-                               (const float2*)lambda, op->kmask.as<int64_t>(), nkm, op->P, op->d, g,
-                               op->lam.as<float2>(), bad.as<int>());
+                               (const float2*)lambda, op->kmask->as<int64_t>(), nkm, op->P, op->d, g,
+                               op->lam->as<float2>(), bad.as<int>());
         MSR_CHECK_LAUNCH();
     }
     int hbad = 0;
@@ -225,7 +225,7 @@ Apply2DArgs args2d(msr_normalop* op, bool f64) {
     a.C = op->C;
     a.f64 = f64 ? 1 : 0;
     a.cmaps = op->identity_cmaps ? nullptr : op->cmaps.as<float2>();
-    a.lam = op->lam.p;
+    a.lam = op->lam->p;
     a.lam_real = op->lam_real;
     a.T = op->T.p;
     a.partial = op->partial.p;
@@ -298,18 +298,18 @@ void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha
 void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hipStream_t st) {
     if (op->nkm == 0) return;
     if (kmask_out)
-        MSR_HIP(hipMemcpyAsync(kmask_out, op->kmask.p, sizeof(int64_t) * op->nkm,
+        MSR_HIP(hipMemcpyAsync(kmask_out, op->kmask->p, sizeof(int64_t) * op->nkm,
                                hipMemcpyDeviceToDevice, st));
     if (lambda_out) {
         int grid = (int)std::min<int64_t>((op->nkm * op->P + 255) / 256, 16384);
         GridDims g = op_grid(op);
         if (op->lam_real)
             hipLaunchKernelGGL(gather_kernel<float>, dim3(grid), dim3(256), 0, st,
-                               op->lam.as<float>(), op->kmask.as<int64_t>(), op->nkm, op->P, op->d,
+                               op->lam->as<float>(), op->kmask->as<int64_t>(), op->nkm, op->P, op->d,
                                g, (float*)lambda_out);
         else
             hipLaunchKernelGGL(gather_kernel<float2>, dim3(grid), dim3(256), 0, st,
-                               op->lam.as<float2>(), op->kmask.as<int64_t>(), op->nkm, op->P,
+                               op->lam->as<float2>(), op->kmask->as<int64_t>(), op->nkm, op->P,
                                op->d, g, (float2*)lambda_out);
         MSR_CHECK_LAUNCH();
     }
@@ -319,6 +319,27 @@ void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hip
 // Buffers of a CG in precision prec (MSR_PREC_*): r, u (and the FP64 x) in the
 // vector type, c in the apply type, and an FP32 copy of u when the vectors are
 // FP64 but the apply is FP32.  Also sizes the apply scratch.
+size_t op_footprint(const msr_normalop* op, int prec) {
+    const bool v64 = prec != MSR_PREC_F32, a64 = prec == MSR_PREC_F64;
+    const size_t n = (size_t)op->V * op->K;
+    const size_t es = a64 ? sizeof(double2) : sizeof(float2);  // apply intermediates
+    const size_t vs = v64 ? sizeof(double2) : sizeof(float2);  // CG vectors
+    const int64_t nx = op->N[0], ny = op->N[1], nz = op->N[2];
+    size_t scratch = es * (size_t)op->G * op->K * op->V;
+    if (op->d == 2) scratch += es * (size_t)op->C * op->K * (2 * nx) * ny;
+    else scratch += es * (size_t)op->C * op->K * nz * ny * 2 * nx * 3;  // T1 + T2
+    size_t cg = vs * n * 2 + es * n;                                   // r, u, c
+    if (v64) cg += sizeof(double2) * n;                                // x in FP64
+    if (prec == MSR_PREC_MIXED) cg += sizeof(float2) * n;
+    cg += sizeof(CgScalars) + sizeof(double) * 2 * RED_BLOCKS;
+    const size_t maps = op->identity_cmaps ? 0 : sizeof(float2) * (size_t)op->C * op->V;
+    return maps + std::max(scratch, (size_t)op->T.bytes + op->T2.bytes + op->partial.bytes) + cg;
+}
+
+size_t op_kernel_bytes(const msr_normalop* op) {
+    return (op->lam ? op->lam->bytes : 0) + (op->kmask ? op->kmask->bytes : 0);
+}
+
 void op_cg_buffers(msr_normalop* op, int prec) {
     const size_t n = (size_t)op->V * op->K;
     const bool v64 = prec != MSR_PREC_F32, a64 = prec == MSR_PREC_F64;

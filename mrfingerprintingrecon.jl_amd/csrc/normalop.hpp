@@ -34,5 +34,11 @@ void op_cg_buffers(msr_normalop* op, int prec);        // allocates; not during 
 template <class VT, class AT>
 CgVecs<VT, AT> op_cg_vecs(msr_normalop* op, float2* x);  // x: caller's FP32 solution
 std::vector<const void*> op_graph_key(const msr_normalop* op);  // buffer addresses
+// Device bytes of the operator's own state for a solve at precision prec (coil
+// maps, apply scratch and CG vectors; the kernel, shared with
+// msr_normalop_create_shared, is not counted).
+size_t op_footprint(const msr_normalop* op, int prec);
+// Device bytes of the kernel and kmask.
+size_t op_kernel_bytes(const msr_normalop* op);
 
 }  // namespace msr

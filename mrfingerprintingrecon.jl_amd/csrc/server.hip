@@ -1,0 +1,155 @@
+// Reconstruction server: continuous batching of CG solves with admission by
+// device-memory footprint, on the stepwise batch engine (batch.hip).
+//
+// Jobs (an operator, its right-hand side and solution vectors, an iteration
+// budget and tolerances) wait in a FIFO queue.  Each step admits queued jobs
+// while the device state of the jobs in flight (coil maps, apply scratch and
+// CG vectors of their operators; kernels are resident and shared between
+// operators built with msr_normalop_create_shared) stays within the budget,
+// starts their CG, runs a number of lockstep CG iterations over every job in
+// flight (one batched launch sequence), and retires the jobs whose CG has
+// stopped, writing their solutions.  A job that joins while others run
+// overlaps with them on the device; every job's result is the one its
+// operator gives alone (the batch engine runs each operator's blocks exactly
+// as a single-operator launch does).
+#include <algorithm>
+
+#include "server.hpp"
+
+namespace msr {
+
+static double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+msr_server* server_create(msr_ctx* ctx, size_t budget, int max_active, int prec) {
+    MSR_REQUIRE(max_active >= 1, MSR_ERR_ARGUMENT, "max_active = %d", max_active);
+    MSR_REQUIRE(prec == MSR_PREC_F32 || prec == MSR_PREC_MIXED || prec == MSR_PREC_F64,
+                MSR_ERR_ARGUMENT, "precision = %d", prec);
+    auto* s = new msr_server;
+    s->ctx = ctx;
+    s->budget = budget;
+    s->max_active = max_active;
+    s->prec = prec;
+    return s;
+}
+
+int64_t server_submit(msr_server* s, msr_normalop* op, float2* x, const float2* b, int maxiter,
+                      double reltol, double abstol) {
+    MSR_REQUIRE(op->ctx == s->ctx, MSR_ERR_ARGUMENT, "the operator belongs to another context");
+    MSR_REQUIRE(maxiter >= 0, MSR_ERR_ARGUMENT, "maxiter = %d", maxiter);
+    for (const auto& kv : s->jobs)
+        MSR_REQUIRE(kv.second.state == MSR_JOB_DONE || kv.second.op != op, MSR_ERR_ARGUMENT,
+                    "the operator already has a job in this server (one solve per operator at "
+                    "a time; msr_normalop_create_shared gives more operators on one kernel)");
+    ServerJob j;
+    j.op = op;
+    j.x = x;
+    j.b = b;
+    j.maxiter = maxiter;
+    j.reltol = reltol;
+    j.abstol = abstol;
+    j.bytes = op_footprint(op, s->prec);
+    MSR_REQUIRE(j.bytes <= s->budget, MSR_ERR_ARGUMENT,
+                "the job needs %zu bytes of device state, above the server budget of %zu", j.bytes,
+                s->budget);
+    j.t_submit = now_ms();
+    const int64_t id = s->next_id++;
+    s->jobs[id] = j;
+    s->queue.push_back(id);
+    return id;
+}
+
+static void rebuild_batch(msr_server* s) {
+    delete s->batch;
+    s->batch = nullptr;
+    if (s->active.empty()) return;
+    std::vector<msr_normalop*> ops;
+    for (int64_t id : s->active) ops.push_back(s->jobs[id].op);
+    s->batch = batch_create(s->ctx, (int)ops.size(), ops.data());
+}
+
+void server_step(msr_server* s, int niter, int* n_active, int* n_queued, hipStream_t st) {
+    MSR_REQUIRE(niter >= 1, MSR_ERR_ARGUMENT, "niter = %d", niter);
+    // admit (FIFO) while the jobs in flight fit the budget; a job alone always fits
+    std::vector<int64_t> admitted;
+    while (!s->queue.empty() && (int)(s->active.size() + admitted.size()) < s->max_active) {
+        ServerJob& j = s->jobs[s->queue.front()];
+        if (s->active_bytes + j.bytes > s->budget) break;
+        s->active_bytes += j.bytes;
+        admitted.push_back(s->queue.front());
+        s->queue.pop_front();
+    }
+    for (int64_t id : admitted) {
+        ServerJob& j = s->jobs[id];
+        std::unique_ptr<msr_batch> one(batch_create(s->ctx, 1, &j.op));
+        batch_cg_start(one.get(), &j.x, &j.b, j.maxiter, j.reltol, j.abstol, true, s->prec, st);
+        MSR_HIP(hipStreamSynchronize(st));  // the start kernels read the batch's tables
+        j.state = MSR_JOB_RUNNING;
+        j.t_start = now_ms();
+        s->active.push_back(id);
+    }
+    if (!admitted.empty()) rebuild_batch(s);
+    if (s->batch) {
+        batch_cg_iterate(s->batch, niter, st);
+        // retire the jobs whose CG has stopped
+        const size_t na = s->active.size();
+        std::vector<int> iters(na), done(na);
+        std::vector<double> res(na);
+        batch_cg_status(s->batch, iters.data(), res.data(), done.data(), st);
+        std::vector<msr_normalop*> fin;
+        std::vector<int64_t> keep;
+        for (size_t i = 0; i < na; ++i) {
+            ServerJob& j = s->jobs[s->active[i]];
+            j.iters = iters[i];
+            j.res = res[i];
+            if (done[i]) fin.push_back(j.op);
+            else keep.push_back(s->active[i]);
+        }
+        if (!fin.empty()) {
+            std::unique_ptr<msr_batch> fb(batch_create(s->ctx, (int)fin.size(), fin.data()));
+            batch_cg_finish(fb.get(), st);
+            MSR_HIP(hipStreamSynchronize(st));
+            const double t = now_ms();
+            for (size_t i = 0; i < na; ++i) {
+                ServerJob& j = s->jobs[s->active[i]];
+                if (!done[i]) continue;
+                j.state = MSR_JOB_DONE;
+                j.t_done = t;
+                s->active_bytes -= j.bytes;
+            }
+            s->active = keep;
+            rebuild_batch(s);
+        }
+    }
+    if (n_active) *n_active = (int)s->active.size();
+    if (n_queued) *n_queued = (int)s->queue.size();
+}
+
+void server_poll(msr_server* s, int64_t id, int* state, int* iters, double* res, double* times) {
+    auto it = s->jobs.find(id);
+    MSR_REQUIRE(it != s->jobs.end(), MSR_ERR_ARGUMENT, "no job %lld in this server", (long long)id);
+    const ServerJob& j = it->second;
+    if (state) *state = j.state;
+    if (iters) *iters = j.iters;
+    if (res) *res = j.res;
+    if (times) {
+        const double t = now_ms();
+        const double start = j.state == MSR_JOB_QUEUED ? t : j.t_start;
+        const double end = j.state == MSR_JOB_DONE ? j.t_done : t;
+        times[0] = start - j.t_submit;  // queued
+        times[1] = end - start;         // in flight
+        times[2] = end - j.t_submit;    // total
+    }
+}
+
+void server_forget(msr_server* s, int64_t id) {
+    auto it = s->jobs.find(id);
+    MSR_REQUIRE(it != s->jobs.end(), MSR_ERR_ARGUMENT, "no job %lld in this server", (long long)id);
+    MSR_REQUIRE(it->second.state == MSR_JOB_DONE, MSR_ERR_ARGUMENT,
+                "job %lld has not finished", (long long)id);
+    s->jobs.erase(it);
+}
+
+}  // namespace msr
