@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end FP64 pipeline check")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 recons) leg")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--apply-only", type=int, default=0, metavar="N",
                     help="profiling mode: build the operator, run N fused applies, N per-coil "
                          "applies and a 256 MiB calibration copy, then exit")
@@ -274,10 +276,193 @@ def main():
         "rel_err_vs_fp64": None,
     }
 
+    if not args.no_batched:
+        result["batched"] = batched_leg(msr)
+    if not args.no_served:
+        result["served"] = served_leg(msr)
     if not args.no_cpu:
         result["cpu_baseline"], result["rel_err_vs_fp64"] = cpu_leg(
             msr, trj, U, cm, data, A, x_gpu, b_gpu, e2e=not args.no_e2e)
     print(json.dumps(result), flush=True)
+
+
+def _job_inputs(msr, torch, dev, rng, N, C, K, spokes, nt, seed):
+    """Synthetic 2D job: golden-angle radial trajectory (2N samples per spoke),
+    T2 basis, Gaussian coil maps, complex Gaussian k-space data."""
+    trj = msr.synth.traj_2d_radial_goldenratio(2 * N, spokes, nt)
+    U = msr.synth.t2_basis(nt, K, seed=seed)
+    cm = msr.synth.gaussian_coil_maps((N, N), C, seed=seed)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    data = torch.randn((C, nt, spokes * 2 * N), dtype=torch.complex64, device=dev, generator=g)
+    return (torch.from_numpy(trj).to(dev), torch.from_numpy(U).to(dev),
+            [torch.from_numpy(c).to(dev) for c in cm], data)
+
+
+def batched_leg(msr, njobs=64, iters=CG_ITERS, seed=1004):
+    """Batched 2D reconstructions (64 jobs, N in {128,160,192,256}, 8-32 coils,
+    K 3-6, 1-4 spokes per frame, 500-1000 frames): backprojections, then one
+    lockstep CG over all jobs (graph), against the one-at-a-time loop (each
+    job's backprojection and single-operator graph CG in turn)."""
+    import numpy as np
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    jobs = []
+    log(f"batched: building {njobs} jobs")
+    for i in range(njobs):
+        N = int(rng.choice([128, 160, 192, 256]))
+        C = int(rng.integers(8, 33))
+        K = int(rng.integers(3, 7))
+        spokes = int(rng.integers(1, 5))
+        nt = int(rng.integers(500, 1001))
+        trj, U, cml, data = _job_inputs(msr, torch, dev, rng, N, C, K, spokes, nt, seed * 100 + i)
+        traj = msr.plan_trajectory(trj)
+        A = msr.NFFTNormalOp((N, N), traj, U, cmaps=cml)
+        b = torch.empty((K, N, N), dtype=torch.complex64, device=dev)
+        jobs.append(dict(N=N, C=C, K=K, M=traj.M, traj=traj, U=U, cml=cml, data=data, A=A, b=b,
+                         x1=torch.empty(K * N * N, dtype=torch.complex64, device=dev),
+                         xb=torch.empty(K * N * N, dtype=torch.complex64, device=dev)))
+    B = msr.NFFTNormalOpBatch([j["A"] for j in jobs])
+
+    def one_at_a_time():
+        for j in jobs:
+            msr.calculate_backprojection(j["data"], j["traj"], j["cml"], U=j["U"], out=j["b"])
+            msr.cg_(j["x1"], j["A"], j["b"].reshape(-1), maxiter=iters, reltol=0.0,
+                    initially_zero=True, mode="graph", precision=PRECISION)
+
+    def batched():
+        for j in jobs:
+            msr.calculate_backprojection(j["data"], j["traj"], j["cml"], U=j["U"], out=j["b"])
+        B.cg_([j["xb"] for j in jobs], [j["b"].reshape(-1) for j in jobs], maxiter=iters,
+              reltol=0.0, initially_zero=True, mode="graph", precision=PRECISION)
+
+    out = {}
+    for name, fn in (("one_at_a_time", one_at_a_time), ("batched", batched)):
+        fn()  # warm-up: graph captures
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        out[name] = time.perf_counter() - t0
+    err = max(float(torch.linalg.norm(j["xb"] - j["x1"]) / torch.linalg.norm(j["x1"])) for j in jobs)
+    res = {"workload": f"{njobs} 2D recons: N in {{128,160,192,256}}, 8-32 coils, K 3-6, 1-4 spokes x "
+                       f"500-1000 frames (radial golden angle), BP + {iters}-iteration CG each",
+           "solves_per_s": round(njobs / out["batched"], 2),
+           "one_at_a_time_solves_per_s": round(njobs / out["one_at_a_time"], 2),
+           "speedup": round(out["one_at_a_time"] / out["batched"], 3),
+           "samples_total": int(sum(j["M"] for j in jobs)),
+           "max_rel_diff_batched_vs_single": err}
+    log(f"batched: {res}")
+    del B, jobs
+    torch.cuda.synchronize()
+    return res
+
+
+def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, niter_step=5):
+    """Served stream: Poisson arrivals (rate jobs/s) of 2D jobs drawn as in the
+    batched leg over a set of protocols whose kernels stay resident; each job
+    brings its own coil maps (an operator sharing its protocol's kernel) and
+    data.  On arrival a job's backprojection runs and its CG is submitted to
+    the server (continuous batching, admission by device memory); latency is
+    arrival to solution.  The same arrivals served one at a time (FIFO, each
+    job's backprojection and single-operator graph CG) are the baseline."""
+    import numpy as np
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    log(f"served: {nproto} protocols, {njobs} jobs at {rate}/s")
+    protos = []
+    for p in range(nproto):
+        N = int(rng.choice([128, 160, 192, 256]))
+        K = int(rng.integers(3, 7))
+        spokes = int(rng.integers(1, 5))
+        nt = int(rng.integers(500, 1001))
+        trj, U, cml, _ = _job_inputs(msr, torch, dev, rng, N, 8, K, spokes, nt, seed * 100 + p)
+        traj = msr.plan_trajectory(trj)
+        A = msr.NFFTNormalOp((N, N), traj, U, cmaps=cml)
+        protos.append(dict(N=N, K=K, spokes=spokes, nt=nt, traj=traj, U=U, A=A))
+    jobs = []
+    t = 0.0
+    for i in range(njobs):
+        t += float(rng.exponential(1.0 / rate))
+        pr = protos[int(rng.integers(0, nproto))]
+        C = int(rng.integers(8, 33))
+        N, K = pr["N"], pr["K"]
+        cml = [torch.from_numpy(c).to(dev) for c in msr.synth.gaussian_coil_maps((N, N), C, seed=i)]
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed * 1000 + i)
+        data = torch.randn((C, pr["nt"], pr["spokes"] * 2 * N), dtype=torch.complex64, device=dev,
+                           generator=g)
+        A = msr.NFFTNormalOp_shared(pr["A"], cmaps=cml)
+        jobs.append(dict(t=t, pr=pr, cml=cml, data=data, A=A,
+                         b=torch.empty((K, N, N), dtype=torch.complex64, device=dev),
+                         x=torch.empty(K * N * N, dtype=torch.complex64, device=dev),
+                         x1=torch.empty(K * N * N, dtype=torch.complex64, device=dev)))
+    srv = msr.NFFTServer(max_active=64, precision=PRECISION)
+
+    def bp(j):
+        msr.calculate_backprojection(j["data"], j["pr"]["traj"], j["cml"], U=j["pr"]["U"], out=j["b"])
+
+    # warm-up (graph captures of the single-operator baseline, first launches)
+    for j in jobs:
+        bp(j)
+        msr.cg_(j["x1"], j["A"], j["b"].reshape(-1), maxiter=iters, reltol=0.0,
+                initially_zero=True, mode="graph", precision=PRECISION)
+    torch.cuda.synchronize()
+
+    def run_fifo():
+        lat = []
+        t0 = time.perf_counter()
+        for j in jobs:
+            while time.perf_counter() - t0 < j["t"]:
+                pass
+            bp(j)
+            msr.cg_(j["x1"], j["A"], j["b"].reshape(-1), maxiter=iters, reltol=0.0,
+                    initially_zero=True, mode="graph", precision=PRECISION)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0 - j["t"])
+        return time.perf_counter() - t0, lat
+
+    def run_server():
+        lat = [None] * len(jobs)
+        ids = {}
+        nxt = 0
+        t0 = time.perf_counter()
+        while nxt < len(jobs) or ids:
+            now = time.perf_counter() - t0
+            while nxt < len(jobs) and jobs[nxt]["t"] <= now:
+                j = jobs[nxt]
+                bp(j)
+                ids[srv.submit(j["A"], j["x"], j["b"].reshape(-1), maxiter=iters)] = nxt
+                nxt += 1
+            if ids:
+                srv.step(niter_step)
+                tdone = time.perf_counter() - t0
+                for jid in [k for k in ids if srv.poll(k)["state"] == srv.DONE]:
+                    i = ids.pop(jid)
+                    lat[i] = tdone - jobs[i]["t"]
+                    srv.forget(jid)
+        return time.perf_counter() - t0, lat
+
+    t_fifo, lat_fifo = run_fifo()
+    t_srv, lat_srv = run_server()
+    err = max(float(torch.linalg.norm(j["x"] - j["x1"]) / torch.linalg.norm(j["x1"])) for j in jobs)
+    ms = lambda v: round(1e3 * v, 3)
+    res = {"workload": f"{njobs} 2D recons, Poisson arrivals at {rate}/s over {nproto} resident "
+                       f"protocols (N in {{128,160,192,256}}, K 3-6, 1-4 spokes x 500-1000 frames), "
+                       f"per-job coil maps (8-32 coils) and data; BP + {iters}-iteration CG each",
+           "throughput_solves_per_s": round(njobs / t_srv, 2),
+           "latency_ms": {"p50": ms(percentile(lat_srv, 50)), "p99": ms(percentile(lat_srv, 99))},
+           "one_at_a_time": {"throughput_solves_per_s": round(njobs / t_fifo, 2),
+                             "latency_ms": {"p50": ms(percentile(lat_fifo, 50)),
+                                            "p99": ms(percentile(lat_fifo, 99))}},
+           "arrival_span_s": round(jobs[-1]["t"], 3),
+           "max_rel_diff_served_vs_single": err}
+    log(f"served: {res}")
+    del srv, jobs, protos
+    torch.cuda.synchronize()
+    return res
 
 
 def cpu_leg(msr, trj, U, cm, data, A, x_gpu, b_gpu, e2e=True):

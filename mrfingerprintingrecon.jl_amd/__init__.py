@@ -117,7 +117,7 @@ class _Traj:
         self.M = n.value
 
     def __del__(self):
-        if getattr(self, "h", None) is not None and _lib._lib is not None:
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._lib is not None:
             _lib.load().msr_traj_destroy(self.h)
             self.h = None
 
@@ -188,7 +188,7 @@ class _NFFTNormalOp:
         return x
 
     def __del__(self):
-        if getattr(self, "h", None) is not None and _lib._lib is not None:
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._lib is not None:
             _lib.load().msr_normalop_destroy(self.h)
             self.h = None
 
@@ -590,7 +590,7 @@ class NFFTNormalOpBatch:
         return ctypes.c_void_p(stream.cuda_stream)
 
     def __del__(self):
-        if getattr(self, "h", None) is not None and _lib._lib is not None:
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._lib is not None:
             _lib.load().msr_batch_destroy(self.h)
             self.h = None
 
@@ -667,6 +667,6 @@ class NFFTServer:
                 return
 
     def __del__(self):
-        if getattr(self, "h", None) is not None and _lib._lib is not None:
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._lib is not None:
             _lib.load().msr_server_destroy(self.h)
             self.h = None
