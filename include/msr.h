@@ -44,6 +44,7 @@ extern "C" {
 #define MSR_ERR_UNSUPPORTED 4 /* size or option this build has no kernel for */
 #define MSR_ERR_NOMEM 5       /* device allocation failed */
 #define MSR_ERR_HIP 6         /* HIP runtime error */
+#define MSR_ERR_IO 7          /* kernel file missing, unreadable, truncated or corrupt */
 
 /* density_compensation modes (src/BackProjection.jl:169-181) */
 #define MSR_DCF_NONE 0
@@ -246,6 +247,65 @@ int msr_normalop_create_shared(msr_ctx *ctx, const msr_normalop *proto, int ncoi
  * (coil maps, apply scratch, CG vectors) and of its (shareable) kernel. */
 int msr_normalop_footprint(const msr_normalop *op, int precision, size_t *own_bytes,
                            size_t *kernel_bytes);
+
+/* ------------------------------------------- kernel persistence and cache */
+/* No counterpart in the reference, whose callers rebuild the kernel per
+ * NFFTNormalOp(img_shape, trj, U) or keep Λ and kmask_indcs themselves for the
+ * precomputed-kernel constructor (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:22-70).
+ * For servers that see the same protocol (trajectory, sample mask, basis,
+ * image size) many times.
+ *
+ * Kernel file: a 128-byte header (magic "MSRTOEPL", version 1, d, K, P,
+ * lambda_real, img_shape[3], nkm, protocol key, payload size and checksum,
+ * header checksum), then kmask (nkm Int64, 1-based) and the packed kernel
+ * (P, nkm) ComplexF32 or Float32: exactly the two arrays of the
+ * precomputed-kernel constructor, in Julia column-major order. */
+
+/* 128-bit key of the kernel msr_normalop_create_from_traj would build from
+ * (traj, img_shape, K, U, u_real): a hash of the image size, K, the basis
+ * values and kind, the masked trajectory points and per-frame sample counts,
+ * and the library version, computed on the device.  Synchronises the stream. */
+int msr_kernel_key(msr_ctx *ctx, const msr_traj *traj, int d, const int64_t *img_shape, int K,
+                   const void *U, int u_real, void *stream, uint64_t *key);
+/* Writes op's kernel and kmask to `path` (through a temporary file renamed
+ * into place).  key: 2 words stored in the header, or NULL (0).
+ * Synchronises the stream. */
+int msr_kernel_save(const msr_normalop *op, const char *path, const uint64_t *key, void *stream);
+/* NFFTNormalOp(img_shape, Λ, kmask_indcs; cmaps) from a kernel file: checks
+ * the header, sizes and checksums (MSR_ERR_IO on any mismatch) and, if
+ * expect_key is not NULL, the stored key (MSR_ERR_ARGUMENT); kmask bounds as
+ * msr_normalop_create.  Synchronises the stream. */
+int msr_kernel_load(msr_ctx *ctx, const char *path, const uint64_t *expect_key, int ncoils,
+                    const void *const *cmaps, void *stream, msr_normalop **out);
+/* Header of a kernel file (any output may be NULL; img_shape 3 entries, key
+ * 2); verify = 1 also reads the payload and checks its checksum.  Host only:
+ * needs no device. */
+int msr_kernel_file_info(const char *path, int verify, int *d, int64_t *img_shape, int *K,
+                         int *lambda_real, int64_t *nkm, uint64_t *key);
+
+/* Kernels by protocol key: in device memory (least recently used dropped
+ * first while the cached kernels exceed budget_bytes; a dropped kernel stays
+ * alive while operators use it) and, if directory is not NULL, in kernel
+ * files <key>.msrk there (read on a memory miss, written after a build; a
+ * corrupt file is rebuilt and replaced).  msr_kernel_cache_get returns a new
+ * operator (own coil maps, scratch and CG state; destroy it as usual) on the
+ * protocol's kernel and reports where the kernel came from. */
+#define MSR_KERNEL_BUILT 0
+#define MSR_KERNEL_MEMORY 1
+#define MSR_KERNEL_DISK 2
+typedef struct msr_kernel_cache msr_kernel_cache;
+int msr_kernel_cache_create(msr_ctx *ctx, size_t budget_bytes, const char *directory,
+                            msr_kernel_cache **out);
+int msr_kernel_cache_destroy(msr_kernel_cache *cache);
+/* source (may be NULL): MSR_KERNEL_*; key (may be NULL): the protocol key. */
+int msr_kernel_cache_get(msr_kernel_cache *cache, const msr_traj *traj, int d,
+                         const int64_t *img_shape, int K, const void *U, int u_real, int ncoils,
+                         const void *const *cmaps, void *stream, msr_normalop **out, int *source,
+                         uint64_t *key);
+/* entries, bytes: cached kernels and their device bytes; counts: 3 entries,
+ * gets served per MSR_KERNEL_* source (any may be NULL). */
+int msr_kernel_cache_stats(msr_kernel_cache *cache, int64_t *entries, size_t *bytes,
+                           int64_t *counts);
 
 /* --------------------------------------------------------------- server */
 /* Continuous batching of CG solves (the serving loop for streams of

@@ -22,12 +22,14 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 
 from . import _lib, synth  # noqa: F401
 from ._lib import ArgumentError, ErrorException  # noqa: F401
 
 __all__ = ["NFFTNormalOp", "calculate_backprojection", "forward", "cg", "cg_", "mul_",
-           "LinearOperator", "plan_trajectory", "ArgumentError", "ErrorException", "synth"]
+           "LinearOperator", "plan_trajectory", "ArgumentError", "ErrorException", "synth",
+           "kernel_key", "save_kernel", "load_kernel", "kernel_file_info", "KernelCache"]
 
 _ctxs = {}
 
@@ -243,6 +245,27 @@ def mul_(res, A, v, alpha=1.0, beta=0.0):
     return A.mul_(res, v, alpha, beta)
 
 
+def _basis(U, traj):
+    """U as the library takes it: contiguous (K, Nt) rows (Julia (Nt, K)),
+    ComplexF32 or Float32 for a real basis; returns (U, real, K)."""
+    torch = _torch()
+    U = _dev_tensor(U, "U")
+    real = not U.is_complex()
+    U = U.to(torch.float32 if real else torch.complex64).contiguous()
+    if U.dim() == 1:
+        U = U.reshape(1, -1)
+    if U.shape[1] != traj.nt:
+        raise ArgumentError(f"U has {U.shape[1]} time points, trj {traj.nt}")
+    return U, real, U.shape[0]
+
+
+def _cmaps_arg(cmaps):
+    torch = _torch()
+    cl = [_dev_tensor(c, "cmaps", torch.complex64) for c in _cmaps_list(cmaps)]
+    carr = (ctypes.c_void_p * max(1, len(cl)))(*[c.data_ptr() for c in cl]) if cl else None
+    return cl, carr
+
+
 def NFFTNormalOp(img_shape, a, b, cmaps=None, sample_mask=None, verbose=False):
     """NFFTNormalOp(img_shape, trj, U; cmaps, sample_mask, verbose) or
     NFFTNormalOp(img_shape, Λ, kmask_indcs; cmaps).
@@ -271,14 +294,7 @@ def NFFTNormalOp(img_shape, a, b, cmaps=None, sample_mask=None, verbose=False):
                   _stream_ptr(dev), ctypes.byref(h))
         return LinearOperator(_NFFTNormalOp(h, img_shape, dev, cl))
     traj = a if isinstance(a, _Traj) else _Traj(a, sample_mask)
-    U = _dev_tensor(b, "U")
-    real = not U.is_complex()
-    U = U.to(torch.float32 if real else torch.complex64).contiguous()
-    if U.dim() == 1:
-        U = U.reshape(1, -1)
-    K = U.shape[0]
-    if U.shape[1] != traj.nt:
-        raise ArgumentError(f"U has {U.shape[1]} time points, trj {traj.nt}")
+    U, real, K = _basis(b, traj)
     h = ctypes.c_void_p()
     _lib.call("msr_normalop_create_from_traj", _ctx(traj.device.index), traj.h, d,
               _i64arr(img_shape), K, _ptr(U), int(real), len(cl), carr,
@@ -308,6 +324,120 @@ def footprint(A, precision="f64"):
     _lib.call("msr_normalop_footprint", _op_state(A).h, _prec(precision), ctypes.byref(own),
               ctypes.byref(ker))
     return own.value, ker.value
+
+
+# ------------------------------------------------ kernel persistence and cache
+KERNEL_SOURCES = {0: "built", 1: "memory", 2: "disk"}
+_U64x2 = ctypes.c_uint64 * 2
+
+
+def _key_arg(key):
+    if key is None:
+        return None
+    v = int(key, 16) if isinstance(key, str) else int(key)
+    return _U64x2(v >> 64, v & ((1 << 64) - 1))
+
+
+def _key_hex(k):
+    return f"{k[0]:016x}{k[1]:016x}"
+
+
+def kernel_key(img_shape, trj, U, sample_mask=None):
+    """128-bit protocol key (32 hex digits) of the kernel
+    NFFTNormalOp(img_shape, trj, U; sample_mask) builds (msr_kernel_key):
+    equal for equal image size, basis, masked trajectory and frame sizes."""
+    img_shape = tuple(int(s) for s in img_shape)
+    traj = trj if isinstance(trj, _Traj) else _Traj(trj, sample_mask)
+    U, real, K = _basis(U, traj)
+    k = _U64x2()
+    _lib.call("msr_kernel_key", _ctx(traj.device.index), traj.h, len(img_shape),
+              _i64arr(img_shape), K, _ptr(U), int(real), _stream_ptr(traj.device), k)
+    return _key_hex(k)
+
+
+def save_kernel(A, path, key=None):
+    """Writes A's packed kernel and kmask to a kernel file (msr_kernel_save):
+    the arrays of the precomputed-kernel constructor
+    (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:22-70) with a checksummed header."""
+    src = _op_state(A)
+    _lib.call("msr_kernel_save", src.h, os.fsencode(path), _key_arg(key), _stream_ptr(src.device))
+
+
+def load_kernel(path, cmaps=None, key=None, device=None):
+    """NFFTNormalOp(img_shape, Λ, kmask_indcs; cmaps) from a kernel file
+    (msr_kernel_load); OSError for a missing, truncated or corrupt file,
+    ArgumentError if `key` is given and the file holds another protocol."""
+    torch = _torch()
+    info = kernel_file_info(path, verify=False)
+    dev = torch.device("cuda", torch.cuda.current_device() if device is None
+                       else torch.device(device).index or 0)
+    cl, carr = _cmaps_arg(cmaps)
+    h = ctypes.c_void_p()
+    _lib.call("msr_kernel_load", _ctx(dev.index), os.fsencode(path), _key_arg(key), len(cl), carr,
+              _stream_ptr(dev), ctypes.byref(h))
+    return LinearOperator(_NFFTNormalOp(h, info["img_shape"], dev, cl))
+
+
+def kernel_file_info(path, verify=True):
+    """Header of a kernel file (msr_kernel_file_info; host only): img_shape,
+    Ncoeff, lambda_real, nkm, key; verify also checks the payload checksum."""
+    d = ctypes.c_int()
+    N = (ctypes.c_int64 * 3)()
+    K = ctypes.c_int()
+    lr = ctypes.c_int()
+    nkm = ctypes.c_int64()
+    k = _U64x2()
+    _lib.call("msr_kernel_file_info", os.fsencode(path), int(bool(verify)), ctypes.byref(d), N,
+              ctypes.byref(K), ctypes.byref(lr), ctypes.byref(nkm), k)
+    return {"img_shape": tuple(N[: d.value]), "Ncoeff": K.value, "lambda_real": bool(lr.value),
+            "nkm": nkm.value, "key": _key_hex(k)}
+
+
+class KernelCache:
+    """Kernels by protocol key (msr_kernel_cache_*): in device memory within
+    `budget_bytes` (least recently used dropped first) and, with `directory`,
+    in kernel files there.  get() returns a new operator on the protocol's
+    kernel, built only on a miss; ``last_source`` says where the kernel came
+    from ("built", "memory" or "disk") and ``last_key`` its key."""
+
+    def __init__(self, budget_bytes=32 << 30, directory=None, device=None):
+        torch = _torch()
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None
+                                   else torch.device(device).index or 0)
+        h = ctypes.c_void_p()
+        _lib.call("msr_kernel_cache_create", _ctx(self.device.index), int(budget_bytes),
+                  None if directory is None else os.fsencode(directory), ctypes.byref(h))
+        self.h = h
+        self.last_source = None
+        self.last_key = None
+
+    def get(self, img_shape, trj, U, cmaps=None, sample_mask=None):
+        img_shape = tuple(int(s) for s in img_shape)
+        traj = trj if isinstance(trj, _Traj) else _Traj(trj, sample_mask)
+        U, real, K = _basis(U, traj)
+        cl, carr = _cmaps_arg(cmaps)
+        h = ctypes.c_void_p()
+        src = ctypes.c_int()
+        k = _U64x2()
+        _lib.call("msr_kernel_cache_get", self.h, traj.h, len(img_shape), _i64arr(img_shape), K,
+                  _ptr(U), int(real), len(cl), carr, _stream_ptr(traj.device), ctypes.byref(h),
+                  ctypes.byref(src), k)
+        self.last_source = KERNEL_SOURCES[src.value]
+        self.last_key = _key_hex(k)
+        return LinearOperator(_NFFTNormalOp(h, img_shape, traj.device, cl))
+
+    def stats(self):
+        n = ctypes.c_int64()
+        b = ctypes.c_size_t()
+        c = (ctypes.c_int64 * 3)()
+        _lib.call("msr_kernel_cache_stats", self.h, ctypes.byref(n), ctypes.byref(b), c)
+        return {"entries": n.value, "bytes": b.value,
+                **{KERNEL_SOURCES[i]: c[i] for i in range(3)}}
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._lib is not None:
+            _lib.load().msr_kernel_cache_destroy(self.h)
+            self.h = None
 
 
 def _test_dimension(data, trj_nt, trj_nsamp, trj_d, U_nt, img_shape, ncoil):

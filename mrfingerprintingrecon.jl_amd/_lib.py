@@ -16,6 +16,7 @@ MSR_ERR_ERROR = 3
 MSR_ERR_UNSUPPORTED = 4
 MSR_ERR_NOMEM = 5
 MSR_ERR_HIP = 6
+MSR_ERR_IO = 7
 
 DCF_MODES = {"none": 0, "radial_2D": 1, "radial_3D": 2}
 DCF_ARRAY = 3
@@ -42,6 +43,7 @@ _EXC = {
     MSR_ERR_UNSUPPORTED: NotImplementedError,
     MSR_ERR_NOMEM: MemoryError,
     MSR_ERR_HIP: HIPError,
+    MSR_ERR_IO: OSError,
 }
 
 # exported symbols and their signatures (kept in step with include/msr.h)
@@ -71,6 +73,14 @@ SIGNATURES = {
     "msr_normalop_apply_f64": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),
     "msr_normalop_create_shared": (_I, [_P, _P, _I, _P, _P, _P]),
     "msr_normalop_footprint": (_I, [_P, _I, _P, _P]),
+    "msr_kernel_key": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, _P]),
+    "msr_kernel_save": (_I, [_P, ctypes.c_char_p, _P, _P]),
+    "msr_kernel_load": (_I, [_P, ctypes.c_char_p, _P, _I, _P, _P, _P]),
+    "msr_kernel_file_info": (_I, [ctypes.c_char_p, _I, _P, _P, _P, _P, _P, _P]),
+    "msr_kernel_cache_create": (_I, [_P, ctypes.c_size_t, ctypes.c_char_p, _P]),
+    "msr_kernel_cache_destroy": (_I, [_P]),
+    "msr_kernel_cache_get": (_I, [_P, _P, _I, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
+    "msr_kernel_cache_stats": (_I, [_P, _P, _P, _P]),
     "msr_server_create": (_I, [_P, ctypes.c_size_t, _I, _I, _P]),
     "msr_server_destroy": (_I, [_P]),
     "msr_server_submit": (_I, [_P, _P, _P, _P, _I, _D, _D, _P]),

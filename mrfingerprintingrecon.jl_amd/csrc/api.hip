@@ -36,6 +36,23 @@ void server_step(msr_server* s, int niter, int* n_active, int* n_queued, hipStre
 void server_poll(msr_server* s, int64_t id, int* state, int* iters, double* res, double* times);
 void server_forget(msr_server* s, int64_t id);
 
+// kcache.hip
+void kernel_key(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, int K,
+                const void* U, int u_real, hipStream_t st, uint64_t key[2]);
+void kernel_save(const msr_normalop* op, const char* path, const uint64_t* key, hipStream_t st);
+msr_normalop* kernel_load(msr_ctx* ctx, const char* path, const uint64_t* expect_key,
+                          int ncoils, const void* const* cmaps, hipStream_t st);
+void kernel_file_info(const char* path, int verify, int* d, int64_t* N, int* K, int* lam_real,
+                      int64_t* nkm, uint64_t* key);
+msr_kernel_cache* kernel_cache_create(msr_ctx* ctx, size_t budget, const char* dir);
+msr_normalop* kernel_cache_get(msr_kernel_cache* c, const msr_traj* traj, int d, const int64_t* N,
+                               int K, const void* U, int u_real, int ncoils,
+                               const void* const* cmaps, hipStream_t st, int* source,
+                               uint64_t* key_out);
+void kernel_cache_stats(msr_kernel_cache* c, int64_t* entries, size_t* bytes, int64_t* counts);
+void kernel_cache_destroy(msr_kernel_cache* c);
+msr_ctx* cache_ctx(const msr_kernel_cache* c);
+
 static thread_local std::string g_err;
 void set_last_error(const std::string& s) { g_err = s; }
 }  // namespace msr
@@ -399,14 +416,8 @@ int msr_normalop_create_shared(msr_ctx* ctx, const msr_normalop* proto, int ncoi
         MSR_REQUIRE(proto->ctx == ctx, MSR_ERR_ARGUMENT, "the prototype belongs to another context");
         ensure_device(ctx);
         hipStream_t st = (hipStream_t)stream;
-        std::unique_ptr<msr_normalop> op(op_new(ctx, proto->d, proto->N, proto->K, proto->lam_real));
-        op->nkm = proto->nkm;
-        op->lam = proto->lam;
-        op->kmask = proto->kmask;
-        op_set_cmaps(op.get(), ncoils, cmaps, st);
-        op_alloc_scratch(op.get());
-        MSR_HIP(hipStreamSynchronize(st));
-        *out = op.release();
+        *out = op_on_kernel(ctx, proto->d, proto->N, proto->K, proto->lam_real, proto->nkm,
+                            proto->lam, proto->kmask, ncoils, cmaps, st);
     });
 }
 
@@ -471,5 +482,88 @@ int msr_server_forget(msr_server* server, int64_t job_id) {
     return guarded([&] {
         MSR_NOTNULL(server);
         server_forget(server, job_id);
+    });
+}
+
+// ------------------------------------------------- kernel persistence and cache
+int msr_kernel_key(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* img_shape, int K,
+                   const void* U, int u_real, void* stream, uint64_t* key) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(traj);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(U);
+        MSR_NOTNULL(key);
+        ensure_device(ctx);
+        kernel_key(ctx, traj, d, img_shape, K, U, u_real, (hipStream_t)stream, key);
+    });
+}
+
+int msr_kernel_save(const msr_normalop* op, const char* path, const uint64_t* key, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        MSR_NOTNULL(path);
+        ensure_device(op->ctx);
+        kernel_save(op, path, key, (hipStream_t)stream);
+    });
+}
+
+int msr_kernel_load(msr_ctx* ctx, const char* path, const uint64_t* expect_key, int ncoils,
+                    const void* const* cmaps, void* stream, msr_normalop** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(path);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        *out = kernel_load(ctx, path, expect_key, ncoils, cmaps, (hipStream_t)stream);
+    });
+}
+
+int msr_kernel_file_info(const char* path, int verify, int* d, int64_t* img_shape, int* K,
+                         int* lambda_real, int64_t* nkm, uint64_t* key) {
+    return guarded([&] {
+        MSR_NOTNULL(path);
+        kernel_file_info(path, verify, d, img_shape, K, lambda_real, nkm, key);
+    });
+}
+
+int msr_kernel_cache_create(msr_ctx* ctx, size_t budget_bytes, const char* directory,
+                            msr_kernel_cache** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(out);
+        *out = kernel_cache_create(ctx, budget_bytes, directory);
+    });
+}
+
+int msr_kernel_cache_destroy(msr_kernel_cache* cache) {
+    return guarded([&] {
+        if (cache) kernel_cache_destroy(cache);
+    });
+}
+
+int msr_kernel_cache_get(msr_kernel_cache* cache, const msr_traj* traj, int d,
+                         const int64_t* img_shape, int K, const void* U, int u_real, int ncoils,
+                         const void* const* cmaps, void* stream, msr_normalop** out, int* source,
+                         uint64_t* key) {
+    return guarded([&] {
+        MSR_NOTNULL(cache);
+        MSR_NOTNULL(traj);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(U);
+        MSR_NOTNULL(out);
+        ensure_device(traj->ctx);
+        MSR_REQUIRE(traj->ctx == cache_ctx(cache), MSR_ERR_ARGUMENT,
+                    "the trajectory belongs to another context");
+        *out = kernel_cache_get(cache, traj, d, img_shape, K, U, u_real, ncoils, cmaps,
+                                (hipStream_t)stream, source, key);
+    });
+}
+
+int msr_kernel_cache_stats(msr_kernel_cache* cache, int64_t* entries, size_t* bytes,
+                           int64_t* counts) {
+    return guarded([&] {
+        MSR_NOTNULL(cache);
+        kernel_cache_stats(cache, entries, bytes, counts);
     });
 }

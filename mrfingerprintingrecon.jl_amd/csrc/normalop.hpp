@@ -40,5 +40,10 @@ std::vector<const void*> op_graph_key(const msr_normalop* op);  // buffer addres
 size_t op_footprint(const msr_normalop* op, int prec);
 // Device bytes of the kernel and kmask.
 size_t op_kernel_bytes(const msr_normalop* op);
+// A new operator on an existing kernel and kmask (kcache.hip).
+msr_normalop* op_on_kernel(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real,
+                           int64_t nkm, const std::shared_ptr<DevBuf>& lam,
+                           const std::shared_ptr<DevBuf>& kmask, int ncoils,
+                           const void* const* cmaps, hipStream_t st);
 
 }  // namespace msr
