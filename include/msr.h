@@ -307,6 +307,37 @@ int msr_kernel_cache_get(msr_kernel_cache *cache, const msr_traj *traj, int d,
 int msr_kernel_cache_stats(msr_kernel_cache *cache, int64_t *entries, size_t *bytes,
                            int64_t *counts);
 
+/* ------------------------------------------------ coil-map calibration */
+/* The GPU part of calculate_coil_maps (ext/MRISubspaceReconCUDAExt/CoilMaps.jl:1-36):
+ * calibration region, coil-wise reconstruction, calibration k-space.  The
+ * ESPIRiT step that turns the calibration k-space into coil maps
+ * (MRICoilSensitivities.jl, CPU) is not part of this library. */
+
+/* mask_calib and trj_calib (CoilMaps.jl:17-20): scale = img_shape ./
+ * calib_size rounded to Float32; trj_out = trj .* scale; mask_out[j] = all
+ * (|trj[:, j]| .* scale .< 0.5) && mask[j].  trj, trj_out (d, nsamp, nt)
+ * Float32; mask (nsamp, nt) uint8 or NULL (all); mask_out (nsamp, nt) uint8.
+ * AssertionError if a calib_size is not 2,3,5-smooth (:14). */
+int msr_calib_select(msr_ctx *ctx, int d, int64_t nsamp, int64_t nt, const float *trj,
+                     const uint8_t *mask, const int64_t *img_shape, const int64_t *calib_size,
+                     float *trj_out, uint8_t *mask_out, void *stream);
+/* reconstruct_coilwise(data, trj, img_shape; U, sample_mask, Niter_cg)
+ * (CoilMaps.jl:56-77): the K = 1 operator of the first basis column U
+ * (Nt values, ComplexF32 or Float32 with u_real; NULL for the default
+ * ones(Nt)), the per-coil backprojection, and cg!(x_c, AᴴA, b_c;
+ * maxiter = niter, reltol = 0) for each coil, all coils in lockstep on one
+ * shared kernel.  precision: MSR_PREC_* of the solves (the reference's
+ * arithmetic is MSR_PREC_F32).  data (nsamp, nt, ncoils); out (img_shape...,
+ * ncoils) ComplexF32.  Synchronises the stream. */
+int msr_reconstruct_coilwise(msr_ctx *ctx, const msr_traj *traj, int d, const int64_t *img_shape,
+                             const void *data, int ncoils, const void *U, int u_real, int niter,
+                             int precision, void *out, void *stream);
+/* kbp = fftshift(fft(fftshift(x, dims)), dims) over the image dimensions of
+ * each of ncoils images (CoilMaps.jl:22-25): x, out (calib_size..., ncoils)
+ * ComplexF32, not aliased.  Synchronises the stream. */
+int msr_calib_kspace(msr_ctx *ctx, int d, const int64_t *calib_size, int ncoils, const void *x,
+                     void *out, void *stream);
+
 /* --------------------------------------------------------------- server */
 /* Continuous batching of CG solves (the serving loop for streams of
  * reconstructions).  No counterpart in the reference.  A job is an operator

@@ -29,9 +29,16 @@ from ._lib import ArgumentError, ErrorException  # noqa: F401
 
 __all__ = ["NFFTNormalOp", "calculate_backprojection", "forward", "cg", "cg_", "mul_",
            "LinearOperator", "plan_trajectory", "ArgumentError", "ErrorException", "synth",
-           "kernel_key", "save_kernel", "load_kernel", "kernel_file_info", "KernelCache"]
+           "kernel_key", "save_kernel", "load_kernel", "kernel_file_info", "KernelCache",
+           "reconstruct_coilwise", "calibration_region", "calibration_kspace",
+           "calculate_coil_maps", "default_calib_size"]
 
 _ctxs = {}
+
+
+def _np():
+    import numpy
+    return numpy
 
 
 def _torch():
@@ -529,6 +536,143 @@ def calculate_backprojection(data, trj, cmaps_or_img_shape, U=None, sample_mask=
               _i64arr(img_shape), _ptr(data), C, K, uptr, real, carr, mode, dcf_ptr, _ptr(out),
               _stream_ptr(data.device))
     return out
+
+
+# ------------------------------------------------------ coil-map calibration
+def _nextprod235(n):
+    m = max(int(n), 1)
+    while True:
+        r = m
+        for p in (2, 3, 5):
+            while r % p == 0:
+                r //= p
+        if r == 1:
+            return m
+        m += 1
+
+
+def default_calib_size(img_shape):
+    """calib_size default of calculate_coil_maps
+    (ext/MRISubspaceReconCUDAExt/CoilMaps.jl:8): nextprod((2,3,5),
+    N_i ÷ (maximum(N) ÷ 32))."""
+    q = max(img_shape) // 32
+    if q == 0:
+        raise ZeroDivisionError("maximum(img_shape) ÷ 32 == 0: pass calib_size")
+    return tuple(_nextprod235(int(n) // q) for n in img_shape)
+
+
+def calibration_region(trj, img_shape, calib_size, sample_mask=None):
+    """(trj_calib, mask_calib) of calculate_coil_maps
+    (ext/MRISubspaceReconCUDAExt/CoilMaps.jl:17-20), on the device
+    (msr_calib_select): trj scaled by img_shape ./ calib_size and the samples
+    inside the calibration box that the sample mask keeps."""
+    torch = _torch()
+    trj = _dev_tensor(trj, "trj", torch.float32)
+    if trj.dim() == 4:
+        trj = trj.reshape(trj.shape[0], -1, trj.shape[3])
+    trj = trj.contiguous()
+    nt, ns, d = trj.shape
+    if len(img_shape) != d or len(calib_size) != d:
+        raise ArgumentError("img_shape and calib_size must have one entry per trajectory dimension")
+    mptr = None
+    if sample_mask is not None:
+        m = _dev_tensor(sample_mask, "sample_mask").reshape(nt, ns).to(torch.uint8).contiguous()
+        mptr = _ptr(m)
+    t_out = torch.empty_like(trj)
+    m_out = torch.empty((nt, ns), dtype=torch.uint8, device=trj.device)
+    _lib.call("msr_calib_select", _ctx(trj.device.index), d, ns, nt, _ptr(trj), mptr,
+              _i64arr(img_shape), _i64arr(calib_size), _ptr(t_out), _ptr(m_out),
+              _stream_ptr(trj.device))
+    return t_out, m_out.to(torch.bool)
+
+
+def reconstruct_coilwise(data, trj, img_shape, U=None, sample_mask=None, Niter_cg=100,
+                         verbose=False, precision="f64"):
+    """reconstruct_coilwise(data, trj, img_shape; U, sample_mask, Niter_cg)
+    (ext/MRISubspaceReconCUDAExt/CoilMaps.jl:56-77): one image per coil from
+    cg!(x_c, AᴴA, b_c; maxiter=Niter_cg, reltol=0) on the K = 1 operator of
+    U[:, 1] (default ones(Nt)); all coils solved in lockstep on one kernel
+    (msr_reconstruct_coilwise).  Returns (C, *img_shape[::-1]) complex64.
+    precision: CG arithmetic ("f32" is the reference's)."""
+    torch = _torch()
+    img_shape = tuple(int(s) for s in img_shape)
+    data = _dev_tensor(data, "data", torch.complex64)
+    if data.dim() == 4:
+        data = data.reshape(data.shape[0], data.shape[1], -1)
+    data = data.contiguous()
+    traj = trj if isinstance(trj, _Traj) else _Traj(trj, sample_mask)
+    uptr, real = None, 1
+    if U is not None:
+        U = _dev_tensor(U, "U")
+        if U.dim() == 2:  # (K, Nt): the first Julia column U[:, 1] is row 0
+            U = U[0]
+        real = int(not U.is_complex())
+        U = U.to(torch.float32 if real else torch.complex64).contiguous()
+        if U.numel() != traj.nt:
+            raise ArgumentError(f"U has {U.numel()} time points, trj {traj.nt}")
+        uptr = _ptr(U)
+    _test_dimension(data, traj.nt, traj.nsamp, traj.d, traj.nt, img_shape, None)
+    C = data.shape[0]
+    out = torch.empty((C,) + img_shape[::-1], dtype=torch.complex64, device=data.device)
+    _lib.call("msr_reconstruct_coilwise", _ctx(data.device.index), traj.h, len(img_shape),
+              _i64arr(img_shape), _ptr(data), C, uptr, real, int(Niter_cg), _prec(precision),
+              _ptr(out), _stream_ptr(data.device))
+    return out
+
+
+def calibration_kspace(x):
+    """kbp = fftshift(fft(fftshift(x, dims)), dims) over the image axes of
+    coil images x (C, *calib_size[::-1]) (ext/MRISubspaceReconCUDAExt/CoilMaps.jl:22-25;
+    msr_calib_kspace)."""
+    torch = _torch()
+    x = _dev_tensor(x, "x", torch.complex64).contiguous()
+    d = x.dim() - 1
+    if d not in (2, 3):
+        raise ArgumentError("x must be (C, Ny, Nx) or (C, Nz, Ny, Nx)")
+    out = torch.empty_like(x)
+    _lib.call("msr_calib_kspace", _ctx(x.device.index), d, _i64arr(tuple(x.shape[1:][::-1])),
+              x.shape[0], _ptr(x), _ptr(out), _stream_ptr(x.device))
+    return out
+
+
+def calculate_coil_maps(data, trj, img_shape, U=None, sample_mask=None, kernel_size=None,
+                        calib_size=None, eigThresh_1=0.01, eigThresh_2=0.9, nmaps=1,
+                        Niter_cg=100, verbose=False, espirit=None, precision="f64"):
+    """calculate_coil_maps (ext/MRISubspaceReconCUDAExt/CoilMaps.jl:1-36): the
+    calibration region, the coil-wise reconstruction and the calibration
+    k-space run on the device; the ESPIRiT step (MRICoilSensitivities.jl,
+    CPU) is the caller's: ``espirit(kbp, img_shape, kernel_size,
+    eigThresh_1=..., eigThresh_2=..., nmaps=...)`` receives kbp as a host
+    array (calib_size..., C) in Julia order (numpy, column-major) and returns
+    maps (img_shape..., C, nmaps).  Without ``espirit`` this raises
+    NotImplementedError after nothing has run."""
+    img_shape = tuple(int(s) for s in img_shape)
+    if espirit is None:
+        raise NotImplementedError("ESPIRiT (MRICoilSensitivities.jl) is not part of this library: "
+                                  "pass espirit=..., or use calibration_kspace(reconstruct_coilwise(...)) "
+                                  "for its input")
+    torch = _torch()
+    if calib_size is None:
+        calib_size = default_calib_size(img_shape)
+    calib_size = tuple(int(c) for c in calib_size)
+    if kernel_size is None:
+        kernel_size = tuple(6 for _ in img_shape)
+    traj_src = _dev_tensor(trj, "trj", torch.float32)
+    if traj_src.dim() == 4:
+        traj_src = traj_src.reshape(traj_src.shape[0], -1, traj_src.shape[3])
+    trj_c, mask_c = calibration_region(traj_src, img_shape, calib_size, sample_mask)
+    x = reconstruct_coilwise(data, trj_c, calib_size, U=U, sample_mask=mask_c,
+                             Niter_cg=Niter_cg, precision=precision)
+    kbp = calibration_kspace(x).cpu().numpy().T  # Julia (calib_size..., C) order
+    maps = espirit(kbp, img_shape, kernel_size, eigThresh_1=eigThresh_1,
+                   eigThresh_2=eigThresh_2, nmaps=nmaps)
+    maps = _np().asarray(maps)
+    d = len(img_shape)
+    if maps.ndim == d + 1:
+        maps = maps[..., None]
+    # [cmaps[img_idx, ic, in] for ic, in] (:33), coil index fastest
+    return [torch.from_numpy(_np().ascontiguousarray(maps[..., c, n].T).astype("complex64"))
+            .to(x.device) for n in range(maps.shape[d + 1]) for c in range(maps.shape[d])]
 
 
 def forward(x, trj, cmaps=None, U=None, sample_mask=None, out=None):

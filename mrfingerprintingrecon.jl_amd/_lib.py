@@ -81,6 +81,9 @@ SIGNATURES = {
     "msr_kernel_cache_destroy": (_I, [_P]),
     "msr_kernel_cache_get": (_I, [_P, _P, _I, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
     "msr_kernel_cache_stats": (_I, [_P, _P, _P, _P]),
+    "msr_calib_select": (_I, [_P, _I, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
+    "msr_reconstruct_coilwise": (_I, [_P, _P, _I, _P, _P, _I, _P, _I, _I, _I, _P, _P]),
+    "msr_calib_kspace": (_I, [_P, _I, _P, _I, _P, _P, _P]),
     "msr_server_create": (_I, [_P, ctypes.c_size_t, _I, _I, _P]),
     "msr_server_destroy": (_I, [_P]),
     "msr_server_submit": (_I, [_P, _P, _P, _P, _I, _D, _D, _P]),

@@ -52,6 +52,15 @@ msr_normalop* kernel_cache_get(msr_kernel_cache* c, const msr_traj* traj, int d,
 void kernel_cache_stats(msr_kernel_cache* c, int64_t* entries, size_t* bytes, int64_t* counts);
 void kernel_cache_destroy(msr_kernel_cache* c);
 msr_ctx* cache_ctx(const msr_kernel_cache* c);
+// coilmaps.hip
+void calib_select(int d, int64_t nsamp, int64_t nt, const float* trj, const uint8_t* mask,
+                  const int64_t* N, const int64_t* Nc, float* trj_out, uint8_t* mask_out,
+                  hipStream_t st);
+void calib_kspace(msr_ctx* ctx, int d, const int64_t* Nc, int ncoils, const float2* x,
+                  float2* out, hipStream_t st);
+void reconstruct_coilwise(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
+                          const void* data, int ncoils, const void* U, int u_real, int niter,
+                          int prec, float2* out, hipStream_t st);
 
 static thread_local std::string g_err;
 void set_last_error(const std::string& s) { g_err = s; }
@@ -565,5 +574,50 @@ int msr_kernel_cache_stats(msr_kernel_cache* cache, int64_t* entries, size_t* by
     return guarded([&] {
         MSR_NOTNULL(cache);
         kernel_cache_stats(cache, entries, bytes, counts);
+    });
+}
+
+// ------------------------------------------------------ coil-map calibration
+int msr_calib_select(msr_ctx* ctx, int d, int64_t nsamp, int64_t nt, const float* trj,
+                     const uint8_t* mask, const int64_t* img_shape, const int64_t* calib_size,
+                     float* trj_out, uint8_t* mask_out, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(trj);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(calib_size);
+        MSR_NOTNULL(trj_out);
+        MSR_NOTNULL(mask_out);
+        ensure_device(ctx);
+        calib_select(d, nsamp, nt, trj, mask, img_shape, calib_size, trj_out, mask_out,
+                     (hipStream_t)stream);
+    });
+}
+
+int msr_reconstruct_coilwise(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* img_shape,
+                             const void* data, int ncoils, const void* U, int u_real, int niter,
+                             int precision, void* out, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(traj);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(data);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        reconstruct_coilwise(ctx, traj, d, img_shape, data, ncoils, U, u_real, niter, precision,
+                             (float2*)out, (hipStream_t)stream);
+    });
+}
+
+int msr_calib_kspace(msr_ctx* ctx, int d, const int64_t* calib_size, int ncoils, const void* x,
+                     void* out, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(calib_size);
+        MSR_NOTNULL(x);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        calib_kspace(ctx, d, calib_size, ncoils, (const float2*)x, (float2*)out,
+                     (hipStream_t)stream);
     });
 }
