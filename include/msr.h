@@ -307,6 +307,48 @@ int msr_kernel_cache_get(msr_kernel_cache *cache, const msr_traj *traj, int d,
 int msr_kernel_cache_stats(msr_kernel_cache *cache, int64_t *entries, size_t *bytes,
                            int64_t *counts);
 
+/* --------------------------------------------------- Cartesian operator */
+/* FFTNormalOp and the Integer-trajectory calculate_backprojection
+ * (src/FFTNormalOp.jl; src/BackProjection.jl:104-156), which the reference
+ * has on the CPU only.  Samples lie on the N grid: integer index k (1..N,
+ * centred k-space) falls in FFT-ordered cell mod(k - N/2 - 1, N).  The
+ * kernel is the full (K, K) block per cell (Julia Λ[j, k, q]); the
+ * operator's kernel info (msr_normalop_info) reports P = K*K and
+ * msr_normalop_export_kernel writes the reference's A.prod!.A.Λ (K, K, Nkm)
+ * and kmask_indcs.  The operator works in every msr_normalop_* apply, msr_cg,
+ * batch and server call (not msr_normalop_apply_percoil, not kernel files). */
+
+/* Integer trajectory (d, nsamp, nt) Int32 device, e.g. traj_cartesian(...;
+ * T=Int32) (src/Trajectories.jl:15-33); mask as msr_traj_create.  Given to
+ * msr_backprojection it selects the Cartesian backprojection (no density
+ * compensation). */
+int msr_traj_create_cartesian(msr_ctx *ctx, int d, int64_t nsamp, int64_t nt, const int32_t *trj,
+                              const uint8_t *mask, void *stream, msr_traj **out);
+/* FFTNormalOp(img_shape, trj, U; cmaps, sample_mask) (src/FFTNormalOp.jl:28-31,
+ * calculate_kernel_cartesian :81-99): the kernel accumulated per cell in FP64
+ * over the cell's samples in sample order, rounded to the element type of U
+ * (ComplexF32, or Float32 with u_real); kmask = cells with Λ[1,1] != 0
+ * (:44-47).  traj from msr_traj_create_cartesian. */
+int msr_fftnormalop_create_from_traj(msr_ctx *ctx, const msr_traj *traj, int d,
+                                     const int64_t *img_shape, int K, const void *U, int u_real,
+                                     int ncoils, const void *const *cmaps, void *stream,
+                                     msr_normalop **out);
+/* FFTNormalOp(M, U; cmaps) (src/FFTNormalOp.jl:33-36, :101-111): M
+ * (img_shape..., Nt) Bool (uint8) device sampling mask in centred k-space
+ * (not modified; the reference ifftshifts it in place). */
+int msr_fftnormalop_create_from_mask(msr_ctx *ctx, int d, const int64_t *img_shape, int64_t nt,
+                                     const uint8_t *M, int K, const void *U, int u_real,
+                                     int ncoils, const void *const *cmaps, void *stream,
+                                     msr_normalop **out);
+/* FFTNormalOp(Λ; cmaps) (src/FFTNormalOp.jl:38-64): lambda (K, K, img_shape...)
+ * ComplexF32 or Float32 (lambda_real) device; cells with Λ[1,1] == 0 are
+ * dropped, the others kept in full. */
+int msr_fftnormalop_create(msr_ctx *ctx, int d, const int64_t *img_shape, int K, int lambda_real,
+                           const void *lambda, int ncoils, const void *const *cmaps, void *stream,
+                           msr_normalop **out);
+/* cartesian = 1 for an FFTNormalOp, 0 for an NFFTNormalOp. */
+int msr_normalop_kind(const msr_normalop *op, int *cartesian);
+
 /* ------------------------------------------------ coil-map calibration */
 /* The GPU part of calculate_coil_maps (ext/MRISubspaceReconCUDAExt/CoilMaps.jl:1-36):
  * calibration region, coil-wise reconstruction, calibration k-space.  The

@@ -31,7 +31,7 @@ __all__ = ["NFFTNormalOp", "calculate_backprojection", "forward", "cg", "cg_", "
            "LinearOperator", "plan_trajectory", "ArgumentError", "ErrorException", "synth",
            "kernel_key", "save_kernel", "load_kernel", "kernel_file_info", "KernelCache",
            "reconstruct_coilwise", "calibration_region", "calibration_kspace",
-           "calculate_coil_maps", "default_calib_size"]
+           "calculate_coil_maps", "default_calib_size", "FFTNormalOp"]
 
 _ctxs = {}
 
@@ -100,7 +100,11 @@ class _Traj:
 
     def __init__(self, trj, sample_mask=None):
         torch = _torch()
-        trj = _dev_tensor(trj, "trj", torch.float32)
+        # integer k-space indices select the Cartesian methods (Integer trj
+        # dispatch of src/BackProjection.jl:104-156 and FFTNormalOp)
+        self.integer = isinstance(trj, torch.Tensor) and not trj.dtype.is_floating_point \
+            and not trj.dtype.is_complex
+        trj = _dev_tensor(trj, "trj", torch.int32 if self.integer else torch.float32)
         if trj.dim() == 4:  # (Nt, Ncyc, Nr, d): 4D wrapper (src/NFFTNormalOp.jl:45-49)
             trj = trj.reshape(trj.shape[0], -1, trj.shape[3])
         if trj.dim() != 3:
@@ -118,7 +122,8 @@ class _Traj:
             mptr = _ptr(m)
         self._trj = trj
         h = ctypes.c_void_p()
-        _lib.call("msr_traj_create", _ctx(self.device.index), self.d, self.nsamp, self.nt,
+        _lib.call("msr_traj_create_cartesian" if self.integer else "msr_traj_create",
+                  _ctx(self.device.index), self.d, self.nsamp, self.nt,
                   _ptr(trj), mptr, _stream_ptr(self.device), ctypes.byref(h))
         self.h = h
         n = ctypes.c_int64()
@@ -151,13 +156,18 @@ class _NFFTNormalOp:
         self.P = P.value
         self.nkm = nkm.value
         self.lambda_real = bool(lr.value)
+        cart = ctypes.c_int()
+        _lib.call("msr_normalop_kind", handle, ctypes.byref(cart))
+        self.cartesian = bool(cart.value)
         self._lam = None
         self._km = None
 
     def _export(self):
         torch = _torch()
         dt = torch.float32 if self.lambda_real else torch.complex64
-        lam = torch.empty((self.nkm, self.P), dtype=dt, device=self.device)
+        # packed (Nkm, P) for Julia (P, Nkm); Cartesian: (Nkm, K, K) for Julia (K, K, Nkm)
+        shape = (self.nkm, self.Ncoeff, self.Ncoeff) if self.cartesian else (self.nkm, self.P)
+        lam = torch.empty(shape, dtype=dt, device=self.device)
         km = torch.empty((self.nkm,), dtype=torch.int64, device=self.device)
         _lib.call("msr_normalop_export_kernel", self.h, _ptr(lam), _ptr(km),
                   _stream_ptr(self.device))
@@ -307,6 +317,59 @@ def NFFTNormalOp(img_shape, a, b, cmaps=None, sample_mask=None, verbose=False):
               _i64arr(img_shape), K, _ptr(U), int(real), len(cl), carr,
               _stream_ptr(traj.device), ctypes.byref(h))
     return LinearOperator(_NFFTNormalOp(h, img_shape, traj.device, cl))
+
+
+def FFTNormalOp(a, b=None, c=None, cmaps=None, sample_mask=None, eltype_x=None):
+    """FFTNormalOp(img_shape, trj, U; cmaps, sample_mask), FFTNormalOp(M, U; cmaps)
+    or FFTNormalOp(Λ; cmaps) (src/FFTNormalOp.jl:28-64), the Cartesian normal
+    operator on the device:
+      img_shape, trj (Nt, Nsamp, d) integer k-space indices (1..N centred), U (K, Nt);
+      M (Nt, *N[::-1]) Bool sampling mask (Julia (N..., Nt)), U (K, Nt);
+      Λ (*N[::-1], K, K) (Julia (K, K, N...)) ComplexF32 or Float32.
+    A real U gives a real kernel.  ``eltype_x`` is accepted for signature
+    parity; vectors are ComplexF32 (ComplexF64 through mul_f64_ / cg)."""
+    torch = _torch()
+    cl, carr = _cmaps_arg(cmaps)
+    h = ctypes.c_void_p()
+    if isinstance(a, (tuple, list)):  # (img_shape, trj, U)
+        img_shape = tuple(int(v) for v in a)
+        traj = b if isinstance(b, _Traj) else _Traj(b, sample_mask)
+        if not traj.integer:
+            raise ArgumentError("FFTNormalOp takes integer k-space indices (use NFFTNormalOp "
+                                "for a non-Cartesian trajectory)")
+        U, real, K = _basis(c, traj)
+        dev = traj.device
+        _lib.call("msr_fftnormalop_create_from_traj", _ctx(dev.index), traj.h, len(img_shape),
+                  _i64arr(img_shape), K, _ptr(U), int(real), len(cl), carr, _stream_ptr(dev),
+                  ctypes.byref(h))
+    elif b is not None:  # (M, U)
+        M = _dev_tensor(a, "M")
+        nt = M.shape[0]
+        img_shape = tuple(int(v) for v in M.shape[1:][::-1])
+        M = M.to(torch.uint8).contiguous()
+        U = _dev_tensor(b, "U")
+        if U.dim() == 1:
+            U = U.reshape(1, -1)
+        real = not U.is_complex()
+        U = U.to(torch.float32 if real else torch.complex64).contiguous()
+        if U.shape[1] != nt:
+            raise ArgumentError(f"U has {U.shape[1]} time points, M {nt}")
+        dev = M.device
+        _lib.call("msr_fftnormalop_create_from_mask", _ctx(dev.index), len(img_shape),
+                  _i64arr(img_shape), nt, _ptr(M), U.shape[0], _ptr(U), int(real), len(cl), carr,
+                  _stream_ptr(dev), ctypes.byref(h))
+    else:  # (Λ)
+        lam = _dev_tensor(a, "Λ")
+        K = lam.shape[-1]
+        if lam.shape[-2] != K:
+            raise ArgumentError("Λ must be (*N[::-1], K, K)")
+        img_shape = tuple(int(v) for v in lam.shape[:-2][::-1])
+        real = not lam.is_complex()
+        lam = lam.to(torch.float32 if real else torch.complex64).contiguous()
+        dev = lam.device
+        _lib.call("msr_fftnormalop_create", _ctx(dev.index), len(img_shape), _i64arr(img_shape),
+                  K, int(real), _ptr(lam), len(cl), carr, _stream_ptr(dev), ctypes.byref(h))
+    return LinearOperator(_NFFTNormalOp(h, img_shape, dev, cl))
 
 
 def NFFTNormalOp_shared(A, cmaps=None):
@@ -483,6 +546,9 @@ def calculate_backprojection(data, trj, cmaps_or_img_shape, U=None, sample_mask=
     if data.dim() == 4:  # (C, Nt, Ncyc, Nr)
         data = data.reshape(data.shape[0], data.shape[1], -1)
     traj = trj if isinstance(trj, _Traj) else _Traj(trj, sample_mask)
+    if traj.integer and not (isinstance(density_compensation, str) and density_compensation == "none"):
+        raise ArgumentError("the Cartesian (integer trajectory) backprojection takes no "
+                            "density_compensation (src/BackProjection.jl:104,131)")
     per_coil = isinstance(cmaps_or_img_shape, (tuple, list)) and len(cmaps_or_img_shape) > 0 \
         and not hasattr(cmaps_or_img_shape[0], "shape")
     if per_coil:

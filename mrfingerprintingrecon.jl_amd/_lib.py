@@ -81,6 +81,11 @@ SIGNATURES = {
     "msr_kernel_cache_destroy": (_I, [_P]),
     "msr_kernel_cache_get": (_I, [_P, _P, _I, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
     "msr_kernel_cache_stats": (_I, [_P, _P, _P, _P]),
+    "msr_traj_create_cartesian": (_I, [_P, _I, _I64, _I64, _P, _P, _P, _P]),
+    "msr_fftnormalop_create_from_traj": (_I, [_P, _P, _I, _P, _I, _P, _I, _I, _P, _P, _P]),
+    "msr_fftnormalop_create_from_mask": (_I, [_P, _I, _P, _I64, _P, _I, _P, _I, _I, _P, _P, _P]),
+    "msr_fftnormalop_create": (_I, [_P, _I, _P, _I, _I, _P, _I, _P, _P, _P]),
+    "msr_normalop_kind": (_I, [_P, _P]),
     "msr_calib_select": (_I, [_P, _I, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     "msr_reconstruct_coilwise": (_I, [_P, _P, _I, _P, _P, _I, _P, _I, _I, _I, _P, _P]),
     "msr_calib_kspace": (_I, [_P, _I, _P, _I, _P, _P, _P]),

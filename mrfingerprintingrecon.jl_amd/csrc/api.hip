@@ -52,6 +52,12 @@ msr_normalop* kernel_cache_get(msr_kernel_cache* c, const msr_traj* traj, int d,
 void kernel_cache_stats(msr_kernel_cache* c, int64_t* entries, size_t* bytes, int64_t* counts);
 void kernel_cache_destroy(msr_kernel_cache* c);
 msr_ctx* cache_ctx(const msr_kernel_cache* c);
+// cartesian.hip
+msr_traj* cart_traj_create(msr_ctx* ctx, int d, int64_t nsamp, int64_t nt, const int32_t* trj,
+                           const uint8_t* mask, hipStream_t st);
+void cart_build_kernel(msr_normalop* op, const msr_traj* traj, const uint8_t* M, int64_t nt_mask,
+                       const void* U, int u_real, hipStream_t st);
+void cart_load_dense(msr_normalop* op, const void* lambda, hipStream_t st);
 // coilmaps.hip
 void calib_select(int d, int64_t nsamp, int64_t nt, const float* trj, const uint8_t* mask,
                   const int64_t* N, const int64_t* Nc, float* trj_out, uint8_t* mask_out,
@@ -426,7 +432,7 @@ int msr_normalop_create_shared(msr_ctx* ctx, const msr_normalop* proto, int ncoi
         ensure_device(ctx);
         hipStream_t st = (hipStream_t)stream;
         *out = op_on_kernel(ctx, proto->d, proto->N, proto->K, proto->lam_real, proto->nkm,
-                            proto->lam, proto->kmask, ncoils, cmaps, st);
+                            proto->lam, proto->kmask, ncoils, cmaps, st, proto->cart);
     });
 }
 
@@ -619,5 +625,85 @@ int msr_calib_kspace(msr_ctx* ctx, int d, const int64_t* calib_size, int ncoils,
         ensure_device(ctx);
         calib_kspace(ctx, d, calib_size, ncoils, (const float2*)x, (float2*)out,
                      (hipStream_t)stream);
+    });
+}
+
+// ------------------------------------------------------ Cartesian operator
+int msr_traj_create_cartesian(msr_ctx* ctx, int d, int64_t nsamp, int64_t nt, const int32_t* trj,
+                              const uint8_t* mask, void* stream, msr_traj** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        *out = cart_traj_create(ctx, d, nsamp, nt, trj, mask, (hipStream_t)stream);
+    });
+}
+
+int msr_fftnormalop_create_from_traj(msr_ctx* ctx, const msr_traj* traj, int d,
+                                     const int64_t* img_shape, int K, const void* U, int u_real,
+                                     int ncoils, const void* const* cmaps, void* stream,
+                                     msr_normalop** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(traj);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(U);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        hipStream_t st = (hipStream_t)stream;
+        std::unique_ptr<msr_normalop> op(op_new(ctx, d, img_shape, K, u_real ? 1 : 0, 1));
+        op_set_cmaps(op.get(), ncoils, cmaps, st);
+        cart_build_kernel(op.get(), traj, nullptr, 0, U, u_real, st);
+        op_alloc_scratch(op.get());
+        MSR_HIP(hipStreamSynchronize(st));
+        *out = op.release();
+    });
+}
+
+int msr_fftnormalop_create_from_mask(msr_ctx* ctx, int d, const int64_t* img_shape, int64_t nt,
+                                     const uint8_t* M, int K, const void* U, int u_real,
+                                     int ncoils, const void* const* cmaps, void* stream,
+                                     msr_normalop** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(M);
+        MSR_NOTNULL(U);
+        MSR_NOTNULL(out);
+        MSR_REQUIRE(nt >= 1, MSR_ERR_ARGUMENT, "Nt = %lld", (long long)nt);
+        ensure_device(ctx);
+        hipStream_t st = (hipStream_t)stream;
+        std::unique_ptr<msr_normalop> op(op_new(ctx, d, img_shape, K, u_real ? 1 : 0, 1));
+        op_set_cmaps(op.get(), ncoils, cmaps, st);
+        cart_build_kernel(op.get(), nullptr, M, nt, U, u_real, st);
+        op_alloc_scratch(op.get());
+        MSR_HIP(hipStreamSynchronize(st));
+        *out = op.release();
+    });
+}
+
+int msr_fftnormalop_create(msr_ctx* ctx, int d, const int64_t* img_shape, int K, int lambda_real,
+                           const void* lambda, int ncoils, const void* const* cmaps, void* stream,
+                           msr_normalop** out) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(img_shape);
+        MSR_NOTNULL(lambda);
+        MSR_NOTNULL(out);
+        ensure_device(ctx);
+        hipStream_t st = (hipStream_t)stream;
+        std::unique_ptr<msr_normalop> op(op_new(ctx, d, img_shape, K, lambda_real ? 1 : 0, 1));
+        op_set_cmaps(op.get(), ncoils, cmaps, st);
+        cart_load_dense(op.get(), lambda, st);
+        op_alloc_scratch(op.get());
+        MSR_HIP(hipStreamSynchronize(st));
+        *out = op.release();
+    });
+}
+
+int msr_normalop_kind(const msr_normalop* op, int* cartesian) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        if (cartesian) *cartesian = op->cart;
     });
 }

@@ -34,9 +34,22 @@ __global__ void combine_kernel(const float2* __restrict__ img, CmapPtrs maps, in
     }
 }
 
+void cart_backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
+                         const void* data, int ncoils, int K, const void* U, int u_real,
+                         const void* const* cmaps, void* out, hipStream_t st);  // cartesian.hip
+
 void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, const void* data,
                     int ncoils, int K, const void* U, int u_real, const void* const* cmaps,
                     int dcf_mode, const float* dcf, void* out, hipStream_t st, bool accumulate) {
+    if (traj->integer) {  // the Integer-trajectory method (src/BackProjection.jl:104-156)
+        MSR_REQUIRE(dcf_mode == MSR_DCF_NONE && !accumulate, MSR_ERR_ARGUMENT,
+                    "an integer (Cartesian) trajectory takes no density compensation");
+        MSR_REQUIRE(ncoils >= 1 && K >= 1, MSR_ERR_ARGUMENT, "ncoils = %d, K = %d", ncoils, K);
+        MSR_REQUIRE(U != nullptr || K == traj->nt, MSR_ERR_ARGUMENT,
+                    "U = I(Nt) needs K = Nt (%lld), got %d", (long long)traj->nt, K);
+        cart_backprojection(ctx, traj, d, N, data, ncoils, K, U, u_real, cmaps, out, st);
+        return;
+    }
     MSR_REQUIRE(traj->d == d, MSR_ERR_ARGUMENT,
                 "`cmaps` contains %d image dimensions, yet the 1st dimension of trj is of "
                 "length %d", d, traj->d);
@@ -191,6 +204,8 @@ void backprojection_channelwise(msr_ctx* ctx, const msr_traj* traj, int d, const
                                 const void* data, int ncoils, int K, const void* U, int u_real,
                                 const void* const* cmaps, int dcf_mode, const float* dcf, void* out,
                                 hipStream_t st) {
+    MSR_REQUIRE(!traj->integer, MSR_ERR_ARGUMENT,
+                "the channel-wise backprojection takes a non-Cartesian trajectory");
     MSR_REQUIRE(U != nullptr, MSR_ERR_ARGUMENT, "the channel-wise backprojection needs a basis U");
     MSR_REQUIRE(cmaps != nullptr, MSR_ERR_ARGUMENT, "the channel-wise backprojection needs coil maps");
     MSR_REQUIRE(ncoils >= 1 && K >= 1, MSR_ERR_ARGUMENT, "ncoils = %d, K = %d", ncoils, K);

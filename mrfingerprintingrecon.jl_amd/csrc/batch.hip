@@ -40,7 +40,7 @@ msr_batch* batch_create(msr_ctx* ctx, int nops, msr_normalop* const* ops) {
         MSR_REQUIRE(seen.insert(ops[j]).second, MSR_ERR_ARGUMENT,
                     "ops[%d] appears twice (each operator owns one scratch and CG state)", j);
         bt->ops.push_back(ops[j]);
-        (ops[j]->d == 2 ? bt->j2d : bt->j3d).push_back(j);
+        (ops[j]->d == 2 && !ops[j]->cart ? bt->j2d : bt->j3d).push_back(j);
     }
     // launch groups of the 2D passes
     std::map<int, int> ia;

@@ -114,6 +114,7 @@ interp_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
 void forward_model(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, const void* x,
                    int K, const void* U, int u_real, int ncoils, const void* const* cmaps, void* out,
                    hipStream_t st) {
+    MSR_REQUIRE(!traj->integer, MSR_ERR_ARGUMENT, "the forward model takes a non-Cartesian trajectory");
     MSR_REQUIRE(traj->d == d, MSR_ERR_ARGUMENT, "trajectory has %d dimensions, image %d", traj->d, d);
     MSR_REQUIRE(K >= 1, MSR_ERR_ARGUMENT, "K = %d", K);
     MSR_REQUIRE(U != nullptr || K == traj->nt, MSR_ERR_ARGUMENT,

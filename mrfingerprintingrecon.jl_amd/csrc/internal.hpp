@@ -77,6 +77,15 @@ struct SpreadPlan {
     DevBuf split_nitems;        // (nsplit) int32
 };
 
+// Cell sort of an integer (Cartesian) trajectory's samples for an N grid
+// (cartesian.hip): masked samples ordered by grid cell, ties in sample order.
+struct CartPlan {
+    int64_t n[3] = {1, 1, 1};
+    int64_t G = 1;
+    DevBuf perm;                // (M) int32: masked sample indices sorted by cell
+    DevBuf offs;                // (G + 1) int32
+};
+
 // CG scalar state kept on the device (FP64).
 struct CgScalars {
     double res, prev, tol;
@@ -98,6 +107,8 @@ struct msr_traj {
     std::vector<int64_t> nsamp_t;     // host copy of the per-frame counts
     mutable std::map<std::array<int64_t, 3>, std::unique_ptr<msr::SpreadPlan>> plans;
     mutable msr::DevBuf bp_work;      // backprojection workspace (grow-only)
+    bool integer = false;             // created from integer (Cartesian) k-space indices
+    mutable std::map<std::array<int64_t, 3>, std::unique_ptr<msr::CartPlan>> cart_plans;
 };
 
 struct msr_normalop {
@@ -107,6 +118,7 @@ struct msr_normalop {
     int64_t V = 1;                    // voxels
     int K = 1, P = 1, C = 1;
     int lam_real = 0;
+    int cart = 0;                     // Cartesian FFTNormalOp: N grid, full K x K kernel per cell
     bool identity_cmaps = true;
     int64_t nkm = 0;
     msr::DevBuf cmaps;                // (C, V) complex

@@ -212,6 +212,8 @@ struct File {
 };
 
 void kernel_save(const msr_normalop* op, const char* path, const uint64_t* key, hipStream_t st) {
+    MSR_REQUIRE(!op->cart, MSR_ERR_UNSUPPORTED,
+                "kernel files hold the non-Cartesian (packed Toeplitz) kernel only");
     const size_t es = op->lam_real ? sizeof(float) : sizeof(float2);
     const size_t kb = sizeof(int64_t) * (size_t)op->nkm, lb = es * (size_t)op->P * op->nkm;
     std::vector<uint8_t> payload(kb + lb);
@@ -354,8 +356,8 @@ msr_normalop* kernel_load(msr_ctx* ctx, const char* path, const uint64_t* expect
 msr_normalop* op_on_kernel(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real,
                            int64_t nkm, const std::shared_ptr<DevBuf>& lam,
                            const std::shared_ptr<DevBuf>& kmask, int ncoils,
-                           const void* const* cmaps, hipStream_t st) {
-    std::unique_ptr<msr_normalop> op(op_new(ctx, d, N, K, lam_real));
+                           const void* const* cmaps, hipStream_t st, int cart) {
+    std::unique_ptr<msr_normalop> op(op_new(ctx, d, N, K, lam_real, cart));
     op->nkm = nkm;
     op->lam = lam;
     op->kmask = kmask;

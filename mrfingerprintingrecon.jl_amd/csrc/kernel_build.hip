@@ -152,6 +152,8 @@ void build_kmask(const msr_traj* traj, const int64_t* n2, DevBuf& mask, DevBuf& 
 
 void op_build_from_traj(msr_normalop* op, const msr_traj* traj, const void* U, int u_real,
                         hipStream_t st) {
+    MSR_REQUIRE(!traj->integer, MSR_ERR_ARGUMENT,
+                "NFFTNormalOp takes a non-Cartesian trajectory (FFTNormalOp takes integer ones)");
     msr_ctx* ctx = op->ctx;
     const int d = op->d;
     const int P = op->P;

@@ -32,7 +32,10 @@ void fill_floats(void* p, int64_t n, float v, hipStream_t st) {
 }
 
 // linear 0-based grid index q (x fastest) -> dense kernel offset of entry p
-__device__ __forceinline__ int64_t dense_offset(int d, const int64_t* n2, int P, int64_t q, int p) {
+// (cell_major: the Cartesian kernel, lam[q * P + p])
+__device__ __forceinline__ int64_t dense_offset(int d, const int64_t* n2, int P, int64_t q, int p,
+                                                int cell_major) {
+    if (cell_major) return q * P + p;
     if (d == 2) {
         const int64_t qx = q % n2[0], qy = q / n2[0];
         return ((qx * P) + p) * n2[1] + qy;
@@ -46,6 +49,7 @@ __device__ __forceinline__ int64_t dense_offset(int d, const int64_t* n2, int P,
 
 struct GridDims {
     int64_t n[3];
+    int cell_major = 0;
 };
 
 template <class LT>
@@ -62,7 +66,7 @@ __global__ void scatter_kernel(const LT* __restrict__ packed, const int64_t* __r
             *bad = 1;
             continue;
         }
-        dense[dense_offset(d, g.n, P, q1 - 1, p)] = packed[e];
+        dense[dense_offset(d, g.n, P, q1 - 1, p, g.cell_major)] = packed[e];
     }
 }
 
@@ -73,7 +77,7 @@ __global__ void gather_kernel(const LT* __restrict__ dense, const int64_t* __res
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = e / P;
         const int p = (int)(e - i * P);
-        packed[e] = dense[dense_offset(d, g.n, P, kmask[i] - 1, p)];
+        packed[e] = dense[dense_offset(d, g.n, P, kmask[i] - 1, p, g.cell_major)];
     }
 }
 
@@ -85,7 +89,7 @@ static int env_int(const char* name, int dflt) {
 
 void op_alloc_scratch(msr_normalop* op) {
     const int64_t ny = op->N[1];
-    if (op->d == 2) {
+    if (op->d == 2 && !op->cart) {
         const int tiles = tiles2d((int)ny, 0);
         int G = (int)ceil_div(512, (int64_t)tiles * op->K);
         G = std::max(1, std::min(G, op->C));
@@ -106,14 +110,18 @@ void op_alloc_scratch(msr_normalop* op) {
 void op_ensure_scratch(msr_normalop* op, bool f64) {
     // the twiddle tables of the apply are device allocations too: create them
     // here, outside any stream capture
+    const int pad = op->cart ? 1 : 2;
     for (int a = 0; a < op->d; ++a) {
-        if (f64) (void)twiddle_table64(op->ctx, (int)(2 * op->N[a]));
-        else (void)twiddle_table(op->ctx, (int)(2 * op->N[a]));
+        if (f64) (void)twiddle_table64(op->ctx, (int)(pad * op->N[a]));
+        else (void)twiddle_table(op->ctx, (int)(pad * op->N[a]));
     }
     const size_t es = f64 ? sizeof(double2) : sizeof(float2);
     if (op->scratch_es >= es) return;
     const int64_t nx = op->N[0], ny = op->N[1], nz = op->N[2];
-    if (op->d == 2) {
+    if (op->cart) {  // two ping-pong grids (C, K, N...)
+        op->T.alloc(es * (size_t)op->C * op->K * op->V);
+        op->T2.alloc(es * (size_t)op->C * op->K * op->V);
+    } else if (op->d == 2) {
         op->T.alloc(es * (size_t)op->C * op->K * (2 * nx) * ny);
     } else {
         op->T.alloc(es * (size_t)op->C * op->K * nz * ny * 2 * nx);
@@ -121,6 +129,14 @@ void op_ensure_scratch(msr_normalop* op, bool f64) {
     }
     op->partial.alloc(es * (size_t)op->G * op->K * op->V);
     op->scratch_es = es;
+}
+
+static void check_shape_cart(int d, const int64_t* N) {
+    MSR_REQUIRE(d == 2 || d == 3, MSR_ERR_ARGUMENT, "image dimension d = %d, expected 2 or 3", d);
+    for (int i = 0; i < d; ++i)
+        MSR_REQUIRE(N[i] >= 1 && fft_len_supported(N[i]), MSR_ERR_UNSUPPORTED,
+                    "Cartesian grid size N[%d] = %lld has no FFT kernel in this build", i,
+                    (long long)N[i]);
 }
 
 static void check_shape(int d, const int64_t* N) {
@@ -157,7 +173,8 @@ void op_set_cmaps(msr_normalop* op, int ncoils, const void* const* cmaps, hipStr
 }
 
 GridDims op_grid(const msr_normalop* op) {
-    GridDims g{{2 * op->N[0], 2 * op->N[1], op->d == 3 ? 2 * op->N[2] : 1}};
+    const int pad = op->cart ? 1 : 2;
+    GridDims g{{pad * op->N[0], pad * op->N[1], op->d == 3 ? pad * op->N[2] : 1}, op->cart};
     return g;
 }
 
@@ -201,8 +218,9 @@ void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, 
                 (long long)G);
 }
 
-msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real) {
-    check_shape(d, N);
+msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real, int cart) {
+    if (cart) check_shape_cart(d, N);
+    else check_shape(d, N);
     MSR_REQUIRE(K >= 1 && K <= 8, MSR_ERR_UNSUPPORTED, "K = %d coefficients, supported 1..8", K);
     auto op = std::make_unique<msr_normalop>();
     op->ctx = ctx;
@@ -211,8 +229,9 @@ msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real)
     for (int i = 0; i < 3; ++i) op->N[i] = i < d ? N[i] : 1;
     for (int i = 0; i < d; ++i) op->V *= N[i];
     op->K = K;
-    op->P = K * (K + 1) / 2;
+    op->P = cart ? K * K : K * (K + 1) / 2;
     op->lam_real = lam_real;
+    op->cart = cart ? 1 : 0;
     return op.release();
 }
 
@@ -249,7 +268,9 @@ double op_scale(const msr_normalop* op) { return 1.0 / (double)op_grid_cells(op)
 void op_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st, const int* skip) {
     MSR_REQUIRE(op->scratch_es >= (f64 ? sizeof(double2) : sizeof(float2)), MSR_ERR_ERROR,
                 "apply scratch not allocated for this precision");
-    if (op->d == 2) {
+    if (op->cart) {
+        launch_cart_passes(op, b, f64, st, skip);
+    } else if (op->d == 2) {
         Apply2DArgs a = args2d(op, f64);
         a.b = b;
         a.skip = skip;
@@ -277,6 +298,8 @@ void op_apply_f64(msr_normalop* op, double2* x, const double2* b, double2 alpha,
 // The reference GPU order: one coil at a time, kernel re-read per coil.
 void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
                       bool beta_zero, hipStream_t st) {
+    MSR_REQUIRE(!op->cart, MSR_ERR_UNSUPPORTED,
+                "the per-coil apply order exists for the non-Cartesian operator only");
     launch_scale<float2>(x, op->V * op->K, beta, beta_zero, st);
     for (int c = 0; c < op->C; ++c) {
         if (op->d == 2) {
@@ -326,7 +349,8 @@ size_t op_footprint(const msr_normalop* op, int prec) {
     const size_t vs = v64 ? sizeof(double2) : sizeof(float2);  // CG vectors
     const int64_t nx = op->N[0], ny = op->N[1], nz = op->N[2];
     size_t scratch = es * (size_t)op->G * op->K * op->V;
-    if (op->d == 2) scratch += es * (size_t)op->C * op->K * (2 * nx) * ny;
+    if (op->cart) scratch += 2 * es * (size_t)op->C * op->K * op->V;
+    else if (op->d == 2) scratch += es * (size_t)op->C * op->K * (2 * nx) * ny;
     else scratch += es * (size_t)op->C * op->K * nz * ny * 2 * nx * 3;  // T1 + T2
     size_t cg = vs * n * 2 + es * n;                                   // r, u, c
     if (v64) cg += sizeof(double2) * n;                                // x in FP64
@@ -489,7 +513,9 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
         MSR_HIP(hipGraphLaunch(op->graph, st));
     } else if (mode == MSR_CG_HOST) {
         // IterativeSolvers-style: the step length and the residual norm are
-        // read back to the host every iteration; applies run coil by coil.
+        // read back to the host every iteration; applies run coil by coil
+        // (the Cartesian operator, which the reference has on the CPU only,
+        // applies all coils at once).
         if constexpr (std::is_same<AT, float2>::value) {
             const int64_t n = op->V * op->K;
             double* part = op->cg_part.as<double>();
@@ -500,8 +526,11 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
                 MSR_HIP(hipStreamSynchronize(st));
                 if (hs.done) break;
                 launch_cg_u(v, n, S, st);
-                op_apply_percoil(op, v.c, v.ua, make_float2(1.f, 0.f), make_float2(0.f, 0.f),
-                                 true, st);
+                if (op->cart)
+                    op_apply(op, v.c, v.ua, make_float2(1.f, 0.f), make_float2(0.f, 0.f), true, st);
+                else
+                    op_apply_percoil(op, v.c, v.ua, make_float2(1.f, 0.f), make_float2(0.f, 0.f),
+                                     true, st);
                 launch_cg_reduce_dot(v.c, 1, n, v, 1.0, part, S, st);
                 MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
                 MSR_HIP(hipStreamSynchronize(st));

@@ -9,7 +9,10 @@
 
 namespace msr {
 
-msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real);
+msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real, int cart = 0);
+// Cartesian operator passes (cartesian.hip): partial <- unscaled sum_c conj(S_c)
+// IFFT(Λ . FFT(S_c b)) on the N grid
+void launch_cart_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st, const int* skip);
 void op_set_cmaps(msr_normalop* op, int ncoils, const void* const* cmaps, hipStream_t st);
 void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, int64_t nkm,
                     hipStream_t st);
@@ -44,6 +47,6 @@ size_t op_kernel_bytes(const msr_normalop* op);
 msr_normalop* op_on_kernel(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real,
                            int64_t nkm, const std::shared_ptr<DevBuf>& lam,
                            const std::shared_ptr<DevBuf>& kmask, int ncoils,
-                           const void* const* cmaps, hipStream_t st);
+                           const void* const* cmaps, hipStream_t st, int cart = 0);
 
 }  // namespace msr
