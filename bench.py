@@ -391,7 +391,8 @@ def batched_leg(msr, njobs=64, iters=CG_ITERS, seed=1004):
     return res
 
 
-def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, niter_step=5):
+def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, niter_step=5,
+               max_active=64):
     """Served stream: Poisson arrivals (rate jobs/s) of 2D jobs drawn as in the
     batched leg over a set of protocols whose kernels stay resident; each job
     brings its own coil maps (an operator sharing its protocol's kernel) and
@@ -431,7 +432,7 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
                          b=torch.empty((K, N, N), dtype=torch.complex64, device=dev),
                          x=torch.empty(K * N * N, dtype=torch.complex64, device=dev),
                          x1=torch.empty(K * N * N, dtype=torch.complex64, device=dev)))
-    srv = msr.NFFTServer(max_active=64, precision=PRECISION)
+    srv = msr.NFFTServer(max_active=max_active, precision=PRECISION)
 
     def bp(j):
         msr.calculate_backprojection(j["data"], j["pr"]["traj"], j["cml"], U=j["pr"]["U"], out=j["b"])
@@ -490,6 +491,7 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
                              "latency_ms": {"p50": ms(percentile(lat_fifo, 50)),
                                             "p99": ms(percentile(lat_fifo, 99))}},
            "arrival_span_s": round(jobs[-1]["t"], 3),
+           "max_active": max_active, "iterations_per_step": niter_step,
            "max_rel_diff_served_vs_single": err}
     log(f"served: {res}")
     del srv, jobs, protos
