@@ -23,9 +23,15 @@ template <> struct TileShape<2> {
     static constexpr int X = 32, Y = 32, Z = 1;
     static constexpr int CHG = 8;   // channels per workgroup (one wavefront each), ComplexF32
 };
+#ifndef MSR_T3X
+#define MSR_T3X 16
+#define MSR_T3Y 8
+#define MSR_T3Z 8
+#define MSR_T3CHG 2
+#endif
 template <> struct TileShape<3> {
-    static constexpr int X = 16, Y = 8, Z = 8;
-    static constexpr int CHG = 2;
+    static constexpr int X = MSR_T3X, Y = MSR_T3Y, Z = MSR_T3Z;
+    static constexpr int CHG = MSR_T3CHG;
 };
 // channels per workgroup for the accumulation type AT (LDS: ComplexF64 tiles
 // take twice the space)
@@ -209,19 +215,27 @@ __device__ __forceinline__ AT channel_value(const ValueSrc& s, const float* kc, 
 }
 
 // ------------------------------------------------------------------ spread
-// One wavefront per value channel; the waves of a workgroup walk the tile's
-// points in sorted order, 64 points per chunk, with the chunk's kernel weights
-// and channel values staged in LDS.  Lane (ox, oy) owns kernel cell
-// (c + ox, c + oy) of the current point (and loops over the 8 z offsets in
-// 3D), so every LDS cell of a channel is updated by one wave in program order:
-// the accumulation order is fixed and the result is bit-reproducible.
+// 2D: one wavefront per value channel; 3D: eight per channel, wave w owning
+// the padded tile's z planes with lz = w (mod 8), so that of the 8 planes a
+// point touches each wave updates exactly one.  The waves of a workgroup walk
+// the tile's points in sorted order, 64 points per chunk, with the chunk's
+// kernel weights and channel values staged in LDS.  Lane (ox, oy) owns kernel
+// cell (c + ox, c + oy) of the current point, so every LDS cell is updated by
+// one wave in program order: the accumulation order is fixed and the result is
+// bit-reproducible.
 constexpr int CHUNK = 64;
+template <int D> constexpr int waves_per_channel() { return D == 3 ? 8 : 1; }
 
+// 2D, and 3D with a slab: every padded tile goes to slab[ch - chan0][bin][cell]
+// with plain stores (tiles without points write nothing), summed afterwards by
+// a gather over the covering tiles.  3D without a slab (slab == nullptr): the
+// tile is added into the grid with atomics.
 template <int D, class AT>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 spread_tiles_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
                     const int* __restrict__ perm, const int* __restrict__ offs, PlanGeom g,
-                    ValueSrc src, int nch, AT* __restrict__ slab, AT* __restrict__ grid) {
+                    ValueSrc src, int nch, int chan0, AT* __restrict__ slab,
+                    AT* __restrict__ grid) {
     using TS = TileShape<D>;
     using RT = real_t<AT>;
     constexpr int H = SPREAD_HALO;
@@ -233,14 +247,17 @@ spread_tiles_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
     __shared__ int s_base[CHUNK][D];
     __shared__ AT s_val[CHG][CHUNK];
     const int bin = blockIdx.x;
-    const int ch0 = blockIdx.y * CHG;
+    const int ch0 = chan0 + blockIdx.y * CHG;
+    if (D == 3 && slab && offs[bin] == offs[bin + 1]) return;  // empty tile: nothing to store
     const int bx = bin % g.nb[0];
     const int by = (bin / g.nb[0]) % g.nb[1];
     const int bz = D == 3 ? bin / (g.nb[0] * g.nb[1]) : 0;
     const int64_t t0[3] = {(int64_t)bx * TS::X, (int64_t)by * TS::Y, (int64_t)bz * TS::Z};
     for (int e = threadIdx.x; e < CHG * CELLS; e += blockDim.x) tile[e] = cz<AT>();
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;  // wave == local channel
+    const int lane = threadIdx.x & 63;
+    const int wave = (threadIdx.x >> 6) / waves_per_channel<D>();  // local channel
+    const int pz = (threadIdx.x >> 6) & 7;                          // 3D: owned plane residue
     const int ox = lane & 7, oy = lane >> 3;
     const int p0 = offs[bin], p1 = offs[bin + 1];
     for (int c0 = p0; c0 < p1; c0 += CHUNK) {
@@ -275,26 +292,24 @@ spread_tiles_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
                     cell->x += wxy * v.x;
                     cell->y += wxy * v.y;
                 } else {
-                    const int lz0 = s_base[pl][D - 1];
-#pragma unroll
-                    for (int oz = 0; oz < 8; ++oz) {
-                        const RT w = wxy * s_w[pl][D - 1][oz];
-                        AT* cell = tw + ((lz0 + oz) * LY + ly) * LX + lx;
-                        cell->x += w * v.x;
-                        cell->y += w * v.y;
-                    }
+                    const int lz0 = s_base[pl][D - 1];  // >= 0
+                    const int oz = (pz - lz0) & 7;      // the offset landing on an owned plane
+                    const RT w = wxy * s_w[pl][D - 1][oz];
+                    AT* cell = tw + ((lz0 + oz) * LY + ly) * LX + lx;
+                    cell->x += w * v.x;
+                    cell->y += w * v.y;
                 }
             }
         }
     }
     __syncthreads();
     const int chg = min(CHG, nch - ch0);
-    if (D == 2) {
+    if (D == 2 || slab) {
         // padded tile -> slab[ch][bin][cell] (plain stores; summed by gather)
-        const int64_t nbins = (int64_t)g.nb[0] * g.nb[1];
+        const int64_t nbins = (int64_t)g.nb[0] * g.nb[1] * (D == 3 ? g.nb[2] : 1);
         for (int e = threadIdx.x; e < chg * CELLS; e += blockDim.x) {
             const int ch = e / CELLS, r = e - ch * CELLS;
-            slab[((int64_t)(ch0 + ch) * nbins + bin) * CELLS + r] = tile[e];
+            slab[((int64_t)(ch0 - chan0 + ch) * nbins + bin) * CELLS + r] = tile[e];
         }
     } else {
         const int64_t G = g.n[0] * g.n[1] * g.n[2];
@@ -376,6 +391,48 @@ __global__ void gather_tiles_2d(const AT* __restrict__ slab, PlanGeom g, int nch
     }
 }
 
+// grid[chan0 + ch][cell] = sum over the non-empty tiles whose padded range
+// covers the cell (fixed order: z, y, x tile order), for nc channels
+template <class AT>
+__global__ void gather_tiles_3d(const AT* __restrict__ slab, const int* __restrict__ offs,
+                                PlanGeom g, int nc, int chan0, AT* __restrict__ grid) {
+    constexpr int H = SPREAD_HALO;
+    using TS = TileShape<3>;
+    constexpr int LX = TS::X + 2 * H, LY = TS::Y + 2 * H, LZ = TS::Z + 2 * H;
+    constexpr int CELLS = LX * LY * LZ;
+    const int64_t G = g.n[0] * g.n[1] * g.n[2];
+    const int64_t nbins = (int64_t)g.nb[0] * g.nb[1] * g.nb[2];
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < G * nc;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(e / G);
+        const int64_t q = e - (int64_t)ch * G;
+        const int64_t gx = q % g.n[0];
+        const int64_t r = q / g.n[0];
+        const int64_t gy = r % g.n[1], gz = r / g.n[1];
+        int tx[6], lxs[6], ty[6], lys[6], tz[6], lzs[6];
+        const int nx = covering(gx, g.n[0], TS::X, g.nb[0], tx, lxs);
+        const int ny = covering(gy, g.n[1], TS::Y, g.nb[1], ty, lys);
+        const int nz = covering(gz, g.n[2], TS::Z, g.nb[2], tz, lzs);
+        AT acc = cz<AT>();
+        const AT* sc = slab + (int64_t)ch * nbins * CELLS;
+        for (int c = 0; c < nz; ++c)
+            for (int b = 0; b < ny; ++b)
+                for (int a = 0; a < nx; ++a) {
+                    const int64_t bin = ((int64_t)tz[c] * g.nb[1] + ty[b]) * g.nb[0] + tx[a];
+                    if (offs[bin] == offs[bin + 1]) continue;  // empty tile: not stored
+                    const AT v = sc[bin * CELLS + ((int64_t)lzs[c] * LY + lys[b]) * LX + lxs[a]];
+                    acc.x += v.x;
+                    acc.y += v.y;
+                }
+        grid[(int64_t)(chan0 + ch) * G + q] = acc;
+    }
+}
+
+// device bytes of the 3D spreading slab per pass (channels per pass chosen
+// so the slab stays within this budget; above it for a single channel the
+// tiles are added with atomics)
+constexpr size_t SPREAD3D_SLAB_BUDGET = (size_t)8 << 30;
+
 size_t spread_workspace_bytes(const SpreadPlan& plan, int nch, bool f64) {
     if (plan.d != 2) return 0;
     constexpr int CELLS = (TileShape<2>::X + 2 * SPREAD_HALO) * (TileShape<2>::Y + 2 * SPREAD_HALO);
@@ -400,17 +457,42 @@ static void spread_t(const msr_traj* traj, const SpreadPlan& plan, const ValueSr
         dim3 grd((unsigned)plan.nbins, (unsigned)ceil_div(nch, CHG));
         hipLaunchKernelGGL((spread_tiles_kernel<2, AT>), grd, dim3(64 * CHG), 0, st,
                            traj->kc.as<float>(), traj->frame.as<int>(), plan.perm.as<int>(),
-                           plan.offs.as<int>(), g, src, nch, slab, grid);
+                           plan.offs.as<int>(), g, src, nch, 0, slab, grid);
         MSR_CHECK_LAUNCH();
         int gr = (int)std::min<int64_t>((G * nch + 255) / 256, 65536);
         hipLaunchKernelGGL(gather_tiles_2d<AT>, dim3(gr), dim3(256), 0, st, slab, g, nch, grid);
     } else {
-        fill_floats(grid, (int64_t)(sizeof(AT) / sizeof(float)) * G * nch, 0.f, st);
         constexpr int CHG = chg_of<3, AT>();
-        dim3 grd((unsigned)plan.nbins, (unsigned)ceil_div(nch, CHG));
-        hipLaunchKernelGGL((spread_tiles_kernel<3, AT>), grd, dim3(64 * CHG), 0, st,
-                           traj->kc.as<float>(), traj->frame.as<int>(), plan.perm.as<int>(),
-                           plan.offs.as<int>(), g, src, nch, slab, grid);
+        constexpr int CELLS = (TileShape<3>::X + 2 * SPREAD_HALO) *
+                              (TileShape<3>::Y + 2 * SPREAD_HALO) * (TileShape<3>::Z + 2 * SPREAD_HALO);
+        const size_t per_ch = sizeof(AT) * (size_t)plan.nbins * CELLS;
+        // channels per pass: a multiple of the workgroup's channel count
+        int cpp = (int)std::min<size_t>((size_t)nch, SPREAD3D_SLAB_BUDGET / per_ch);
+        cpp = cpp >= CHG ? cpp / CHG * CHG : cpp;
+        if (cpp < CHG) {  // slab of one channel group above the budget: atomics
+            fill_floats(grid, (int64_t)(sizeof(AT) / sizeof(float)) * G * nch, 0.f, st);
+            dim3 grd((unsigned)plan.nbins, (unsigned)ceil_div(nch, CHG));
+            hipLaunchKernelGGL((spread_tiles_kernel<3, AT>), grd, dim3(64 * 8 * CHG), 0, st,
+                               traj->kc.as<float>(), traj->frame.as<int>(), plan.perm.as<int>(),
+                               plan.offs.as<int>(), g, src, nch, 0, (AT*)nullptr, grid);
+        } else {
+            DevBuf work;
+            work.alloc(per_ch * cpp);
+            for (int c0 = 0; c0 < nch; c0 += cpp) {
+                const int nc = std::min(cpp, nch - c0);
+                dim3 grd((unsigned)plan.nbins, (unsigned)ceil_div(nc, CHG));
+                hipLaunchKernelGGL((spread_tiles_kernel<3, AT>), grd, dim3(64 * 8 * CHG), 0, st,
+                                   traj->kc.as<float>(), traj->frame.as<int>(),
+                                   plan.perm.as<int>(), plan.offs.as<int>(), g, src,
+                                   c0 + nc, c0, work.as<AT>(), grid);
+                MSR_CHECK_LAUNCH();
+                const int gr = (int)std::min<int64_t>((G * nc + 255) / 256, 65536);
+                hipLaunchKernelGGL(gather_tiles_3d<AT>, dim3(gr), dim3(256), 0, st, work.as<AT>(),
+                                   plan.offs.as<int>(), g, nc, c0, grid);
+                MSR_CHECK_LAUNCH();
+            }
+            MSR_HIP(hipStreamSynchronize(st));  // the slab is released on return
+        }
     }
     MSR_CHECK_LAUNCH();
 }
