@@ -203,3 +203,25 @@ def test_cartesian_operator_in_a_batch_and_errors(torch, msr):
         A.A.mul_(torch.empty_like(bA), bA, percoil=True)
     with pytest.raises(NotImplementedError):
         msr.FFTNormalOp((30, 24), dev(torch, trj), dev(torch, U))  # no FFT of length 30
+
+
+def test_cartesian_backprojection_inverts_nufft_on_the_grid(torch, msr):
+    """test/backprojection_cart.jl: data simulated by the (type-2 NUFFT)
+    forward model at the float Cartesian grid k = (i - 1 - N/2)/N, then
+    backprojected with the equivalent integer trajectory i, returns the image
+    (coil maps normalised over coils; the reference asserts atol 4e-5)."""
+    Nx, C = 32, 9
+    rng = np.random.default_rng(6)
+    x = (rng.normal(size=(1, Nx, Nx)) + 1j * rng.normal(size=(1, Nx, Nx))).astype(np.complex64)
+    cm = msr.synth.gaussian_coil_maps((Nx, Nx), C, seed=6).astype(np.complex128)
+    cm /= np.linalg.norm(cm, axis=0)[None]
+    cm = cm.astype(np.complex64)
+    cml = [dev(torch, c) for c in cm]
+    ti = traj_cartesian(Nx, Nx, 1)                            # integer 1..N
+    tf = ((ti - 1 - Nx // 2) / Nx).astype(np.float32)         # traj_cartesian(...; T=Float32)
+    U = np.ones((1, 1), dtype=np.complex64)
+    data = msr.forward(dev(torch, x), dev(torch, tf), cmaps=cml, U=dev(torch, U))
+    reco = msr.calculate_backprojection(data, dev(torch, ti), cml, U=dev(torch, U)).cpu().numpy()
+    err = np.abs(reco - x).max()
+    print(f"cartesian backprojection of NUFFT grid data: max abs err {err:.2e}", flush=True)
+    assert err < 4e-5
