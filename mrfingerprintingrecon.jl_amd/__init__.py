@@ -815,9 +815,11 @@ def cg_(x, A, b, maxiter=None, reltol=REL_TOL_F32, abstol=0.0, initially_zero=Fa
         side = _side_stream(op.device)
         side.wait_stream(cur)
     st = ctypes.c_void_p((side or cur).cuda_stream)
+    # the iteration count and residual are read back (a stream synchronisation)
+    # only when asked for
     _lib.call("msr_cg", op.h, _ptr(x), _ptr(b), int(maxiter), float(reltol), float(abstol),
               int(bool(initially_zero)), _lib.CG_MODES[mode], _prec(precision),
-              ctypes.byref(it), ctypes.byref(res), st)
+              ctypes.byref(it) if log else None, ctypes.byref(res) if log else None, st)
     if side is not None:
         cur.wait_stream(side)
     if log:
