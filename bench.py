@@ -392,14 +392,16 @@ def batched_leg(msr, njobs=64, iters=CG_ITERS, seed=1004):
 
 
 def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, niter_step=5,
-               max_active=64):
+               max_active=64, policy="fifo"):
     """Served stream: Poisson arrivals (rate jobs/s) of 2D jobs drawn as in the
     batched leg over a set of protocols whose kernels stay resident; each job
     brings its own coil maps (an operator sharing its protocol's kernel) and
     data.  On arrival a job's backprojection runs and its CG is submitted to
     the server (continuous batching, admission by device memory); latency is
     arrival to solution.  The same arrivals served one at a time (FIFO, each
-    job's backprojection and single-operator graph CG) are the baseline."""
+    job's backprojection and single-operator graph CG) are the baseline; the
+    server also runs them under shortest-work-first admission with one job in
+    flight (lower median latency, longer tail)."""
     import numpy as np
     import torch
     dev = torch.device("cuda:0")
@@ -432,7 +434,7 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
                          b=torch.empty((K, N, N), dtype=torch.complex64, device=dev),
                          x=torch.empty(K * N * N, dtype=torch.complex64, device=dev),
                          x1=torch.empty(K * N * N, dtype=torch.complex64, device=dev)))
-    srv = msr.NFFTServer(max_active=max_active, precision=PRECISION)
+    srv = msr.NFFTServer(max_active=max_active, precision=PRECISION, policy=policy)
 
     def bp(j):
         msr.calculate_backprojection(j["data"], j["pr"]["traj"], j["cml"], U=j["pr"]["U"], out=j["b"])
@@ -457,6 +459,8 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
             lat.append(time.perf_counter() - t0 - j["t"])
         return time.perf_counter() - t0, lat
 
+    step_iters = [niter_step]
+
     def run_server():
         lat = [None] * len(jobs)
         ids = {}
@@ -470,7 +474,7 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
                 ids[srv.submit(j["A"], j["x"], j["b"].reshape(-1), maxiter=iters)] = nxt
                 nxt += 1
             if ids:
-                srv.step(niter_step)
+                srv.step(step_iters[0])
                 tdone = time.perf_counter() - t0
                 for jid in [k for k in ids if srv.poll(k)["state"] == srv.DONE]:
                     i = ids.pop(jid)
@@ -480,6 +484,10 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
 
     t_fifo, lat_fifo = run_fifo()
     t_srv, lat_srv = run_server()
+    # the same arrivals under shortest-work-first admission, one job in flight
+    srv = msr.NFFTServer(max_active=1, precision=PRECISION, policy="shortest_first")
+    step_iters[0] = iters
+    t_sjf, lat_sjf = run_server()
     err = max(float(torch.linalg.norm(j["x"] - j["x1"]) / torch.linalg.norm(j["x1"])) for j in jobs)
     ms = lambda v: round(1e3 * v, 3)
     res = {"workload": f"{njobs} 2D recons, Poisson arrivals at {rate}/s over {nproto} resident "
@@ -490,8 +498,12 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
            "one_at_a_time": {"throughput_solves_per_s": round(njobs / t_fifo, 2),
                              "latency_ms": {"p50": ms(percentile(lat_fifo, 50)),
                                             "p99": ms(percentile(lat_fifo, 99))}},
+           "shortest_first": {"max_active": 1, "iterations_per_step": iters,
+                              "throughput_solves_per_s": round(njobs / t_sjf, 2),
+                              "latency_ms": {"p50": ms(percentile(lat_sjf, 50)),
+                                             "p99": ms(percentile(lat_sjf, 99))}},
            "arrival_span_s": round(jobs[-1]["t"], 3),
-           "max_active": max_active, "iterations_per_step": niter_step,
+           "max_active": max_active, "iterations_per_step": niter_step, "policy": policy,
            "max_rel_diff_served_vs_single": err}
     log(f"served: {res}")
     del srv, jobs, protos

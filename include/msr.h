@@ -401,6 +401,14 @@ int msr_server_create(msr_ctx *ctx, size_t budget_bytes, int max_active, int pre
 int msr_server_destroy(msr_server *server);
 int msr_server_submit(msr_server *server, msr_normalop *op, void *x, const void *b, int maxiter,
                       double reltol, double abstol, int64_t *job_id);
+/* Admission order of queued jobs: MSR_SERVER_FIFO (arrival order, the default)
+ * or MSR_SERVER_SHORTEST_FIRST (least estimated work first: maxiter x coils x
+ * K x grid cells of the operator; ties in arrival order).  With a small
+ * max_active the second approximates shortest-job-first scheduling, which
+ * lowers the median latency of a mixed stream at high load. */
+#define MSR_SERVER_FIFO 0
+#define MSR_SERVER_SHORTEST_FIRST 1
+int msr_server_set_policy(msr_server *server, int policy);
 /* n_active / n_queued (may be NULL): jobs in flight / waiting after the step;
  * synchronises the stream. */
 int msr_server_step(msr_server *server, int niter, int *n_active, int *n_queued, void *stream);

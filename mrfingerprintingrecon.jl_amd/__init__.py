@@ -941,15 +941,18 @@ __all__ += ["NFFTNormalOpBatch"]
 
 
 class NFFTServer:
-    """Continuous batching of CG solves (msr_server_*): jobs wait in a FIFO
-    queue, are admitted while the device state of the jobs in flight fits
+    """Continuous batching of CG solves (msr_server_*): jobs wait in a queue
+    (admitted in arrival order, or least estimated work first with
+    ``policy="shortest_first"``), are admitted while the device state of the jobs in flight fits
     ``budget_bytes`` (at most ``max_active`` of them), run in lockstep batched
     CG iterations and retire when their CG stops.  Each job's result is the
     one ``cg_`` gives for its operator alone (same precision and stopping)."""
 
     QUEUED, RUNNING, DONE = 0, 1, 2
+    POLICIES = {"fifo": 0, "shortest_first": 1}
 
-    def __init__(self, device=None, budget_bytes=None, max_active=64, precision="f64"):
+    def __init__(self, device=None, budget_bytes=None, max_active=64, precision="f64",
+                 policy="fifo"):
         torch = _torch()
         dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         if budget_bytes is None:
@@ -962,6 +965,9 @@ class NFFTServer:
         _lib.call("msr_server_create", _ctx(dev.index), ctypes.c_size_t(int(budget_bytes)),
                   int(max_active), _prec(precision), ctypes.byref(h))
         self.h = h
+        if policy not in self.POLICIES:
+            raise ArgumentError(f"server policy {policy!r}: one of {sorted(self.POLICIES)}")
+        _lib.call("msr_server_set_policy", h, self.POLICIES[policy])
 
     def submit(self, A, x, b, maxiter, reltol=0.0, abstol=0.0):
         """Queue the solve of A x = b (x, b: contiguous complex64 device vectors

@@ -93,6 +93,7 @@ SIGNATURES = {
     "msr_server_destroy": (_I, [_P]),
     "msr_server_submit": (_I, [_P, _P, _P, _P, _I, _D, _D, _P]),
     "msr_server_step": (_I, [_P, _I, _P, _P, _P]),
+    "msr_server_set_policy": (_I, [_P, _I]),
     "msr_server_poll": (_I, [_P, ctypes.c_int64, _P, _P, _P, _P]),
     "msr_server_forget": (_I, [_P, ctypes.c_int64]),
     "msr_batch_create": (_I, [_P, _I, _P, _P]),

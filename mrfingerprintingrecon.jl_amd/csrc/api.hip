@@ -35,6 +35,7 @@ int64_t server_submit(msr_server* s, msr_normalop* op, float2* x, const float2* 
 void server_step(msr_server* s, int niter, int* n_active, int* n_queued, hipStream_t st);
 void server_poll(msr_server* s, int64_t id, int* state, int* iters, double* res, double* times);
 void server_forget(msr_server* s, int64_t id);
+void server_set_policy(msr_server* s, int policy);
 
 // kcache.hip
 void kernel_key(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, int K,
@@ -705,5 +706,12 @@ int msr_normalop_kind(const msr_normalop* op, int* cartesian) {
     return guarded([&] {
         MSR_NOTNULL(op);
         if (cartesian) *cartesian = op->cart;
+    });
+}
+
+int msr_server_set_policy(msr_server* server, int policy) {
+    return guarded([&] {
+        MSR_NOTNULL(server);
+        server_set_policy(server, policy);
     });
 }

@@ -51,6 +51,9 @@ int64_t server_submit(msr_server* s, msr_normalop* op, float2* x, const float2* 
     j.reltol = reltol;
     j.abstol = abstol;
     j.bytes = op_footprint(op, s->prec);
+    int64_t cells = 1;
+    for (int a = 0; a < op->d; ++a) cells *= (op->cart ? 1 : 2) * op->N[a];
+    j.work = (double)maxiter * op->C * op->K * (double)cells;
     MSR_REQUIRE(j.bytes <= s->budget, MSR_ERR_ARGUMENT,
                 "the job needs %zu bytes of device state, above the server budget of %zu", j.bytes,
                 s->budget);
@@ -70,8 +73,18 @@ static void rebuild_batch(msr_server* s) {
     s->batch = batch_create(s->ctx, (int)ops.size(), ops.data());
 }
 
+void server_set_policy(msr_server* s, int policy) {
+    MSR_REQUIRE(policy == MSR_SERVER_FIFO || policy == MSR_SERVER_SHORTEST_FIRST,
+                MSR_ERR_ARGUMENT, "server policy %d", policy);
+    s->policy = policy;
+}
+
 void server_step(msr_server* s, int niter, int* n_active, int* n_queued, hipStream_t st) {
     MSR_REQUIRE(niter >= 1, MSR_ERR_ARGUMENT, "niter = %d", niter);
+    if (s->policy == MSR_SERVER_SHORTEST_FIRST)  // least work first, ties in arrival order
+        std::stable_sort(s->queue.begin(), s->queue.end(), [&](int64_t a, int64_t b) {
+            return s->jobs[a].work < s->jobs[b].work;
+        });
     // admit (FIFO) while the jobs in flight fit the budget; a job alone always fits
     std::vector<int64_t> admitted;
     while (!s->queue.empty() && (int)(s->active.size() + admitted.size()) < s->max_active) {

@@ -21,6 +21,7 @@ struct ServerJob {
     int iters = 0;
     double res = 0.0;
     size_t bytes = 0;
+    double work = 0.0;        // estimated work (admission order of MSR_SERVER_SHORTEST_FIRST)
     double t_submit = 0.0, t_start = 0.0, t_done = 0.0;  // host clock, ms
 };
 
@@ -31,6 +32,7 @@ struct msr_server {
     size_t budget = 0;
     int max_active = 0;
     int prec = MSR_PREC_F64;
+    int policy = MSR_SERVER_FIFO;
     std::deque<int64_t> queue;
     std::vector<int64_t> active;
     std::map<int64_t, msr::ServerJob> jobs;

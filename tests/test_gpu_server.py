@@ -143,3 +143,38 @@ def test_shared_kernel_operators_match_owned_copies(torch, jobs):
     del shared  # the kernel stays with the prototype
     y3 = (j["A"] @ j["b"]).cpu().numpy()
     assert np.isfinite(y3).all()
+
+
+def test_shortest_first_policy_admits_least_work_first(torch, jobs):
+    """MSR_SERVER_SHORTEST_FIRST with one job in flight: jobs finish in order
+    of estimated work (iterations x coils x K x grid cells), ties in arrival
+    order; results as alone."""
+    msr = load_package()
+    srv = msr.NFFTServer(max_active=1, policy="shortest_first")
+    xs = [torch.empty_like(j["b"]) for j in jobs]
+    ids = [srv.submit(j["A"], x, j["b"], maxiter=j["iters"]) for j, x in zip(jobs, xs)]
+    order = []
+    while len(order) < len(ids):
+        srv.step(niter=50)
+        for i, jid in enumerate(ids):
+            if i not in order and srv.poll(jid)["state"] == srv.DONE:
+                order.append(i)
+
+    def work(i):
+        j = jobs[i]
+        cells = 1
+        for n in j["N"]:
+            cells *= 2 * n
+        C = 1 if j["cml"] is None else len(j["cml"])
+        K = int(j["lam"].shape[1] if j["lam"].ndim == 2 else 1)
+        K = int((np.sqrt(8 * K + 1) - 1) // 2)
+        return j["iters"] * C * K * cells
+
+    expect = sorted(range(len(jobs)), key=lambda i: (work(i), i))
+    print(f"completion order {order}, expected {expect}", flush=True)
+    assert order == expect
+    for j, x in zip(jobs, xs):
+        single = msr.cg(j["A"], j["b"], maxiter=j["iters"], reltol=0.0, mode="device").cpu().numpy()
+        assert rel_err(x.cpu().numpy(), single) <= 1e-6
+    with pytest.raises(msr.ArgumentError):
+        msr.NFFTServer(policy="random")
