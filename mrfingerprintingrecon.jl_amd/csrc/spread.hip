@@ -7,6 +7,7 @@
 // so the point's support stays within 4 cells of its tile: a tile of TB cells
 // per dim is accumulated in a (TB + 8)^d LDS block.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <map>
 #include <tuple>
@@ -429,9 +430,13 @@ __global__ void gather_tiles_3d(const AT* __restrict__ slab, const int* __restri
 }
 
 // device bytes of the 3D spreading slab per pass (channels per pass chosen
-// so the slab stays within this budget; above it for a single channel the
-// tiles are added with atomics)
-constexpr size_t SPREAD3D_SLAB_BUDGET = (size_t)8 << 30;
+// so the slab stays within this budget; above it for one channel group the
+// tiles are added with atomics).  MSR_SPREAD3D_SLAB_BUDGET (bytes) overrides
+// the 8 GiB default.
+static size_t spread3d_slab_budget() {
+    const char* e = getenv("MSR_SPREAD3D_SLAB_BUDGET");
+    return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)8 << 30;
+}
 
 size_t spread_workspace_bytes(const SpreadPlan& plan, int nch, bool f64) {
     if (plan.d != 2) return 0;
@@ -467,7 +472,7 @@ static void spread_t(const msr_traj* traj, const SpreadPlan& plan, const ValueSr
                               (TileShape<3>::Y + 2 * SPREAD_HALO) * (TileShape<3>::Z + 2 * SPREAD_HALO);
         const size_t per_ch = sizeof(AT) * (size_t)plan.nbins * CELLS;
         // channels per pass: a multiple of the workgroup's channel count
-        int cpp = (int)std::min<size_t>((size_t)nch, SPREAD3D_SLAB_BUDGET / per_ch);
+        int cpp = (int)std::min<size_t>((size_t)nch, spread3d_slab_budget() / per_ch);
         cpp = cpp >= CHG ? cpp / CHG * CHG : cpp;
         if (cpp < CHG) {  // slab of one channel group above the budget: atomics
             fill_floats(grid, (int64_t)(sizeof(AT) / sizeof(float)) * G * nch, 0.f, st);
