@@ -116,6 +116,10 @@ def main():
                     help="3D case instead: 128^3 image, 16 coils, K=4, koosh-ball trajectory "
                          "(2000 frames x 4 spokes x 256 samples), backprojection + 15-iteration "
                          "CG; prints its own JSON line")
+    ap.add_argument("--rows", action="store_true",
+                    help="the operators beside the solve instead (forward model, Cartesian "
+                         "operator, coil-map calibration) at the same size; prints its own "
+                         "JSON line")
     ap.add_argument("--apply-only", type=int, default=0, metavar="N",
                     help="profiling mode: build the operator, run N fused applies, N per-coil "
                          "applies and a 256 MiB calibration copy, then exit")
@@ -148,6 +152,9 @@ def main():
     torch.cuda.synchronize()
     build_ms = (time.perf_counter() - t0) * 1e3
     nvec = K * N_IMG[0] * N_IMG[1]
+    if args.rows:
+        rows_leg(msr, torch, dev, trj_d, traj, U_d, cml, data_d)
+        return
     if args.apply_only:
         xin = torch.randn(nvec, dtype=torch.complex64, device=dev)
         y = torch.empty_like(xin)
@@ -335,6 +342,90 @@ def volume_leg(msr, torch, dev, args, N=128, C=16, K3=4, nt=2000, spokes=4, iter
                       "s_per_solve": round(wall, 4), "steps": steps,
                       "kernel_build_s": round(build_s, 3),
                       "device_state_bytes": list(msr.footprint(A, PRECISION))}), flush=True)
+
+
+def rows_leg(msr, torch, dev, trj_d, traj, U_d, cml, data_d, lines=8, calib_iters=100, reps=20):
+    """The operators either side of the solve, at the same 256x256 / 20-coil /
+    K=5 / 1000-frame size: the forward model A x, the Cartesian operator
+    (8 random phase-encode lines x 256 readout samples per frame: kernel
+    build, applies, backprojection + 25-iteration FP64 CG) and the
+    calibration front end of calculate_coil_maps (calibration region,
+    coil-wise K=1 reconstruction, calibration k-space).  Wall-clock times of
+    `reps` calls between device synchronisations."""
+    import numpy as np
+    nx, ny = N_IMG
+    nvec = K * nx * ny
+
+    def wall(fn, n=reps, warm=2):
+        for _ in range(warm):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    x = torch.randn((K, ny, nx), dtype=torch.complex64, device=dev, generator=g)
+    y_fwd = torch.empty((COILS, NT, NR), dtype=torch.complex64, device=dev)
+    fwd_ms = wall(lambda: msr.forward(x, traj, cmaps=cml, U=U_d, out=y_fwd))
+    fwd = {"ms": round(fwd_ms, 4), "samples_per_s": round(COILS * NT * NR / fwd_ms * 1e3, 1)}
+
+    rng = np.random.default_rng(11)
+    pe = np.stack([rng.choice(ny, lines, replace=False) for _ in range(NT)]) + 1  # 1-based
+    trj_c = np.empty((NT, lines * nx, 2), dtype=np.int32)
+    trj_c[:, :, 0] = np.tile(np.arange(1, nx + 1), lines)
+    trj_c[:, :, 1] = np.repeat(pe, nx, axis=1)
+    trj_cd = torch.from_numpy(trj_c).to(dev)
+    dc = torch.randn((COILS, NT, lines * nx), dtype=torch.complex64, device=dev, generator=g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    traj_c = msr.plan_trajectory(trj_cd)
+    Ac = msr.FFTNormalOp(N_IMG, traj_c, U_d, cmaps=cml)
+    torch.cuda.synchronize()
+    cart_build = (time.perf_counter() - t0) * 1e3
+    v32 = torch.randn(nvec, dtype=torch.complex64, device=dev, generator=g)
+    w32 = torch.empty_like(v32)
+    v64 = v32.to(torch.complex128)
+    w64 = torch.empty_like(v64)
+    a64 = wall(lambda: Ac.A.mul_f64_(w64, v64), 50)
+    a32 = wall(lambda: Ac.A.mul_(w32, v32), 50)
+    bc = torch.empty((K, ny, nx), dtype=torch.complex64, device=dev)
+    xc = torch.empty(nvec, dtype=torch.complex64, device=dev)
+    bp_ms = wall(lambda: msr.calculate_backprojection(dc, traj_c, cml, U=U_d, out=bc))
+
+    def solve():
+        msr.calculate_backprojection(dc, traj_c, cml, U=U_d, out=bc)
+        msr.cg_(xc, Ac, bc.reshape(-1), maxiter=CG_ITERS, reltol=0.0, initially_zero=True,
+                mode="graph", precision=PRECISION)
+
+    solve_ms = wall(solve, 10)
+    cart = {"kernel_build_ms": round(cart_build, 3), "apply_f64_ms": round(a64, 4),
+            "apply_f32_ms": round(a32, 4), "backprojection_ms": round(bp_ms, 4),
+            "solve_ms": round(solve_ms, 4), "solves_per_s": round(1e3 / solve_ms, 2),
+            "samples": int(NT * lines * nx)}
+
+    calib = msr.default_calib_size(N_IMG)
+    out = {}
+
+    def calibrate():
+        tc, mc = msr.calibration_region(trj_d, N_IMG, calib)
+        xcoil = msr.reconstruct_coilwise(data_d, tc, calib, sample_mask=mc, Niter_cg=calib_iters)
+        out["kbp"] = msr.calibration_kspace(xcoil)
+
+    cal_ms = wall(calibrate, 3, 1)
+    print(json.dumps({"workload": f"operators beside the solve at {nx}x{ny}, {COILS} coils, K={K}, "
+                                  f"{NT} frames", "forward": fwd,
+                      "cartesian": dict(cart, workload=f"{lines} phase-encode lines x {nx} samples "
+                                                       f"per frame, BP + {CG_ITERS}-iteration "
+                                                       f"{PRECISION.upper()} CG"),
+                      "calibration": {"calib_size": list(calib), "cg_iters": calib_iters,
+                                      "ms": round(cal_ms, 3),
+                                      "workload": "calibration region + coil-wise K=1 "
+                                                  "reconstruction + calibration k-space "
+                                                  "(ESPIRiT itself excluded)"}}), flush=True)
 
 
 def cache_leg(msr, torch, traj, U_d, cml):

@@ -540,31 +540,48 @@ void launch_cart_passes(msr_normalop* op, const void* b, bool f64, hipStream_t s
 
 // ------------------------------------------------------------ backprojection
 // dataU[cell, c, k] = sum over the cell's samples (sample order, ComplexF32 as
-// the reference) of data * conj(U[t, k])  (src/BackProjection.jl:113-121)
+// the reference) of data * conj(U[t, k])  (src/BackProjection.jl:113-121).
+// One thread per (cell, coil) holds CART_KB coefficients in registers, so a
+// sample is read once per CART_KB coefficients; neighbouring threads are
+// neighbouring readout cells, whose e-th samples are neighbouring samples of
+// one readout line (coalesced data reads).
+constexpr int CART_KB = 8;
 __global__ __launch_bounds__(256) void cart_grid_kernel(
     const int* __restrict__ perm, const int* __restrict__ offs, const int* __restrict__ frame,
     const int64_t* __restrict__ sel, const float2* __restrict__ data, int64_t nsel,
     const void* __restrict__ U, int u_real, int64_t nt, int K, int C, int64_t G,
     float2* __restrict__ grid) {
-    const int64_t n = G * C * K;
+    const int64_t n = G * C;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t q = i % G;
-        const int64_t ck = i / G;
-        const int k = (int)(ck % K);
-        const int c = (int)(ck / K);
-        float2 acc = make_float2(0.f, 0.f);
-        for (int e = offs[q]; e < offs[q + 1]; ++e) {
-            const int j = perm[e];
-            const int t = frame[j];
-            float2 u;
-            if (!U) u = make_float2(t == k ? 1.f : 0.f, 0.f);
-            else if (u_real) u = make_float2(static_cast<const float*>(U)[t + (int64_t)k * nt], 0.f);
-            else u = static_cast<const float2*>(U)[t + (int64_t)k * nt];
-            const float2 y = data[(int64_t)c * nsel + (sel ? sel[j] : j)];
-            acc = cadd(acc, cmulc(u, y));  // y * conj(u)
+        const int c = (int)(i / G);
+        const int e0 = offs[q], e1 = offs[q + 1];
+        const float2* yc = data + (int64_t)c * nsel;
+        for (int k0 = 0; k0 < K; k0 += CART_KB) {
+            float2 acc[CART_KB];
+#pragma unroll
+            for (int kk = 0; kk < CART_KB; ++kk) acc[kk] = make_float2(0.f, 0.f);
+            for (int e = e0; e < e1; ++e) {
+                const int j = perm[e];
+                const int t = frame[j];
+                const float2 y = yc[sel ? sel[j] : j];
+#pragma unroll
+                for (int kk = 0; kk < CART_KB; ++kk) {
+                    const int k = k0 + kk;
+                    if (k >= K) break;
+                    float2 u;
+                    if (!U) u = make_float2(t == k ? 1.f : 0.f, 0.f);
+                    else if (u_real)
+                        u = make_float2(static_cast<const float*>(U)[t + (int64_t)k * nt], 0.f);
+                    else u = static_cast<const float2*>(U)[t + (int64_t)k * nt];
+                    acc[kk] = cadd(acc[kk], cmulc(u, y));  // y * conj(u)
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < CART_KB; ++kk)
+                if (k0 + kk < K) grid[((int64_t)c * K + k0 + kk) * G + q] = acc[kk];
         }
-        grid[ck * G + q] = acc;
     }
 }
 
@@ -610,7 +627,7 @@ void cart_backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_
     work.alloc(2 * sizeof(float2) * (size_t)nch * V);
     float2* Gd = work.as<float2>();
     float2* H = Gd + nch * V;
-    const int grid = (int)std::min<int64_t>((nch * V + 255) / 256, 16384);
+    const int grid = (int)std::min<int64_t>(((int64_t)ncoils * V + 255) / 256, 16384);
     hipLaunchKernelGGL(cart_grid_kernel, dim3(grid), dim3(256), 0, st, plan.perm.as<int>(),
                        plan.offs.as<int>(), traj->frame.as<int>(),
                        traj->sel.p ? traj->sel.as<int64_t>() : nullptr, (const float2*)data,

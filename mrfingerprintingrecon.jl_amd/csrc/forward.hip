@@ -49,7 +49,15 @@ struct FwdGeom {
     int64_t n[3];
 };
 
-// one wavefront per (sample, coil); lane = kernel cell (ox, oy), z looped
+// one wavefront per sample; lane = kernel cell (ox, oy), z looped.  The
+// sample's position, weights and frame are read once; its coils run in
+// groups of FW_CB whose grid loads are independent of each other (several
+// loads in flight per wavefront); a group is reduced over the 64 cells by one
+// reduce-scatter (10 lane exchanges for 4 coils instead of 48).  Each
+// (sample, coil) sums its K channels in channel order and its cells in a
+// fixed tree.
+constexpr int FW_CB = 4;  // the reduce-scatter below splits lane bits 5..3 over 4 coils x 2 parts
+static_assert(FW_CB == 4, "the interpolation reduce-scatter is written for 4 coils");
 template <int D>
 __global__ void __launch_bounds__(256)
 interp_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
@@ -61,9 +69,7 @@ interp_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
     const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int64_t Gc = g.n[0] * g.n[1] * g.n[2];
-    for (int64_t w = wave0; w < M * C; w += nwaves) {
-        const int64_t j = w % M;
-        const int c = (int)(w / M);
+    for (int64_t j = wave0; j < M; j += nwaves) {
         int64_t cx, cy, cz = 0;
         float fx, fy, fz = 0.f;
         cell_frac(kc[j * D + 0], g.n[0], cx, fx);
@@ -74,39 +80,63 @@ interp_kernel(const float* __restrict__ kc, const int* __restrict__ frame,
         if (gx < 0) gx += g.n[0];
         int64_t gy = (cy + oy) % g.n[1];
         if (gy < 0) gy += g.n[1];
+        const int64_t off = gy * g.n[0] + gx;
         const int t = frame[j];
-        float2 acc = make_float2(0.f, 0.f);
-        for (int k = 0; k < K; ++k) {
-            float2 u;
-            if (!U) u = make_float2(t == k ? 1.f : 0.f, 0.f);
-            else if (u_real) u = make_float2(reinterpret_cast<const float*>(U)[t + (int64_t)k * nt], 0.f);
-            else u = reinterpret_cast<const float2*>(U)[t + (int64_t)k * nt];
-            const float2* Gk = G + ((int64_t)c * K + k) * Gc;
-            float2 s = make_float2(0.f, 0.f);
-            if (D == 2) {
-                const float2 v = Gk[gy * g.n[0] + gx];
-                s = make_float2(wxy * v.x, wxy * v.y);
-            } else {
+        const int64_t idx = sel ? sel[j] : j;
+        for (int c0 = 0; c0 < C; c0 += FW_CB) {
+            float2 acc[FW_CB];
 #pragma unroll
-                for (int oz = -3; oz <= 4; ++oz) {
-                    int64_t gz = (cz + oz) % g.n[2];
-                    if (gz < 0) gz += g.n[2];
-                    const float wz = wxy * es_phi_fw((float)oz - fz);
-                    const float2 v = Gk[(gz * g.n[1] + gy) * g.n[0] + gx];
-                    s.x += wz * v.x;
-                    s.y += wz * v.y;
+            for (int cc = 0; cc < FW_CB; ++cc) acc[cc] = make_float2(0.f, 0.f);
+            for (int k = 0; k < K; ++k) {
+                float2 u;
+                if (!U) u = make_float2(t == k ? 1.f : 0.f, 0.f);
+                else if (u_real) u = make_float2(reinterpret_cast<const float*>(U)[t + (int64_t)k * nt], 0.f);
+                else u = reinterpret_cast<const float2*>(U)[t + (int64_t)k * nt];
+#pragma unroll
+                for (int cc = 0; cc < FW_CB; ++cc) {
+                    const int c = c0 + cc < C ? c0 + cc : C - 1;  // a spare slot repeats the last coil
+                    const float2* Gk = G + ((int64_t)c * K + k) * Gc;
+                    float2 s = make_float2(0.f, 0.f);
+                    if (D == 2) {
+                        const float2 v = Gk[off];
+                        s = make_float2(wxy * v.x, wxy * v.y);
+                    } else {
+#pragma unroll
+                        for (int oz = -3; oz <= 4; ++oz) {
+                            int64_t gz = (cz + oz) % g.n[2];
+                            if (gz < 0) gz += g.n[2];
+                            const float wz = wxy * es_phi_fw((float)oz - fz);
+                            const float2 v = Gk[gz * g.n[1] * g.n[0] + off];
+                            s.x += wz * v.x;
+                            s.y += wz * v.y;
+                        }
+                    }
+                    cfma(acc[cc], u, s);
                 }
             }
-            cfma(acc, u, s);
-        }
+            // reduce-scatter over the 64 lanes: lane bits 5, 4 pick the coil
+            // of the group, bit 3 the component, then bits 2..0 sum it
+            float a[2 * FW_CB];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            acc.x += __shfl_down(acc.x, o, 64);
-            acc.y += __shfl_down(acc.y, o, 64);
-        }
-        if (lane == 0) {
-            const int64_t idx = sel ? sel[j] : j;
-            out[idx + (int64_t)c * nsel] = acc;
+            for (int cc = 0; cc < FW_CB; ++cc) {
+                a[2 * cc] = acc[cc].x;
+                a[2 * cc + 1] = acc[cc].y;
+            }
+#pragma unroll
+            for (int h = 2 * FW_CB / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+                const bool up = (lane & bit) != 0;
+#pragma unroll
+                for (int i = 0; i < h; ++i) {
+                    const float send = up ? a[i] : a[h + i];
+                    const float keep = up ? a[h + i] : a[i];
+                    a[i] = keep + __shfl_xor(send, bit, 64);
+                }
+            }
+#pragma unroll
+            for (int o = 4; o > 0; o >>= 1) a[0] += __shfl_xor(a[0], o, 64);
+            const int cc = lane >> 4, comp = (lane >> 3) & 1;
+            if ((lane & 7) == 0 && c0 + cc < C)
+                reinterpret_cast<float*>(out)[2 * (idx + (int64_t)(c0 + cc) * nsel) + comp] = a[0];
         }
     }
 }
@@ -211,7 +241,7 @@ void forward_model(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N, 
     const int64_t nsel = traj->nsamp * traj->nt;
     if (traj->sel.p) fill_floats(out, 2 * nsel * C, 0.f, st);  // masked-out samples are 0
     FwdGeom g{{nos[0], nos[1], nos[2]}};
-    const int64_t waves = traj->M * C;
+    const int64_t waves = traj->M;
     const int blocks = (int)std::min<int64_t>((waves * 64 + 255) / 256, 65536);
     if (d == 2)
         hipLaunchKernelGGL(interp_kernel<2>, dim3(blocks), dim3(256), 0, st, traj->kc.as<float>(),

@@ -113,10 +113,13 @@ def test_kernel_apply_and_constructors_match_oracle(torch, msr, N, K, C, nt, rea
     assert np.array_equal((D @ xd).cpu().numpy(), (A @ xd).cpu().numpy())
 
 
-@pytest.mark.parametrize("per_coil,use_U,three_d", [(False, True, False), (True, True, False),
-                                                    (False, False, False), (False, True, True)])
-def test_backprojection_matches_oracle(torch, msr, per_coil, use_U, three_d):
-    N, nt = ((16, 16, 24), 4) if three_d else ((32, 24), 6)
+# U = I(Nt) with 11 frames gives K = 11: the gridding's coefficient blocks of 8
+# run once whole and once partial
+@pytest.mark.parametrize("per_coil,use_U,three_d,nt", [(False, True, False, 6), (True, True, False, 6),
+                                                       (False, False, False, 6), (False, True, True, 4),
+                                                       (True, False, False, 11)])
+def test_backprojection_matches_oracle(torch, msr, per_coil, use_U, three_d, nt):
+    N = (16, 16, 24) if three_d else (32, 24)
     trj, mask, U, cm, x = make(N, 3, 3, nt, False, 0.6, three_d, seed=1)
     rng = np.random.default_rng(2)
     data = (rng.normal(size=(3,) + trj.shape[:2]) + 1j * rng.normal(size=(3,) + trj.shape[:2])).astype(np.complex64)
@@ -126,7 +129,7 @@ def test_backprojection_matches_oracle(torch, msr, per_coil, use_U, three_d):
                                        sample_mask=dev(torch, mask)).cpu().numpy()
     ref = ocart.backprojection(data, trj, N if per_coil else cm, U=U if use_U else None,
                                sample_mask=mask)
-    print(f"cartesian bp per_coil={per_coil} U={use_U} 3d={three_d}: rel err "
+    print(f"cartesian bp per_coil={per_coil} U={use_U} 3d={three_d} nt={nt}: rel err "
           f"{rel_err(out, ref):.2e}", flush=True)
     assert out.shape == ref.shape
     assert rel_err(out, ref) < 2e-6

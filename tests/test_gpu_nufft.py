@@ -289,19 +289,23 @@ def test_backprojection_and_kernel_are_bit_reproducible(torch, msr):
     assert np.array_equal(l1, l2)
 
 
-@pytest.mark.parametrize("masked", [False, True])
-def test_forward_model_matches_oracle_and_is_adjoint(torch, msr, masked):
+@pytest.mark.parametrize("masked,C", [(False, 3), (True, 3), (False, 6), (True, 9), (False, 0)])
+def test_forward_model_matches_oracle_and_is_adjoint(torch, msr, masked, C):
     """Forward (type-2) model vs the FP64 oracle, and <A x, y> = <x, A^H y> with
-    the device backprojection as A^H."""
+    the device backprojection as A^H.  Coil counts cover whole and partial
+    groups of the interpolation; C = 0 is no coil maps (one unit coil)."""
     N = (32, 24)
-    trj, U, cm, x = radial_case(N, 3, 3, 8, 3)
+    trj, U, cm, x = radial_case(N, 3, max(C, 1), 8, 3)
+    if C == 0:
+        cm = np.ones((1,) + N[::-1], dtype=np.complex64)
     mask = None
     if masked:
         mask = np.ones(trj.shape[:2], dtype=bool)
         mask[2, 10:50] = False
     md = None if mask is None else dev(torch, mask)
     cml = [dev(torch, c) for c in cm]
-    y = msr.forward(dev(torch, x), dev(torch, trj), cml, U=dev(torch, U), sample_mask=md).cpu().numpy()
+    y = msr.forward(dev(torch, x), dev(torch, trj), cml if C else None, U=dev(torch, U),
+                    sample_mask=md).cpu().numpy()
     ref = toeplitz.forward(x.astype(np.complex128), trj, U, cm.astype(np.complex128), sample_mask=mask)
     print(f"forward rel err {rel_err(y, ref):.3e}", flush=True)
     assert rel_err(y, ref) < 1e-5
