@@ -562,10 +562,7 @@ __global__ __launch_bounds__(256) void cart_grid_kernel(
             float2 acc[CART_KB];
 #pragma unroll
             for (int kk = 0; kk < CART_KB; ++kk) acc[kk] = make_float2(0.f, 0.f);
-            for (int e = e0; e < e1; ++e) {
-                const int j = perm[e];
-                const int t = frame[j];
-                const float2 y = yc[sel ? sel[j] : j];
+            auto add = [&](int t, float2 y) {
 #pragma unroll
                 for (int kk = 0; kk < CART_KB; ++kk) {
                     const int k = k0 + kk;
@@ -577,6 +574,25 @@ __global__ __launch_bounds__(256) void cart_grid_kernel(
                     else u = static_cast<const float2*>(U)[t + (int64_t)k * nt];
                     acc[kk] = cadd(acc[kk], cmulc(u, y));  // y * conj(u)
                 }
+            };
+            // four samples' loads issued together, then added in sample order
+            int e = e0;
+            for (; e + 4 <= e1; e += 4) {
+                int j[4], t[4];
+                float2 y[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) j[r] = perm[e + r];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    t[r] = frame[j[r]];
+                    y[r] = yc[sel ? sel[j[r]] : j[r]];
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) add(t[r], y[r]);
+            }
+            for (; e < e1; ++e) {
+                const int jj = perm[e];
+                add(frame[jj], yc[sel ? sel[jj] : jj]);
             }
 #pragma unroll
             for (int kk = 0; kk < CART_KB; ++kk)
