@@ -26,6 +26,11 @@
  *  - Handles are not re-entrant: one in-flight call per operator handle, as
  *    the reference's operator shares its scratch arrays
  *    (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:36-37).
+ *  - A trajectory handle may be shared: calls that use it from several
+ *    threads are serialised on the host, and a backprojection on another
+ *    stream than the handle's previous backprojection is ordered after it
+ *    (the handle's sorted-sample plans and backprojection workspace are
+ *    shared by all its calls).
  */
 #ifndef MSR_H
 #define MSR_H

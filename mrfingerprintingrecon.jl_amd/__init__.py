@@ -73,6 +73,35 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _check_vec(t, n, name, device, dtype):
+    """Raises ArgumentError unless t is a contiguous tensor of n elements of
+    dtype on device: the library reads and writes n elements through the
+    pointer, so a mismatch must stop here (the reference throws
+    DimensionMismatch from mul!)."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise ArgumentError(f"`{name}` must be a tensor")
+    if t.device != device:
+        raise ArgumentError(f"`{name}` is on {t.device}, the operator on {device}")
+    if t.dtype != dtype:
+        raise ArgumentError(f"`{name}` has dtype {t.dtype}, expected {dtype}")
+    if t.numel() != n:
+        raise ArgumentError(f"`{name}` has {t.numel()} elements, expected {n}")
+    if not t.is_contiguous():
+        raise ArgumentError(f"`{name}` must be contiguous")
+    return t
+
+
+def _check_cmaps(cl, img_shape):
+    """Every coil map must be an image of img_shape (row-major view
+    img_shape[::-1]); the library reads prod(img_shape) values of each."""
+    want = tuple(int(v) for v in img_shape)[::-1]
+    for i, c in enumerate(cl):
+        if tuple(c.shape) != want:
+            raise ArgumentError(f"cmaps[{i}] has shape {tuple(c.shape)}, expected {want} "
+                                f"(img_shape {tuple(img_shape)} reversed)")
+
+
 def _i64arr(vals):
     return (ctypes.c_int64 * len(vals))(*[int(v) for v in vals])
 
@@ -187,6 +216,10 @@ class _NFFTNormalOp:
 
     def mul_(self, x, b, alpha=1.0, beta=0.0, percoil=False):
         """mul!(x, S, b, α, β) (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:193-218)."""
+        torch = _torch()
+        n = math.prod(self.shape) * self.Ncoeff
+        _check_vec(x, n, "x", self.device, torch.complex64)
+        _check_vec(b, n, "b", self.device, torch.complex64)
         a = complex(alpha)
         bt = complex(beta)
         name = "msr_normalop_apply_percoil" if percoil else "msr_normalop_apply"
@@ -198,8 +231,9 @@ class _NFFTNormalOp:
         """mul! on ComplexF64 vectors with ComplexF64 arithmetic (the apply of
         the "f64" CG precision; the kernel and coil maps stay ComplexF32)."""
         torch = _torch()
-        if x.dtype != torch.complex128 or b.dtype != torch.complex128:
-            raise ArgumentError("mul_f64_ takes complex128 vectors")
+        n = math.prod(self.shape) * self.Ncoeff
+        _check_vec(x, n, "x", self.device, torch.complex128)
+        _check_vec(b, n, "b", self.device, torch.complex128)
         a = complex(alpha)
         bt = complex(beta)
         _lib.call("msr_normalop_apply_f64", self.h, _ptr(x), _ptr(b), a.real, a.imag, bt.real,
@@ -247,8 +281,10 @@ class LinearOperator:
 
     def __matmul__(self, v):
         torch = _torch()
+        v = v.contiguous()
+        _check_vec(v, self.ncol, "v", self.A.device, torch.complex64)
         out = torch.empty_like(v)
-        return self.prod(out, v.contiguous(), 1.0, 0.0)
+        return self.prod(out, v, 1.0, 0.0)
 
     def __repr__(self):
         return (f"Linear operator  nrow: {self.nrow}  ncol: {self.ncol}  eltype: ComplexF32  "
@@ -294,6 +330,7 @@ def NFFTNormalOp(img_shape, a, b, cmaps=None, sample_mask=None, verbose=False):
     img_shape = tuple(int(s) for s in img_shape)
     d = len(img_shape)
     cl = [_dev_tensor(c, "cmaps", torch.complex64) for c in _cmaps_list(cmaps)]
+    _check_cmaps(cl, img_shape)
     carr = (ctypes.c_void_p * max(1, len(cl)))(*[c.data_ptr() for c in cl]) if cl else None
     if isinstance(b, torch.Tensor) and b.dtype in (torch.int64, torch.int32) and b.dim() == 1:
         lam = _dev_tensor(a, "Λ")
@@ -339,6 +376,7 @@ def FFTNormalOp(a, b=None, c=None, cmaps=None, sample_mask=None, eltype_x=None):
                                 "for a non-Cartesian trajectory)")
         U, real, K = _basis(c, traj)
         dev = traj.device
+        _check_cmaps(cl, img_shape)
         _lib.call("msr_fftnormalop_create_from_traj", _ctx(dev.index), traj.h, len(img_shape),
                   _i64arr(img_shape), K, _ptr(U), int(real), len(cl), carr, _stream_ptr(dev),
                   ctypes.byref(h))
@@ -355,6 +393,7 @@ def FFTNormalOp(a, b=None, c=None, cmaps=None, sample_mask=None, eltype_x=None):
         if U.shape[1] != nt:
             raise ArgumentError(f"U has {U.shape[1]} time points, M {nt}")
         dev = M.device
+        _check_cmaps(cl, img_shape)
         _lib.call("msr_fftnormalop_create_from_mask", _ctx(dev.index), len(img_shape),
                   _i64arr(img_shape), nt, _ptr(M), U.shape[0], _ptr(U), int(real), len(cl), carr,
                   _stream_ptr(dev), ctypes.byref(h))
@@ -367,6 +406,7 @@ def FFTNormalOp(a, b=None, c=None, cmaps=None, sample_mask=None, eltype_x=None):
         real = not lam.is_complex()
         lam = lam.to(torch.float32 if real else torch.complex64).contiguous()
         dev = lam.device
+        _check_cmaps(cl, img_shape)
         _lib.call("msr_fftnormalop_create", _ctx(dev.index), len(img_shape), _i64arr(img_shape),
                   K, int(real), _ptr(lam), len(cl), carr, _stream_ptr(dev), ctypes.byref(h))
     return LinearOperator(_NFFTNormalOp(h, img_shape, dev, cl))
@@ -379,6 +419,7 @@ def NFFTNormalOp_shared(A, cmaps=None):
     torch = _torch()
     src = _op_state(A)
     cl = [_dev_tensor(c, "cmaps", torch.complex64) for c in _cmaps_list(cmaps)]
+    _check_cmaps(cl, src.shape)
     carr = (ctypes.c_void_p * max(1, len(cl)))(*[c.data_ptr() for c in cl]) if cl else None
     h = ctypes.c_void_p()
     _lib.call("msr_normalop_create_shared", _ctx(src.device.index), src.h, len(cl), carr,
@@ -557,7 +598,10 @@ def calculate_backprojection(data, trj, cmaps_or_img_shape, U=None, sample_mask=
         ncoil = None
     else:
         cl = [_dev_tensor(c, "cmaps", torch.complex64) for c in _cmaps_list(cmaps_or_img_shape)]
+        if not cl:
+            raise ArgumentError("calculate_backprojection needs coil maps or img_shape")
         img_shape = tuple(int(s) for s in cl[0].shape[::-1])
+        _check_cmaps(cl, img_shape)
         ncoil = len(cl)
     if U is None:
         K = traj.nt
@@ -750,6 +794,9 @@ def forward(x, trj, cmaps=None, U=None, sample_mask=None, out=None):
     traj = trj if isinstance(trj, _Traj) else _Traj(trj, sample_mask)
     cl = [_dev_tensor(c, "cmaps", torch.complex64) for c in _cmaps_list(cmaps)]
     img_shape = tuple(int(s) for s in x.shape[1:][::-1])
+    if len(img_shape) != traj.d:
+        raise ArgumentError(f"x has {len(img_shape)} image dimensions, trj {traj.d}")
+    _check_cmaps(cl, img_shape)
     if U is None:
         K, uptr, real = traj.nt, None, 0
     else:
@@ -766,6 +813,8 @@ def forward(x, trj, cmaps=None, U=None, sample_mask=None, out=None):
     shape = (C, traj.nt, traj.nsamp)
     if out is None:
         out = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    else:
+        _check_vec(out, math.prod(shape), "out", x.device, torch.complex64)
     carr = (ctypes.c_void_p * len(cl))(*[c.data_ptr() for c in cl]) if cl else None
     _lib.call("msr_forward", _ctx(x.device.index), traj.h, len(img_shape), _i64arr(img_shape),
               _ptr(x), K, uptr, real, len(cl), carr, _ptr(out), _stream_ptr(x.device))
@@ -802,9 +851,12 @@ def cg_(x, A, b, maxiter=None, reltol=REL_TOL_F32, abstol=0.0, initially_zero=Fa
     (ComplexF32 vectors and apply, the reference's arithmetic).  x and b are
     ComplexF32 in every precision."""
     op = A.A if isinstance(A, LinearOperator) else A
-    if maxiter is None:
-        maxiter = A.ncol if isinstance(A, LinearOperator) else math.prod(op.shape) * op.Ncoeff
     torch = _torch()
+    n = math.prod(op.shape) * op.Ncoeff
+    _check_vec(x, n, "x", op.device, torch.complex64)
+    _check_vec(b, n, "b", op.device, torch.complex64)
+    if maxiter is None:
+        maxiter = n
     it = ctypes.c_int()
     res = ctypes.c_double()
     cur = torch.cuda.current_stream(op.device)
@@ -867,10 +919,10 @@ class NFFTNormalOpBatch:
     def _ptrs(self, ts, name):
         if len(ts) != len(self.ops):
             raise ArgumentError(f"{name} has {len(ts)} vectors for {len(self.ops)} operators")
-        for t, A in zip(ts, self.ops):
-            n = math.prod(A.shape) * A.Ncoeff
-            if t.numel() != n or not t.is_contiguous() or t.device != self.device:
-                raise ArgumentError(f"{name}: expected contiguous device vectors of length {n}")
+        torch = _torch()
+        for i, (t, A) in enumerate(zip(ts, self.ops)):
+            _check_vec(t, math.prod(A.shape) * A.Ncoeff, f"{name}[{i}]", self.device,
+                       torch.complex64)
         return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
 
     def mul_(self, xs, bs, alpha=1.0, beta=0.0):
@@ -974,9 +1026,9 @@ class NFFTServer:
         of length V*K); returns the job id."""
         op = _op_state(A)
         n = math.prod(op.shape) * op.Ncoeff
+        torch = _torch()
         for t, name in ((x, "x"), (b, "b")):
-            if t.numel() != n or not t.is_contiguous() or t.device != self.device:
-                raise ArgumentError(f"{name}: expected a contiguous device vector of length {n}")
+            _check_vec(t, n, name, self.device, torch.complex64)
         jid = ctypes.c_int64()
         _lib.call("msr_server_submit", self.h, op.h, _ptr(x), _ptr(b), int(maxiter), float(reltol),
                   float(abstol), ctypes.byref(jid))

@@ -74,6 +74,11 @@ void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
         Gos *= nos[a];
     }
     const int nch = ncoils * K;
+    // the plan's point streams and the workspace belong to the handle: hold
+    // it for the call, and order this stream after the workspace's last user
+    std::lock_guard<std::recursive_mutex> lock(traj->mu);
+    if (traj->work_event && traj->work_stream != st)
+        MSR_HIP(hipStreamWaitEvent(st, traj->work_event, 0));
     const SpreadPlan& plan = spread_plan(traj, nos, st);
 
     ValueSrc src{};
@@ -193,6 +198,11 @@ void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
             MSR_CHECK_LAUNCH();
         }
     }
+    // the next user of the workspace on another stream waits for this call
+    if (!traj->work_event)
+        MSR_HIP(hipEventCreateWithFlags(&traj->work_event, hipEventDisableTiming));
+    MSR_HIP(hipEventRecord(traj->work_event, st));
+    traj->work_stream = st;
 }
 
 // The reference's call sequence (ext/MRISubspaceReconCUDAExt/BackProjection.jl:

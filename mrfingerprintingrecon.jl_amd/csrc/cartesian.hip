@@ -101,6 +101,7 @@ __global__ void cart_cells_kernel(const float* __restrict__ kc, int64_t M, int d
 // Masked samples sorted by cell (stable radix sort: a cell's samples keep
 // their sample order) and the cells' offsets, per (trajectory, N).
 const CartPlan& cart_plan(const msr_traj* traj, const int64_t* N, int d, hipStream_t st) {
+    std::lock_guard<std::recursive_mutex> lock(traj->mu);
     std::array<int64_t, 3> key{N[0], N[1], d == 3 ? N[2] : 1};
     auto it = traj->cart_plans.find(key);
     if (it != traj->cart_plans.end()) return *it->second;
@@ -651,9 +652,11 @@ void cart_backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_
     MSR_CHECK_LAUNCH();
     // inverse FFT with fftshifted output: Gd -> H (3D through Gd)
     cart_fft(ctx, d, N, nch, Gd, H, Gd, +1, SEL_CENTER, false, nullptr, st);
+    // the host staging array lives until the stream synchronisation below
     DevBuf ptrs;
+    std::vector<const float2*> hp;
     if (cmaps) {
-        std::vector<const float2*> hp(ncoils);
+        hp.resize(ncoils);
         for (int c = 0; c < ncoils; ++c) {
             MSR_REQUIRE(cmaps[c] != nullptr, MSR_ERR_ARGUMENT, "cmaps[%d] is NULL", c);
             hp[c] = (const float2*)cmaps[c];

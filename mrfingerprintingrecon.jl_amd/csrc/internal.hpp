@@ -109,6 +109,17 @@ struct msr_traj {
     mutable msr::DevBuf bp_work;      // backprojection workspace (grow-only)
     bool integer = false;             // created from integer (Cartesian) k-space indices
     mutable std::map<std::array<int64_t, 3>, std::unique_ptr<msr::CartPlan>> cart_plans;
+    // The plan maps and the workspace are shared by every call on the handle:
+    // `mu` serialises the host side of those calls (plan creation, workspace
+    // growth), and a backprojection on another stream than the workspace's
+    // last user waits for `work_event` (recorded after that user's kernels),
+    // so calls from several threads or streams on one handle do not race.
+    mutable std::recursive_mutex mu;
+    mutable hipEvent_t work_event = nullptr;
+    mutable hipStream_t work_stream = nullptr;
+    ~msr_traj() {
+        if (work_event) (void)hipEventDestroy(work_event);
+    }
 };
 
 struct msr_normalop {

@@ -85,25 +85,28 @@ void server_step(msr_server* s, int niter, int* n_active, int* n_queued, hipStre
         std::stable_sort(s->queue.begin(), s->queue.end(), [&](int64_t a, int64_t b) {
             return s->jobs[a].work < s->jobs[b].work;
         });
-    // admit (FIFO) while the jobs in flight fit the budget; a job alone always fits
-    std::vector<int64_t> admitted;
-    while (!s->queue.empty() && (int)(s->active.size() + admitted.size()) < s->max_active) {
-        ServerJob& j = s->jobs[s->queue.front()];
-        if (s->active_bytes + j.bytes > s->budget) break;
-        s->active_bytes += j.bytes;
-        admitted.push_back(s->queue.front());
-        s->queue.pop_front();
+    // admit (queue order) while the jobs in flight fit the budget; a job alone
+    // always fits (submit refuses jobs above the budget)
+    bool admitted = false;
+    try {
+        admitted = admit_jobs(s->queue, s->active, s->active_bytes, s->budget, s->max_active,
+                              [&](int64_t id) -> size_t { return s->jobs[id].bytes; },
+                              [&](int64_t id) {
+                                  ServerJob& j = s->jobs[id];
+                                  std::unique_ptr<msr_batch> one(batch_create(s->ctx, 1, &j.op));
+                                  batch_cg_start(one.get(), &j.x, &j.b, j.maxiter, j.reltol,
+                                                 j.abstol, true, s->prec, st);
+                                  // the start kernels read the batch's tables
+                                  MSR_HIP(hipStreamSynchronize(st));
+                                  j.state = MSR_JOB_RUNNING;
+                                  j.t_start = now_ms();
+                              });
+    } catch (...) {
+        // the jobs started before the failure are in flight: batch them
+        if (!s->active.empty()) rebuild_batch(s);
+        throw;
     }
-    for (int64_t id : admitted) {
-        ServerJob& j = s->jobs[id];
-        std::unique_ptr<msr_batch> one(batch_create(s->ctx, 1, &j.op));
-        batch_cg_start(one.get(), &j.x, &j.b, j.maxiter, j.reltol, j.abstol, true, s->prec, st);
-        MSR_HIP(hipStreamSynchronize(st));  // the start kernels read the batch's tables
-        j.state = MSR_JOB_RUNNING;
-        j.t_start = now_ms();
-        s->active.push_back(id);
-    }
-    if (!admitted.empty()) rebuild_batch(s);
+    if (admitted) rebuild_batch(s);
     if (s->batch) {
         batch_cg_iterate(s->batch, niter, st);
         // retire the jobs whose CG has stopped

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "server_admit.hpp"
 
 namespace msr {
 

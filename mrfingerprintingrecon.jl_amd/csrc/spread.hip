@@ -126,6 +126,7 @@ __global__ void iota_kernel(int* __restrict__ p, int64_t n) {
 // Points sorted by tile with a stable radix sort (ties keep sample order), so
 // the spreading order, and with it the FP32 result, is reproducible.
 const SpreadPlan& spread_plan(const msr_traj* traj, const int64_t* n, hipStream_t st) {
+    std::lock_guard<std::recursive_mutex> lock(traj->mu);
     std::array<int64_t, 3> key{n[0], traj->d > 1 ? n[1] : 1, traj->d > 2 ? n[2] : 1};
     auto it = traj->plans.find(key);
     if (it != traj->plans.end()) return *it->second;

@@ -102,10 +102,13 @@ def kmask_indcs(n, kc):
         iy = np.mod(cell[:, 1, None] + offs[None], n[1])
         marks[iy[:, :, None], ix[:, None, :]] = True
     else:
-        ix = np.mod(cell[:, 0, None] + offs[None], n[0])
-        iy = np.mod(cell[:, 1, None] + offs[None], n[1])
-        iz = np.mod(cell[:, 2, None] + offs[None], n[2])
-        marks[iz[:, :, None, None], iy[:, None, :, None], ix[:, None, None, :]] = True
+        # chunks of samples keep the (m, 8, 8, 8) index arrays small
+        for a in range(0, cell.shape[0], 65536):
+            cs = cell[a:a + 65536]
+            ix = np.mod(cs[:, 0, None] + offs[None], n[0])
+            iy = np.mod(cs[:, 1, None] + offs[None], n[1])
+            iz = np.mod(cs[:, 2, None] + offs[None], n[2])
+            marks[iz[:, :, None, None], iy[:, None, :, None], ix[:, None, None, :]] = True
     return np.flatnonzero(marks.reshape(-1)).astype(np.int64)
 
 

@@ -65,3 +65,29 @@ def test_host_api_rejects_host_arrays(msr):
     import numpy as np
     with pytest.raises(TypeError):
         msr.NFFTNormalOp((8, 8), np.zeros((2, 16, 2), np.float32), np.ones((1, 2), np.complex64))
+
+
+def test_vector_checks_reject_mismatches_before_any_call(msr):
+    """mul_, cg_, forward(out=...), the batch and the server check every vector
+    the library reads or writes (device, dtype, length, contiguity) and every
+    coil map's shape before a pointer crosses the boundary; exercised here on
+    CPU tensors against a CPU 'device' (the checks do not touch the GPU)."""
+    import torch
+    cpu = torch.device("cpu")
+    ok = torch.zeros(12, dtype=torch.complex64)
+    assert msr._check_vec(ok, 12, "x", cpu, torch.complex64) is ok
+    bad = [
+        (torch.zeros(11, dtype=torch.complex64), "elements"),
+        (torch.zeros(12, dtype=torch.complex128), "dtype"),
+        (torch.zeros(24, dtype=torch.complex64)[::2], "contiguous"),
+        (torch.zeros(12, dtype=torch.float32), "dtype"),
+    ]
+    for t, what in bad:
+        with pytest.raises(msr.ArgumentError, match=what):
+            msr._check_vec(t, 12, "x", cpu, torch.complex64)
+    with pytest.raises(msr.ArgumentError, match="tensor"):
+        msr._check_vec([0.0] * 12, 12, "x", cpu, torch.complex64)
+    # coil maps: (Ny, Nx) row-major views of img_shape (Nx, Ny)
+    msr._check_cmaps([torch.zeros(6, 4), torch.zeros(6, 4)], (4, 6))
+    with pytest.raises(msr.ArgumentError, match=r"cmaps\[1\]"):
+        msr._check_cmaps([torch.zeros(6, 4), torch.zeros(4, 6)], (4, 6))
