@@ -19,7 +19,13 @@ function (tests/test_oracle_*.py):
   x = mod(-2 pi k, 2 pi)).  That library is not vendored in the reference, so
   the rule is recalled, not read: PARITY UNPINNED against the reference for the
   exact footprint (pinned only by invariants: containment of all sample cells,
-  point symmetry for point-symmetric trajectories, size bounds).
+  point symmetry for point-symmetric trajectories, size bounds).  Effect size
+  (``kmask_sensitivity.py``, FP64 CG on the same data and unmasked kernel):
+  changing the rule to half-support 3 or 5, or to a symmetric 9-cell footprint,
+  moves the config-2 solution (256^2, 20 coils, K = 5, 25 iterations) by
+  3.3e-5 to 4.7e-5 and the config-1 solution (64^2, 8 coils, K = 4, 20
+  iterations) by 3.1e-4 to 4.4e-4; the other edge convention for samples on a
+  cell edge changes no cell at either size.
 * ``toeplitz.toeplitz_kernel`` (complex U): pinned by brute force (the masked
   operator equals A^H A from dense NUDFT matrices when the mask covers the
   grid), the Hermitian-per-voxel invariant and the K = 1 classic Toeplitz case.

@@ -6,8 +6,9 @@ The reference CPU path cannot run here (Julia is absent), so this restates its
 algorithm and cost structure in ComplexF32 with NumPy/SciPy:
   * apply: src/NFFTNormalOp.jl:227-267 -- per coil, per coefficient zero-pad
     S_c b_k into the 2N grid and FFT (threaded FFTs), per-voxel K x K multiply
-    over the kmask (full (K, K, Nkm) kernel), inverse FFT per coefficient,
-    crop and accumulate conj(S_c) times the result.
+    over the kmask (full (K, K, Nkm) kernel) split over the host threads as
+    the reference's @tasks loop splits it (:254), inverse FFT per
+    coefficient, crop and accumulate conj(S_c) times the result.
   * backprojection: src/BackProjection.jl:62-102 -- one type-1 NUFFT per
     (coefficient, coil): data * conj(U[t, k]), spread onto the sigma = 2 grid
     with an 8-point kernel (NonuniformFFTs.jl half-support 4), FFT, crop and
@@ -19,6 +20,7 @@ from __future__ import annotations
 
 import math
 import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import scipy.fft as sfft
@@ -38,6 +40,23 @@ class CpuNormalOp:
         self.kmask = np.asarray(kmask)
         self.cmaps = np.asarray(cmaps, dtype=np.complex64)
         self.workers = workers or os.cpu_count()
+        # voxel chunks of the K x K multiply, one task each (numpy's matmul
+        # loops release the GIL, so the threads run in parallel)
+        nkm = self.kmask.size
+        edges = np.linspace(0, nkm, self.workers + 1).astype(np.int64)
+        self.chunks = [slice(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:]) if b > a]
+        self.pool = ThreadPoolExecutor(max_workers=self.workers)
+
+    def _multiply(self, v):
+        """o[i, j] = sum_k Lam_i[j, k] v[i, k] over the kmask voxels, in
+        parallel voxel chunks."""
+        o = np.empty_like(v)
+
+        def run(sl):
+            o[sl] = np.matmul(v[sl, None, :], self.full[sl])[:, 0, :]
+
+        list(self.pool.map(run, self.chunks))
+        return o
 
     def apply(self, b):
         K = self.K
@@ -51,7 +70,7 @@ class CpuNormalOp:
             kl1[crop] = cm[None] * b
             kl1 = sfft.fftn(kl1, axes=axes, workers=self.workers)
             v = kl1.reshape(K, G)[:, self.kmask].T                       # (Nkm, K)
-            o = np.einsum("ikj,ik->ij", self.full, v)                    # row j = sum_k Lam[j,k] v_k
+            o = self._multiply(v)                                        # row j = sum_k Lam[j,k] v_k
             kl2 = np.zeros((K, G), dtype=np.complex64)
             kl2[:, self.kmask] = o.T
             img = sfft.ifftn(kl2.reshape((K,) + self.n[::-1]), axes=axes, workers=self.workers)
