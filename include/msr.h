@@ -196,6 +196,27 @@ int msr_cg(msr_normalop *op, void *x, const void *b, int maxiter, double reltol,
            int initially_zero, int mode, int precision, int *iters_out, double *residual_out,
            void *stream);
 
+/* msr_cg in MSR_CG_DEVICE mode that allocates nothing: no hipMalloc, hipFree
+ * or allocating library call, so that it can run under HIP graph capture or
+ * in loops that must not allocate.  No counterpart in the reference; for
+ * callers that cannot allocate device memory.  Same arguments and result as
+ * msr_cg (the same kernels in the same order).  The CG vectors, scalars and
+ * reduction partials live in one library-owned device array of
+ * MSR_CG_NOALLOC_BYTES per device (held from the library's load to the end
+ * of the process); the operator's own apply scratch must already be sized
+ * for `precision` (it is for MSR_PREC_F32 / MSR_PREC_MIXED from creation, and
+ * for MSR_PREC_F64 after any msr_normalop_apply_f64 or msr_cg at that
+ * precision).  ArgumentError if the solve needs more than
+ * MSR_CG_NOALLOC_BYTES (about 64 bytes per unknown at MSR_PREC_F64, so up to
+ * about 1 M unknowns, e.g. 256^2 x K = 16) or the apply scratch is not sized.
+ * Every call on one device shares that array (the start kernels rewrite
+ * everything a call reads): calls that overlap on one device, from any
+ * thread, stream or operator, race and give wrong results. */
+#define MSR_CG_NOALLOC_BYTES (64u << 20)
+int msr_cg_noalloc(msr_normalop *op, void *x, const void *b, int maxiter, double reltol,
+                   double abstol, int initially_zero, int precision, int *iters_out,
+                   double *residual_out, void *stream);
+
 /* msr_normalop_apply on ComplexF64 vectors with ComplexF64 arithmetic (the
  * apply of MSR_PREC_F64).  No counterpart in the reference's GPU extension,
  * which is ComplexF32 only; the reference's CPU mul! is generic in the element

@@ -70,6 +70,7 @@ SIGNATURES = {
     "msr_backprojection_channelwise": (_I, [_P, _P, _I, _P, _P, _I, _I, _P, _I, _P, _I, _P, _P, _P]),
     "msr_forward": (_I, [_P, _P, _I, _P, _P, _I, _P, _I, _I, _P, _P, _P]),
     "msr_cg": (_I, [_P, _P, _P, _I, _D, _D, _I, _I, _I, _P, _P, _P]),
+    "msr_cg_noalloc": (_I, [_P, _P, _P, _I, _D, _D, _I, _I, _P, _P, _P]),
     "msr_normalop_apply_f64": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),
     "msr_normalop_create_shared": (_I, [_P, _P, _I, _P, _P, _P]),
     "msr_normalop_footprint": (_I, [_P, _I, _P, _P]),

@@ -120,6 +120,7 @@ int msr_ctx_create(int device, msr_ctx** out) {
         c->device = device;
         try {
             ensure_device(c);
+            cg_noalloc_setup(device);  // device address of msr_cg_noalloc's scratch array
         } catch (...) {
             delete c;
             throw;
@@ -341,6 +342,21 @@ int msr_cg(msr_normalop* op, void* x, const void* b, int maxiter, double reltol,
         ensure_device(op->ctx);
         op_cg(op, (float2*)x, (const float2*)b, maxiter, reltol, abstol, initially_zero != 0, mode,
               precision, iters_out, residual_out, (hipStream_t)stream);
+    });
+}
+
+int msr_cg_noalloc(msr_normalop* op, void* x, const void* b, int maxiter, double reltol,
+                   double abstol, int initially_zero, int precision, int* iters_out,
+                   double* residual_out, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        MSR_NOTNULL(x);
+        MSR_NOTNULL(b);
+        MSR_REQUIRE(x != b, MSR_ERR_ARGUMENT, "x and b must not alias");
+        ensure_device(op->ctx);
+        op_cg_noalloc(op, (float2*)x, (const float2*)b, maxiter, reltol, abstol,
+                      initially_zero != 0, precision, iters_out, residual_out,
+                      (hipStream_t)stream);
     });
 }
 

@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "apply3d.hpp"
@@ -410,13 +412,13 @@ __global__ void f32_to_f64_kernel(const float2* __restrict__ a, double2* __restr
         b[i] = make_double2(a[i].x, a[i].y);
 }
 
-// one CG iteration with device scalars (all kernels skip once done is set)
+// one CG iteration with device scalars S and reduction partials part (all
+// kernels skip once done is set)
 template <class VT, class AT>
-static void cg_iteration(msr_normalop* op, const CgVecs<VT, AT>& v, hipStream_t st) {
+static void cg_iteration(msr_normalop* op, const CgVecs<VT, AT>& v, CgScalars* S, double* part,
+                         hipStream_t st) {
     constexpr bool a64 = std::is_same<AT, double2>::value;
     const int64_t n = op->V * op->K;
-    auto* S = op->cg_scal.as<CgScalars>();
-    double* part = op->cg_part.as<double>();
     launch_cg_u(v, n, S, st);
     op_passes(op, v.ua, a64, st, &S->done);
     launch_cg_reduce_dot(op->partial.as<AT>(), op->G, n, v, op_scale(op), part, S, st);
@@ -441,18 +443,15 @@ static void cg_apply_x0(msr_normalop* op, const CgVecs<VT, AT>& v, const float2*
 }
 
 template <class VT, class AT>
-static void cg_prologue(msr_normalop* op, const CgVecs<VT, AT>& v, float2* x, const float2* b,
-                        int maxiter, double reltol, double abstol, bool initially_zero,
-                        hipStream_t st) {
+static void cg_prologue(msr_normalop* op, const CgVecs<VT, AT>& v, CgScalars* S, double* part,
+                        float2* x, const float2* b, int maxiter, double reltol, double abstol,
+                        bool initially_zero, hipStream_t st) {
     const int64_t n = op->V * op->K;
-    auto* S = op->cg_scal.as<CgScalars>();
     if (initially_zero) {
-        launch_cg_start(b, x, v, n, true, op->cg_part.as<double>(), S, reltol, abstol, maxiter,
-                        false, st);
+        launch_cg_start(b, x, v, n, true, part, S, reltol, abstol, maxiter, false, st);
     } else {
         cg_apply_x0(op, v, x, st);
-        launch_cg_start(b, x, v, n, false, op->cg_part.as<double>(), S, reltol, abstol, maxiter,
-                        true, st);
+        launch_cg_start(b, x, v, n, false, part, S, reltol, abstol, maxiter, true, st);
     }
 }
 
@@ -466,9 +465,10 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
                    double abstol, bool initially_zero, int mode, int prec, hipStream_t st) {
     const CgVecs<VT, AT> v = op_cg_vecs<VT, AT>(op, x);
     auto* S = op->cg_scal.as<CgScalars>();
+    double* part = op->cg_part.as<double>();
     if (mode == MSR_CG_DEVICE) {
-        cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
-        for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, st);
+        cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, st);
+        for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, S, part, st);
         cg_epilogue(op, v, x, st);
     } else if (mode == MSR_CG_GRAPH) {
         // the captured kernels hold the buffer addresses: they are part of the
@@ -489,8 +489,8 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
             hipGraph_t g = nullptr;
             MSR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
-                for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, st);
+                cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, st);
+                for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, S, part, st);
                 cg_epilogue(op, v, x, st);
             } catch (...) {
                 (void)hipStreamEndCapture(st, &g);
@@ -518,8 +518,7 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
         // applies all coils at once).
         if constexpr (std::is_same<AT, float2>::value) {
             const int64_t n = op->V * op->K;
-            double* part = op->cg_part.as<double>();
-            cg_prologue(op, v, x, b, maxiter, reltol, abstol, initially_zero, st);
+            cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, st);
             CgScalars hs;
             for (int it = 0; it < maxiter; ++it) {
                 MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
@@ -562,6 +561,115 @@ void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double rel
     if (iters_out || res_out) {
         CgScalars hs;
         MSR_HIP(hipMemcpyAsync(&hs, op->cg_scal.p, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+        MSR_HIP(hipStreamSynchronize(st));
+        if (iters_out) *iters_out = hs.iter;
+        if (res_out) *res_out = hs.res;
+    }
+}
+
+// ------------------------------------------------- CG without allocation
+// Device memory of msr_cg_noalloc: the CG vectors, scalars and reduction
+// partials of one solve, carved from one array of CG_NOALLOC_BYTES that each
+// device holds from the library's load to the process end.  In host code the
+// array's name stands for a host-side shadow, not a device address: the
+// device address is looked up once per device (cg_noalloc_setup, from
+// msr_ctx_create with the device current) and only that address is used.
+
+}  // namespace msr
+
+static __device__ __align__(256) unsigned char scratch[msr::CG_NOALLOC_BYTES];
+
+namespace msr {
+
+static std::mutex g_noalloc_mu;
+static std::map<int, unsigned char*> g_noalloc_ptr;  // device -> address of `scratch`
+
+void cg_noalloc_setup(int device) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr.count(device)) return;
+    void* p = nullptr;
+    MSR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MSR_REQUIRE(p != nullptr, MSR_ERR_HIP, "no device address for the CG scratch array");
+    g_noalloc_ptr[device] = static_cast<unsigned char*>(p);
+}
+
+static unsigned char* noalloc_base(int device) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    auto it = g_noalloc_ptr.find(device);
+    MSR_REQUIRE(it != g_noalloc_ptr.end(), MSR_ERR_ERROR,
+                "CG scratch of device %d not set up (msr_ctx_create does it)", device);
+    return it->second;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of scratch a solve of n unknowns at precision prec takes.
+size_t cg_noalloc_bytes(int64_t n, int prec) {
+    const bool v64 = prec != MSR_PREC_F32, a64 = prec == MSR_PREC_F64;
+    const size_t vs = v64 ? sizeof(double2) : sizeof(float2);
+    const size_t es = a64 ? sizeof(double2) : sizeof(float2);
+    size_t b = 2 * align256(vs * n) + align256(es * n);        // r, u, c
+    if (v64) b += align256(sizeof(double2) * n);               // x in FP64
+    if (prec == MSR_PREC_MIXED) b += align256(sizeof(float2) * n);
+    return b + align256(sizeof(CgScalars)) + align256(sizeof(double) * 2 * RED_BLOCKS);
+}
+
+template <class VT, class AT>
+static void cg_noalloc_run(msr_normalop* op, unsigned char* base, float2* x, const float2* b,
+                           int maxiter, double reltol, double abstol, bool initially_zero,
+                           hipStream_t st, CgScalars** S_out) {
+    const size_t n = (size_t)op->V * op->K;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + off;
+        off += align256(bytes);
+        return p;
+    };
+    CgVecs<VT, AT> v;
+    v.r = reinterpret_cast<VT*>(take(sizeof(VT) * n));
+    v.u = reinterpret_cast<VT*>(take(sizeof(VT) * n));
+    v.c = reinterpret_cast<AT*>(take(sizeof(AT) * n));
+    if constexpr (std::is_same<VT, double2>::value) v.x = reinterpret_cast<double2*>(take(sizeof(double2) * n));
+    else v.x = x;
+    if constexpr (std::is_same<VT, AT>::value) v.ua = v.u;
+    else v.ua = reinterpret_cast<AT*>(take(sizeof(AT) * n));
+    auto* S = reinterpret_cast<CgScalars*>(take(sizeof(CgScalars)));
+    auto* part = reinterpret_cast<double*>(take(sizeof(double) * 2 * RED_BLOCKS));
+    // every buffer above is written by the start kernels before it is read
+    // (r, u, x and the scalars by cg_start; c, part by each iteration), so
+    // nothing carried over from an earlier call is used
+    cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, st);
+    for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, S, part, st);
+    cg_epilogue(op, v, x, st);
+    *S_out = S;
+}
+
+void op_cg_noalloc(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
+                   double abstol, bool initially_zero, int prec, int* iters_out, double* res_out,
+                   hipStream_t st) {
+    MSR_REQUIRE(prec == MSR_PREC_F32 || prec == MSR_PREC_MIXED || prec == MSR_PREC_F64,
+                MSR_ERR_ARGUMENT, "unknown CG precision %d", prec);
+    MSR_REQUIRE(maxiter >= 0, MSR_ERR_ARGUMENT, "maxiter = %d", maxiter);
+    const size_t need = cg_noalloc_bytes(op->V * op->K, prec);
+    MSR_REQUIRE(need <= CG_NOALLOC_BYTES, MSR_ERR_ARGUMENT,
+                "the solve needs %zu bytes of CG state, above the %zu bytes this entry point "
+                "holds (use msr_cg)", need, (size_t)CG_NOALLOC_BYTES);
+    const size_t es = prec == MSR_PREC_F64 ? sizeof(double2) : sizeof(float2);
+    MSR_REQUIRE(op->scratch_es >= es, MSR_ERR_ARGUMENT,
+                "the operator's apply scratch is not sized for this precision yet (this entry "
+                "point allocates nothing: run one msr_normalop_apply_f64 or msr_cg at this "
+                "precision first)");
+    unsigned char* base = noalloc_base(op->ctx->device);
+    CgScalars* S = nullptr;
+    if (prec == MSR_PREC_F64)
+        cg_noalloc_run<double2, double2>(op, base, x, b, maxiter, reltol, abstol, initially_zero, st, &S);
+    else if (prec == MSR_PREC_MIXED)
+        cg_noalloc_run<double2, float2>(op, base, x, b, maxiter, reltol, abstol, initially_zero, st, &S);
+    else
+        cg_noalloc_run<float2, float2>(op, base, x, b, maxiter, reltol, abstol, initially_zero, st, &S);
+    if (iters_out || res_out) {
+        CgScalars hs;
+        MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
         MSR_HIP(hipStreamSynchronize(st));
         if (iters_out) *iters_out = hs.iter;
         if (res_out) *res_out = hs.res;

@@ -28,6 +28,15 @@ void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double rel
            double abstol, bool initially_zero, int mode, int prec, int* iters_out,
            double* res_out, hipStream_t st);
 
+// Device-mode CG whose vectors, scalars and partials live in one static device
+// array per device (msr_cg_noalloc): no allocation of any kind.
+constexpr size_t CG_NOALLOC_BYTES = MSR_CG_NOALLOC_BYTES;  // include/msr.h
+void cg_noalloc_setup(int device);  // once per device, with the device current
+size_t cg_noalloc_bytes(int64_t n, int prec);
+void op_cg_noalloc(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
+                   double abstol, bool initially_zero, int prec, int* iters_out, double* res_out,
+                   hipStream_t st);
+
 // pieces used by the batch engine
 Apply2DArgs args2d(msr_normalop* op, bool f64);        // b = skip = nullptr
 void op_ensure_scratch(msr_normalop* op, bool f64);    // grow-only; not during capture
