@@ -36,7 +36,7 @@ K = 5
 NT = 1000
 NR = 512
 CG_ITERS = 25
-PRECISION = "f64"          # CG precision of the measured solve (FP64 vectors and apply)
+PRECISION = "f64c"         # CG precision of the measured solve (FP64 vectors and apply arithmetic, 8-byte apply intermediates)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -66,22 +66,26 @@ def make_inputs(msr, seed=1002):
     return trj, U, cm, np.ascontiguousarray(data)
 
 
-def apply_bytes(nkm, lam_real, f64):
-    """Algorithmic bytes of one fused apply: s*(2VK + 8CKV) + 8*2CV + Nkm*P*s_lam
-    (b read and x written once, coil maps read twice, the x-FFT and y-FFT
+def apply_bytes(nkm, lam_real, precision="f32"):
+    """Bytes of one fused apply: s_v*2VK + s_t*8CKV + 8*2CV + Nkm*P*s_lam (b
+    read and x written once, coil maps read twice, the x-FFT and y-FFT
     intermediates written and read once each, the kernel read once on its
-    mask), with s = 16 for the ComplexF64 apply (vectors and intermediates in
-    FP64) and 8 for the ComplexF32 one; coil maps and kernel are ComplexF32."""
+    mask).  precision "f32" gives the algorithmic bytes of the roofline
+    (ComplexF32 vectors and intermediates, s_v = s_t = 8); "f64" the bytes the
+    ComplexF64 apply stores (s_v = s_t = 16); "f64c" those of the apply with
+    compact intermediates (s_v = 16, s_t = 8).  Coil maps and kernel are
+    ComplexF32 in every case."""
     V = N_IMG[0] * N_IMG[1]
     P = K * (K + 1) // 2
     s_lam = 4 if lam_real else 8
-    s = 16 if f64 else 8
-    return s * (2 * V * K + 8 * COILS * K * V) + 8 * 2 * COILS * V + nkm * P * s_lam
+    s_v = 8 if precision == "f32" else 16
+    s_t = 16 if precision == "f64" else 8
+    return s_v * 2 * V * K + s_t * 8 * COILS * K * V + 8 * 2 * COILS * V + nkm * P * s_lam
 
 
 def read_traffic(key):
     """HBM bytes per apply from the committed rocprofv3 counter summary, if any
-    (line '<key> = <int>', key HBM_BYTES_PER_APPLY_F64 or _F32)."""
+    (line '<key> = <int>', key HBM_BYTES_PER_APPLY_F64, _F64C or _F32)."""
     p = os.path.join(ROOT, "profiles", "apply2d_256_c20_k5_counters.md")
     if not os.path.exists(p):
         return None
@@ -103,6 +107,7 @@ def percentile(v, q):
 
 
 def main():
+    global PRECISION
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -120,10 +125,14 @@ def main():
                     help="the operators beside the solve instead (forward model, Cartesian "
                          "operator, coil-map calibration) at the same size; prints its own "
                          "JSON line")
+    ap.add_argument("--precision", default=PRECISION, choices=["f64", "f64c"],
+                    help="CG precision of the measured solve (FP64 vectors and apply; f64c "
+                         "stores the apply's FFT intermediates in 8 bytes)")
     ap.add_argument("--apply-only", type=int, default=0, metavar="N",
                     help="profiling mode: build the operator, run N fused applies, N per-coil "
                          "applies and a 256 MiB calibration copy, then exit")
     args = ap.parse_args()
+    PRECISION = args.precision
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py measures one GPU: this build has no multi-GPU path "
                  "(a reconstruction shards by coil or by job; see DESIGN.md)")
@@ -163,6 +172,8 @@ def main():
         for _ in range(args.apply_only):
             A.A.mul_f64_(y64, x64)
         for _ in range(args.apply_only):
+            A.A.mul_f64_(y64, x64, compact=True)
+        for _ in range(args.apply_only):
             A.A.mul_(y, xin)
         for _ in range(args.apply_only):
             A.A.mul_(y, xin, percoil=True)
@@ -172,8 +183,8 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"apply_only": args.apply_only, "copy_bytes": 2 * src.numel() * 4,
-                          "apply_bytes_f64": apply_bytes(A.A.nkm, A.A.lambda_real, True),
-                          "apply_bytes_f32": apply_bytes(A.A.nkm, A.A.lambda_real, False),
+                          **{f"apply_bytes_{p}": apply_bytes(A.A.nkm, A.A.lambda_real, p)
+                             for p in ("f64", "f64c", "f32")},
                           "nkm": A.A.nkm}), flush=True)
         return
     b = torch.empty((K,) + N_IMG[::-1], dtype=torch.complex64, device=dev)
@@ -234,21 +245,34 @@ def main():
         torch.cuda.synchronize()
         return s_ev.elapsed_time(e_ev) / napp
 
-    def roofline_of(t_ms, f64, name):
-        B = apply_bytes(A.A.nkm, A.A.lambda_real, f64)  # kernel bytes counted on the kmask only
+    names = {"f64": "Toeplitz apply, ComplexF64 arithmetic and intermediates (3 fused passes "
+                    "+ epilogue)",
+             "f64c": "Toeplitz apply, ComplexF64 arithmetic, intermediates in 8 bytes (3 fused "
+                     "passes + epilogue)",
+             "f32": "Toeplitz apply, ComplexF32 (3 fused passes + epilogue)"}
+
+    def roofline_of(t_ms, precision):
+        # achieved = the algorithmic bytes of the apply (ComplexF32 vectors and
+        # intermediates, kernel on its kmask) per launch time; the bytes the
+        # precision actually stores are reported beside it
+        B = apply_bytes(A.A.nkm, A.A.lambda_real, "f32")
         achieved = B / (t_ms * 1e-3) / 1e9
+        Bs = apply_bytes(A.A.nkm, A.A.lambda_real, precision)
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": read_traffic("HBM_BYTES_PER_APPLY_F64" if f64 else "HBM_BYTES_PER_APPLY_F32"),
-                "kernel": name, "bytes_per_launch": B, "ms_per_launch": round(t_ms, 4)}
+                "traffic": read_traffic("HBM_BYTES_PER_APPLY_" + precision.upper()),
+                "kernel": names[precision], "bytes_per_launch": B, "ms_per_launch": round(t_ms, 4),
+                "stored_bytes_per_launch": Bs,
+                "stored_frac": round(Bs / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
     x64 = xin.to(torch.complex128)
     y64 = torch.empty_like(x64)
     y = torch.empty_like(xin)
-    t64 = time_apply(lambda: A.A.mul_f64_(y64, x64))
-    t32 = time_apply(lambda: A.A.mul_(y, xin))
-    roofline = roofline_of(t64, True, "Toeplitz apply, ComplexF64 (3 fused passes + epilogue)")
-    roofline_f32 = roofline_of(t32, False, "Toeplitz apply, ComplexF32 (3 fused passes + epilogue)")
+    t_app = {"f64": time_apply(lambda: A.A.mul_f64_(y64, x64)),
+             "f64c": time_apply(lambda: A.A.mul_f64_(y64, x64, compact=True)),
+             "f32": time_apply(lambda: A.A.mul_(y, xin))}
+    roofline = roofline_of(t_app[PRECISION], PRECISION)
+    roofline_other = {p: roofline_of(t, p) for p, t in t_app.items() if p != PRECISION}
     # plain (per-coil) apply for comparison
     s_ev.record()
     for _ in range(5):
@@ -268,8 +292,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 apply and CG vectors (complex128 FFTs / kernel multiply / intermediates); "
-                 "f32 kernel, coil maps, gridding and backprojection",
+        "dtype": ("f64 apply and CG vectors (complex128 FFTs / kernel multiply / intermediates); "
+                  "f32 kernel, coil maps, gridding and backprojection") if PRECISION == "f64" else
+                 ("f64 apply arithmetic and CG vectors, apply intermediates stored in 8 bytes "
+                  "(shared exponent, 28-bit mantissas); f32 kernel, coil maps, gridding and "
+                  "backprojection"),
         "data": "synthetic",
         "config": {"workload": "single 2D recon 256x256 (grid 512x512), 20 coils, 1000 frames x "
                                "1 golden-angle spoke x 512 samples, K=5, BP + 25-iteration CG",
@@ -285,7 +312,7 @@ def main():
                       "apply_ms": round(t_apply_plain, 4)},
         "kernel_build_ms": round(build_ms, 1),
         "roofline": roofline,
-        "roofline_f32_apply": roofline_f32,
+        "roofline_other_applies": roofline_other,
         "cpu_baseline": None,
         "rel_err_vs_fp64": None,
     }

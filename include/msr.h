@@ -67,6 +67,10 @@ extern "C" {
 #define MSR_PREC_MIXED 1 /* FP64 vectors, ComplexF32 apply on an FP32 copy of the search vector */
 #define MSR_PREC_F64 2   /* FP64 vectors and ComplexF64 apply (FFTs, kernel multiply and
                             intermediates in FP64; kernel and coil maps stored as ComplexF32) */
+#define MSR_PREC_F64C 3  /* as MSR_PREC_F64, with the 2D apply's FFT intermediates stored in
+                            8 bytes per complex value (a shared exponent and two 28-bit
+                            mantissas: relative rounding 2^-27, 8x below ComplexF32 storage);
+                            3D and Cartesian operators keep ComplexF64 intermediates */
 
 typedef struct msr_ctx msr_ctx;
 typedef struct msr_traj msr_traj;
@@ -223,6 +227,13 @@ int msr_cg_noalloc(msr_normalop *op, void *x, const void *b, int maxiter, double
  * type (src/NFFTNormalOp.jl:227-267). */
 int msr_normalop_apply_f64(msr_normalop *op, void *x, const void *b, double alpha_re,
                            double alpha_im, double beta_re, double beta_im, void *stream);
+
+/* msr_normalop_apply_f64 with the intermediates of MSR_PREC_F64C (the apply
+ * msr_cg runs at that precision): ComplexF64 vectors and arithmetic, FFT
+ * intermediates stored in 8 bytes per value.  No counterpart in the
+ * reference (see msr_normalop_apply_f64). */
+int msr_normalop_apply_f64c(msr_normalop *op, void *x, const void *b, double alpha_re,
+                            double alpha_im, double beta_re, double beta_im, void *stream);
 
 /* -------------------------------------------------------------- batches */
 /* Many reconstructions (uneven image size, coil count and K, 2D and 3D)

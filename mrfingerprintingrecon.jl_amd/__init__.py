@@ -227,17 +227,19 @@ class _NFFTNormalOp:
                   _stream_ptr(self.device))
         return x
 
-    def mul_f64_(self, x, b, alpha=1.0, beta=0.0):
+    def mul_f64_(self, x, b, alpha=1.0, beta=0.0, compact=False):
         """mul! on ComplexF64 vectors with ComplexF64 arithmetic (the apply of
-        the "f64" CG precision; the kernel and coil maps stay ComplexF32)."""
+        the "f64" CG precision; the kernel and coil maps stay ComplexF32).
+        compact=True stores the FFT intermediates in 8 bytes per value (the
+        apply of the "f64c" precision)."""
         torch = _torch()
         n = math.prod(self.shape) * self.Ncoeff
         _check_vec(x, n, "x", self.device, torch.complex128)
         _check_vec(b, n, "b", self.device, torch.complex128)
         a = complex(alpha)
         bt = complex(beta)
-        _lib.call("msr_normalop_apply_f64", self.h, _ptr(x), _ptr(b), a.real, a.imag, bt.real,
-                  bt.imag, _stream_ptr(self.device))
+        _lib.call("msr_normalop_apply_f64c" if compact else "msr_normalop_apply_f64", self.h,
+                  _ptr(x), _ptr(b), a.real, a.imag, bt.real, bt.imag, _stream_ptr(self.device))
         return x
 
     def __del__(self):
@@ -847,9 +849,10 @@ def cg_(x, A, b, maxiter=None, reltol=REL_TOL_F32, abstol=0.0, initially_zero=Fa
     (device-resident scalars), "graph" (hipGraph-captured iterations) or
     "host" (scalars read back every iteration, per-coil applies: the plain
     reference-order GPU path).  precision: "f64" (FP64 vectors and a
-    ComplexF64 apply), "mixed" (FP64 vectors, ComplexF32 apply) or "f32"
-    (ComplexF32 vectors and apply, the reference's arithmetic).  x and b are
-    ComplexF32 in every precision."""
+    ComplexF64 apply), "f64c" (the same with the 2D apply's FFT intermediates
+    stored in 8 bytes per value), "mixed" (FP64 vectors, ComplexF32 apply) or
+    "f32" (ComplexF32 vectors and apply, the reference's arithmetic).  x and b
+    are ComplexF32 in every precision."""
     op = A.A if isinstance(A, LinearOperator) else A
     torch = _torch()
     n = math.prod(op.shape) * op.Ncoeff

@@ -21,7 +21,7 @@ MSR_ERR_IO = 7
 DCF_MODES = {"none": 0, "radial_2D": 1, "radial_3D": 2}
 DCF_ARRAY = 3
 CG_MODES = {"device": 0, "host": 1, "graph": 2}
-CG_PRECISIONS = {"f32": 0, "mixed": 1, "f64": 2}
+CG_PRECISIONS = {"f32": 0, "mixed": 1, "f64": 2, "f64c": 3}
 
 
 class ArgumentError(ValueError):
@@ -72,6 +72,7 @@ SIGNATURES = {
     "msr_cg": (_I, [_P, _P, _P, _I, _D, _D, _I, _I, _I, _P, _P, _P]),
     "msr_cg_noalloc": (_I, [_P, _P, _P, _I, _D, _D, _I, _I, _P, _P, _P]),
     "msr_normalop_apply_f64": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),
+    "msr_normalop_apply_f64c": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),
     "msr_normalop_create_shared": (_I, [_P, _P, _I, _P, _P, _P]),
     "msr_normalop_footprint": (_I, [_P, _I, _P, _P]),
     "msr_kernel_key": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, _P]),

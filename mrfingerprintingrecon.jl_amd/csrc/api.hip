@@ -330,6 +330,20 @@ int msr_normalop_apply_f64(msr_normalop* op, void* x, const void* b, double alph
     });
 }
 
+int msr_normalop_apply_f64c(msr_normalop* op, void* x, const void* b, double alpha_re,
+                            double alpha_im, double beta_re, double beta_im, void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        MSR_NOTNULL(x);
+        MSR_NOTNULL(b);
+        MSR_REQUIRE(x != b, MSR_ERR_ARGUMENT, "x and b must not alias");
+        ensure_device(op->ctx);
+        op_apply_f64(op, (double2*)x, (const double2*)b, make_double2(alpha_re, alpha_im),
+                     make_double2(beta_re, beta_im), beta_re == 0.0 && beta_im == 0.0,
+                     (hipStream_t)stream, AM_F64C);
+    });
+}
+
 int msr_cg(msr_normalop* op, void* x, const void* b, int maxiter, double reltol, double abstol,
            int initially_zero, int mode, int precision, int* iters_out, double* residual_out,
            void* stream) {
@@ -457,8 +471,7 @@ int msr_normalop_footprint(const msr_normalop* op, int precision, size_t* own_by
                            size_t* kernel_bytes) {
     return guarded([&] {
         MSR_NOTNULL(op);
-        MSR_REQUIRE(precision >= MSR_PREC_F32 && precision <= MSR_PREC_F64, MSR_ERR_ARGUMENT,
-                    "precision = %d", precision);
+        MSR_REQUIRE(prec_valid(precision), MSR_ERR_ARGUMENT, "precision = %d", precision);
         if (own_bytes) *own_bytes = op_footprint(op, precision);
         if (kernel_bytes) *kernel_bytes = op_kernel_bytes(op);
     });

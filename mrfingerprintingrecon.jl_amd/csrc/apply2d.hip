@@ -7,10 +7,11 @@
 
 namespace msr {
 
-template <int NX2, class CT>
+template <int NX2, class CT, class ST>
 constexpr int rowpass_block() {
-    return (rows_per_tile<CT>() * FFTPlan<NX2>::T) < 256 ? rows_per_tile<CT>() * FFTPlan<NX2>::T
-                                                         : 256;
+    return (rows_per_tile<CT, ST>() * FFTPlan<NX2>::T) < 256
+               ? rows_per_tile<CT, ST>() * FFTPlan<NX2>::T
+               : 256;
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
@@ -24,19 +25,19 @@ __device__ __forceinline__ int64_t xcd_order(int64_t lb, int64_t nblk) {
 }
 
 // ----------------------------------------------------------------- pass A
-// block lb = (tile, k, c), tile fastest
-template <int NX2, class CT>
+// block lb = (tile, k, c), tile fastest; CT arithmetic, ST storage of T
+template <int NX2, class CT, class ST>
 __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     if (a.skip && *a.skip) return;
-    constexpr int RT = rows_per_tile<CT>();
+    constexpr int RT = rows_per_tile<CT, ST>();
     constexpr int TP = FFTPlan<NX2>::T;
-    constexpr int BLK = rowpass_block<NX2, CT>();
+    constexpr int BLK = rowpass_block<NX2, CT, ST>();
     constexpr int F = BLK / TP;              // rows transformed concurrently
     constexpr int LD = lds_line<NX2>() + 1;  // LDS row pitch (odd: transposed access)
     __shared__ CT s_tw[NX2];
     __shared__ CT s_rows[RT * LD];
     const int nx = a.nx, ny = a.ny, K = a.K;
-    const int tiles = tiles2d(ny, sizeof(CT) == 16);
+    const int tiles = tiles2d(ny, RT);
     lb = xcd_order(lb, (int64_t)tiles * K * a.C);
     const int tile = (int)(lb % tiles);
     const int k = (int)((lb / tiles) % K);
@@ -62,34 +63,34 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
 #pragma unroll
     for (int r0 = 0; r0 < RT; r0 += F) fft_lds<NX2, -1>(s_rows + (r0 + g) * LD, s_tw, t);
     // transposed write: T[c][k][qx][y], y fastest
-    CT* Tk = static_cast<CT*>(a.T) + ((size_t)c * K + k) * NX2 * ny;
+    ST* Tk = static_cast<ST*>(a.T) + ((size_t)c * K + k) * NX2 * ny;
     for (int e = tid; e < RT * NX2; e += BLK) {
         const int yl = e % RT, qx = e / RT;
         const int y = y0 + yl;
-        if (y < ny) Tk[(size_t)qx * ny + y] = s_rows[yl * LD + lds_pad(qx)];
+        if (y < ny) st_store<CT, ST>(Tk + (size_t)qx * ny + y, s_rows[yl * LD + lds_pad(qx)]);
     }
 }
 
-template <int NX2, class CT>
+template <int NX2, class CT, class ST>
 __global__ void __launch_bounds__(256) pass_a_2d(Apply2DArgs a) {
-    pass_a_body<NX2, CT>(a, blockIdx.x);
+    pass_a_body<NX2, CT, ST>(a, blockIdx.x);
 }
 
-template <int NX2, class CT>
+template <int NX2, class CT, class ST>
 __global__ void __launch_bounds__(256)
 pass_a_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pre, int nj) {
     const int j = batch_job(pre, nj, blockIdx.x);
-    pass_a_body<NX2, CT>(jobs[j], (int64_t)blockIdx.x - pre[j]);
+    pass_a_body<NX2, CT, ST>(jobs[j], (int64_t)blockIdx.x - pre[j]);
 }
 
 // ----------------------------------------------------------------- pass C
 // block lb = (tile, k, coil group), tile fastest
-template <int NX2, class CT>
+template <int NX2, class CT, class ST>
 __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     if (a.skip && *a.skip) return;
-    constexpr int RT = rows_per_tile<CT>();
+    constexpr int RT = rows_per_tile<CT, ST>();
     constexpr int TP = FFTPlan<NX2>::T;
-    constexpr int BLK = rowpass_block<NX2, CT>();
+    constexpr int BLK = rowpass_block<NX2, CT, ST>();
     constexpr int F = BLK / TP;
     constexpr int LD = lds_line<NX2>() + 1;
     constexpr int NXC = NX2 / 2;                 // = nx
@@ -97,7 +98,7 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     __shared__ CT s_tw[NX2];
     __shared__ CT s_rows[RT * LD];
     const int nx = a.nx, ny = a.ny, K = a.K, C = a.C, G = a.G;
-    const int tiles = tiles2d(ny, sizeof(CT) == 16);
+    const int tiles = tiles2d(ny, RT);
     lb = xcd_order(lb, (int64_t)tiles * K * G);
     const int tile = (int)(lb % tiles);
     const int k = (int)((lb / tiles) % K);
@@ -113,15 +114,16 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     // the tile's rows of one coil, NLE elements per thread, prefetched into
     // registers one coil ahead
     constexpr int NLE = (RT * NX2 + BLK - 1) / BLK;
-    CT pre[NLE];
+    ST pre[NLE];  // kept in the storage type (half the registers for cx64)
     auto prefetch = [&](int c) {
-        const CT* Tk = static_cast<const CT*>(a.T) + ((size_t)c * K + k) * NX2 * ny;
+        const ST* Tk = static_cast<const ST*>(a.T) + ((size_t)c * K + k) * NX2 * ny;
 #pragma unroll
         for (int r = 0; r < NLE; ++r) {
             const int e = tid + r * BLK;
             const int yl = e % RT, qx = e / RT;
             const int y = y0 + yl;
-            pre[r] = (e < RT * NX2 && y < ny) ? Tk[(size_t)qx * ny + y] : cz<CT>();
+            if (e < RT * NX2 && y < ny) pre[r] = Tk[(size_t)qx * ny + y];
+            else pre[r] = ST{};
         }
     };
     if (c0 < c1) prefetch(c0);
@@ -129,7 +131,7 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
 #pragma unroll
         for (int r = 0; r < NLE; ++r) {
             const int e = tid + r * BLK;
-            if (e < RT * NX2) s_rows[(e % RT) * LD + lds_pad(e / RT)] = pre[r];
+            if (e < RT * NX2) s_rows[(e % RT) * LD + lds_pad(e / RT)] = st_conv<CT, ST>(pre[r]);
         }
         __syncthreads();
         if (c + 1 < c1) prefetch(c + 1);  // in flight during this coil's transforms
@@ -163,16 +165,16 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     }
 }
 
-template <int NX2, class CT>
+template <int NX2, class CT, class ST>
 __global__ void __launch_bounds__(256) pass_c_2d(Apply2DArgs a) {
-    pass_c_body<NX2, CT>(a, blockIdx.x);
+    pass_c_body<NX2, CT, ST>(a, blockIdx.x);
 }
 
-template <int NX2, class CT>
+template <int NX2, class CT, class ST>
 __global__ void __launch_bounds__(256)
 pass_c_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pre, int nj) {
     const int j = batch_job(pre, nj, blockIdx.x);
-    pass_c_body<NX2, CT>(jobs[j], (int64_t)blockIdx.x - pre[j]);
+    pass_c_body<NX2, CT, ST>(jobs[j], (int64_t)blockIdx.x - pre[j]);
 }
 
 // ----------------------------------------------------------------- dispatch
@@ -186,13 +188,23 @@ static unsigned grid1d(int64_t nblocks) {
 template <int N>
 constexpr bool row_len_ok() { return N <= 512; }
 
-template <class CT>
+// Calls f.template operator()<CT, ST>() for the arithmetic / storage pair of
+// the apply mode.
+template <class F>
+static void dispatch_mode(int f64, int compact, F&& f) {
+    if (!f64) f.template operator()<float2, float2>();
+    else if (compact) f.template operator()<double2, cx64>();
+    else f.template operator()<double2, double2>();
+}
+
 static void launch_a(const Apply2DArgs& a, hipStream_t st) {
     const int NX2 = 2 * a.nx;
     bool ok = dispatch_fft_len(NX2, [&]<int N>() {
         if constexpr (row_len_ok<N>())
-            hipLaunchKernelGGL((pass_a_2d<N, CT>), dim3(grid1d(blocks_a2d(a))),
-                               dim3(rowpass_block<N, CT>()), 0, st, a);
+            dispatch_mode(a.f64, a.compact, [&]<class CT, class ST>() {
+                hipLaunchKernelGGL((pass_a_2d<N, CT, ST>), dim3(grid1d(blocks_a2d(a))),
+                                   dim3(rowpass_block<N, CT, ST>()), 0, st, a);
+            });
         else
             fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Nx = %d above 512", N);
     });
@@ -200,22 +212,21 @@ static void launch_a(const Apply2DArgs& a, hipStream_t st) {
     MSR_CHECK_LAUNCH();
 }
 
-template <class CT>
 static void launch_c(const Apply2DArgs& a, hipStream_t st) {
     dispatch_fft_len(2 * a.nx, [&]<int N>() {
         if constexpr (row_len_ok<N>())
-            hipLaunchKernelGGL((pass_c_2d<N, CT>), dim3(grid1d(blocks_c2d(a))),
-                               dim3(rowpass_block<N, CT>()), 0, st, a);
+            dispatch_mode(a.f64, a.compact, [&]<class CT, class ST>() {
+                hipLaunchKernelGGL((pass_c_2d<N, CT, ST>), dim3(grid1d(blocks_c2d(a))),
+                                   dim3(rowpass_block<N, CT, ST>()), 0, st, a);
+            });
     });
     MSR_CHECK_LAUNCH();
 }
 
 void launch_apply2d_passes(const Apply2DArgs& a, hipStream_t st) {
-    if (a.f64) launch_a<double2>(a, st);
-    else launch_a<float2>(a, st);
+    launch_a(a, st);
     launch_pass_b_2d(a, st);
-    if (a.f64) launch_c<double2>(a, st);
-    else launch_c<float2>(a, st);
+    launch_c(a, st);
 }
 
 int pass_b_colblocks(int nx2, int ny2, int K, int f64) {
@@ -226,55 +237,46 @@ int pass_b_colblocks(int nx2, int ny2, int K, int f64) {
     return r;
 }
 
-void launch_pass_b_2d_batch(int ny2, int K, int lam_real, int f64, const Apply2DArgs* jobs,
+void launch_pass_b_2d_batch(int ny2, int K, int lam_real, int am, const Apply2DArgs* jobs,
                             const int* pre, int nj, int64_t nblocks, hipStream_t st) {
-    // the template arguments come from (ny2, K, lam_real, f64); the job table
+    // the template arguments come from (ny2, K, lam_real, am); the job table
     // itself stays on the device
     Apply2DArgs shape{};
     shape.ny = ny2 / 2;
     shape.K = K;
     shape.lam_real = lam_real;
-    shape.f64 = f64;
+    shape.f64 = am != AM_F32;
+    shape.compact = am == AM_F64C;
 #define MSR_CALLB(KK) pass_b_2d_k##KK(shape, jobs, pre, nj, nblocks, st)
     MSR_PASSB_SWITCH(K, MSR_CALLB)
 #undef MSR_CALLB
     MSR_CHECK_LAUNCH();
 }
 
-template <class CT>
-static void launch_a_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
-                           int64_t nblocks, hipStream_t st) {
+void launch_pass_a_2d_batch(int nx2, int am, const Apply2DArgs* jobs, const int* pre, int nj,
+                            int64_t nblocks, hipStream_t st) {
     bool ok = dispatch_fft_len(nx2, [&]<int N>() {
         if constexpr (row_len_ok<N>())
-            hipLaunchKernelGGL((pass_a_2d_batch<N, CT>), dim3(grid1d(nblocks)),
-                               dim3(rowpass_block<N, CT>()), 0, st, jobs, pre, nj);
+            dispatch_mode(am != AM_F32, am == AM_F64C, [&]<class CT, class ST>() {
+                hipLaunchKernelGGL((pass_a_2d_batch<N, CT, ST>), dim3(grid1d(nblocks)),
+                                   dim3(rowpass_block<N, CT, ST>()), 0, st, jobs, pre, nj);
+            });
     });
     MSR_REQUIRE(ok && nx2 <= 512, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", nx2);
     MSR_CHECK_LAUNCH();
 }
 
-template <class CT>
-static void launch_c_batch(int nx2, const Apply2DArgs* jobs, const int* pre, int nj,
-                           int64_t nblocks, hipStream_t st) {
+void launch_pass_c_2d_batch(int nx2, int am, const Apply2DArgs* jobs, const int* pre, int nj,
+                            int64_t nblocks, hipStream_t st) {
     bool ok = dispatch_fft_len(nx2, [&]<int N>() {
         if constexpr (row_len_ok<N>())
-            hipLaunchKernelGGL((pass_c_2d_batch<N, CT>), dim3(grid1d(nblocks)),
-                               dim3(rowpass_block<N, CT>()), 0, st, jobs, pre, nj);
+            dispatch_mode(am != AM_F32, am == AM_F64C, [&]<class CT, class ST>() {
+                hipLaunchKernelGGL((pass_c_2d_batch<N, CT, ST>), dim3(grid1d(nblocks)),
+                                   dim3(rowpass_block<N, CT, ST>()), 0, st, jobs, pre, nj);
+            });
     });
     MSR_REQUIRE(ok && nx2 <= 512, MSR_ERR_UNSUPPORTED, "unsupported grid size 2Nx = %d", nx2);
     MSR_CHECK_LAUNCH();
-}
-
-void launch_pass_a_2d_batch(int nx2, int f64, const Apply2DArgs* jobs, const int* pre, int nj,
-                            int64_t nblocks, hipStream_t st) {
-    if (f64) launch_a_batch<double2>(nx2, jobs, pre, nj, nblocks, st);
-    else launch_a_batch<float2>(nx2, jobs, pre, nj, nblocks, st);
-}
-
-void launch_pass_c_2d_batch(int nx2, int f64, const Apply2DArgs* jobs, const int* pre, int nj,
-                            int64_t nblocks, hipStream_t st) {
-    if (f64) launch_c_batch<double2>(nx2, jobs, pre, nj, nblocks, st);
-    else launch_c_batch<float2>(nx2, jobs, pre, nj, nblocks, st);
 }
 
 }  // namespace msr

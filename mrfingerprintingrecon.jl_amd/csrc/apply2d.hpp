@@ -30,14 +30,20 @@
 
 namespace msr {
 
-// Arithmetic and intermediates are ComplexF32 (f64 = 0) or ComplexF64
-// (f64 = 1); the kernel and the coil maps are ComplexF32 in both cases
-// (promoted on load).  b, T, partial and the twiddles have the element type of
-// the arithmetic.
+// Apply arithmetic and intermediate storage: ComplexF32 throughout,
+// ComplexF64 throughout, or ComplexF64 arithmetic with the FFT intermediates
+// in the compact 64-bit storage (cx64, common.hpp).
+enum ApplyMode { AM_F32 = 0, AM_F64 = 1, AM_F64C = 2 };
+
+// Arithmetic is ComplexF32 (f64 = 0) or ComplexF64 (f64 = 1); the
+// intermediate T has the arithmetic type, or cx64 when compact (with f64).
+// The kernel and the coil maps are ComplexF32 in every case (promoted on
+// load).  b, partial and the twiddles have the element type of the arithmetic.
 struct Apply2DArgs {
     int nx, ny;           // image size
     int K, C;             // coefficients, coils (C >= 1)
-    int f64;              // arithmetic / intermediate precision
+    int f64;              // arithmetic precision
+    int compact;          // T stored as cx64 (with f64)
     const void* b;        // input (K, ny, nx)
     const float2* cmaps;  // (C, ny, nx) or nullptr for the identity map
     const void* lam;      // dense kernel (2nx, P, 2ny), float2 or float
@@ -52,19 +58,31 @@ struct Apply2DArgs {
 };
 
 // rows per tile in passes A and C: 64-byte segments of the transposed
-// intermediate (8 ComplexF32 or 4 ComplexF64 rows), which keeps a 512-point
-// tile near 40 KB of LDS (three workgroups per CU)
-template <class CT>
-constexpr int rows_per_tile() { return 64 / (int)sizeof(CT); }
-__host__ __device__ inline int tiles2d(int ny, int f64) {
-    const int rt = f64 ? 4 : 8;
-    return (ny + rt - 1) / rt;
+// intermediate for ComplexF32 (8 rows) and ComplexF64 (4 rows); with compact
+// storage the rows are ComplexF64 in LDS, and 4 rows (32-byte segments, whose
+// 128-byte lines neighbouring tiles on the same XCD complete in L2) keep three
+// workgroups per CU, against one for 8 rows (measured 0.30 against 0.41 ms
+// per apply at 256^2, 20 coils, K = 5)
+#ifndef MSR_CX_ROWS
+#define MSR_CX_ROWS 4
+#endif
+__host__ __device__ constexpr int tile_rows(int f64, int compact) {
+    return !f64 ? 8 : (compact ? MSR_CX_ROWS : 4);
 }
+template <class CT, class ST>
+constexpr int rows_per_tile() {
+    return tile_rows(sizeof(CT) == 16, std::is_same<ST, cx64>::value);
+}
+__host__ __device__ inline int tiles2d(int ny, int rows) { return (ny + rows - 1) / rows; }
 int pass_b_colblocks(int nx2, int ny2, int K, int f64);  // column blocks of pass B per coil group
 
 // block counts per operator of each pass
-inline int64_t blocks_a2d(const Apply2DArgs& a) { return (int64_t)tiles2d(a.ny, a.f64) * a.K * a.C; }
-inline int64_t blocks_c2d(const Apply2DArgs& a) { return (int64_t)tiles2d(a.ny, a.f64) * a.K * a.G; }
+inline int64_t blocks_a2d(const Apply2DArgs& a) {
+    return (int64_t)tiles2d(a.ny, tile_rows(a.f64, a.compact)) * a.K * a.C;
+}
+inline int64_t blocks_c2d(const Apply2DArgs& a) {
+    return (int64_t)tiles2d(a.ny, tile_rows(a.f64, a.compact)) * a.K * a.G;
+}
 inline int64_t blocks_b2d(const Apply2DArgs& a) {
     return (int64_t)pass_b_colblocks(2 * a.nx, 2 * a.ny, a.K, a.f64) * a.Gb;
 }
@@ -74,15 +92,15 @@ inline int64_t blocks_b2d(const Apply2DArgs& a) {
 // 1/(4 nx ny) normalisation.
 void launch_apply2d_passes(const Apply2DArgs& a, hipStream_t st);
 
-// Batched form.  The operators of one launch share the precision and 2nx
+// Batched form (am: ApplyMode).  The operators of one launch share the mode and 2nx
 // (passes A, C) or (2ny, K, lam_real) (pass B); `jobs` is a device array of nj
 // operators and `pre` a device array of nj + 1 block offsets (pre[0] = 0) for
 // that pass.
-void launch_pass_a_2d_batch(int nx2, int f64, const Apply2DArgs* jobs, const int* pre, int nj,
+void launch_pass_a_2d_batch(int nx2, int am, const Apply2DArgs* jobs, const int* pre, int nj,
                             int64_t nblocks, hipStream_t st);
-void launch_pass_b_2d_batch(int ny2, int K, int lam_real, int f64, const Apply2DArgs* jobs,
+void launch_pass_b_2d_batch(int ny2, int K, int lam_real, int am, const Apply2DArgs* jobs,
                             const int* pre, int nj, int64_t nblocks, hipStream_t st);
-void launch_pass_c_2d_batch(int nx2, int f64, const Apply2DArgs* jobs, const int* pre, int nj,
+void launch_pass_c_2d_batch(int nx2, int am, const Apply2DArgs* jobs, const int* pre, int nj,
                             int64_t nblocks, hipStream_t st);
 
 // Operator of block `blk` in a batched launch: the j with pre[j] <= blk < pre[j+1].

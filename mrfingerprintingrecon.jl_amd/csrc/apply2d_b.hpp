@@ -68,8 +68,9 @@ struct PassBShape {
 };
 
 // block lb = (column block, coil group), column block fastest.  The kernel
-// stays ComplexF32 (or Float32) and is promoted to the arithmetic type CT.
-template <int NY2, int K, bool LREAL, class CT>
+// stays ComplexF32 (or Float32) and is promoted to the arithmetic type CT; the
+// intermediate T is stored as ST.
+template <int NY2, int K, bool LREAL, class CT, class ST>
 __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     if (a.skip && *a.skip) return;
     using S = PassBShape<NY2, K, CT>;
@@ -81,7 +82,7 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     __shared__ CT s_tw[NY2];
     __shared__ CT s_buf[WC * K * LL];
     const LT* lam = reinterpret_cast<const LT*>(a.lam);
-    CT* T = static_cast<CT*>(a.T);
+    ST* T = static_cast<ST*>(a.T);
     const int nx2 = 2 * a.nx, ny = a.ny, C = a.C, Gb = a.Gb;
     const int ncb = (nx2 + WC - 1) / WC;
     const int tid = threadIdx.x;
@@ -129,14 +130,14 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     CT pre[NR];  // the next coil's line, loaded one coil ahead
     auto prefetch = [&](int c) {
         // branch-free: a thread without a line reads line 0 of the WG and keeps zero
-        const CT* src = T + (((size_t)c * K + kl) * nx2 + (col_ok ? qx0 + wcl : qx0)) * ny;
+        const ST* src = T + (((size_t)c * K + kl) * nx2 + (col_ok ? qx0 + wcl : qx0)) * ny;
 #pragma unroll
         for (int p = 0; p < NR / R0; ++p)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
                 const int i = t + p * TP + r * NB0;
                 if (r * NB0 < NY2 / 2) {
-                    const CT x = src[i < NY2 / 2 ? i : 0];
+                    const CT x = st_load<CT, ST>(src + (i < NY2 / 2 ? i : 0));
                     pre[p * R0 + r] = (i < NY2 / 2 && col_ok) ? x : cz<CT>();
                 } else {
                     pre[p * R0 + r] = cz<CT>();
@@ -183,38 +184,47 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
             // stage costs more registers than the LDS round trip); the crop
             // stores elements i < ny, lanes on consecutive i
             fft_wave<NY2, +1, false, false>(line, s_tw, t, true, xr);
-            CT* dst = T + (((size_t)c * K + kl) * nx2 + qx0 + wcl) * ny;
+            ST* dst = T + (((size_t)c * K + kl) * nx2 + qx0 + wcl) * ny;
 #pragma unroll
             for (int m = 0; m < NR / 2; ++m) {
                 const int i = t + m * TP;
-                if (col_ok) dst[i] = line[lds_pad(i)];
+                if (col_ok) st_store<CT, ST>(dst + i, line[lds_pad(i)]);
             }
         }
     }
 }
 
-template <int NY2, int K, bool LREAL, class CT>
+template <int NY2, int K, bool LREAL, class CT, class ST>
 __global__ void __launch_bounds__(512, PB_WAVES) pass_b_2d(Apply2DArgs a) {
-    pass_b_body<NY2, K, LREAL, CT>(a, blockIdx.x);
+    pass_b_body<NY2, K, LREAL, CT, ST>(a, blockIdx.x);
 }
 
-template <int NY2, int K, bool LREAL, class CT>
+template <int NY2, int K, bool LREAL, class CT, class ST>
 __global__ void __launch_bounds__(512, PB_WAVES)
 pass_b_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pre, int nj) {
     const int j = batch_job(pre, nj, blockIdx.x);
-    pass_b_body<NY2, K, LREAL, CT>(jobs[j], (int64_t)blockIdx.x - pre[j]);
+    pass_b_body<NY2, K, LREAL, CT, ST>(jobs[j], (int64_t)blockIdx.x - pre[j]);
 }
 
-template <int NY2, int K, bool LREAL, class CT>
+template <int NY2, int K, bool LREAL, class CT, class ST>
 void launch_pass_b_t(const Apply2DArgs& a, const Apply2DArgs* jobs, const int* pre, int nj,
                      int64_t nblocks, hipStream_t st) {
     using S = PassBShape<NY2, K, CT>;
     dim3 grid((unsigned)nblocks);
     if (jobs)
-        hipLaunchKernelGGL((pass_b_2d_batch<NY2, K, LREAL, CT>), grid, dim3(S::BLK), 0, st, jobs,
-                           pre, nj);
+        hipLaunchKernelGGL((pass_b_2d_batch<NY2, K, LREAL, CT, ST>), grid, dim3(S::BLK), 0, st,
+                           jobs, pre, nj);
     else
-        hipLaunchKernelGGL((pass_b_2d<NY2, K, LREAL, CT>), grid, dim3(S::BLK), 0, st, a);
+        hipLaunchKernelGGL((pass_b_2d<NY2, K, LREAL, CT, ST>), grid, dim3(S::BLK), 0, st, a);
+}
+
+// the arithmetic / storage pair of the apply mode
+template <int NY2, int K, bool LREAL>
+void launch_pass_b_mode(const Apply2DArgs& a, const Apply2DArgs* jobs, const int* pre, int nj,
+                        int64_t nblocks, hipStream_t st) {
+    if (!a.f64) launch_pass_b_t<NY2, K, LREAL, float2, float2>(a, jobs, pre, nj, nblocks, st);
+    else if (a.compact) launch_pass_b_t<NY2, K, LREAL, double2, cx64>(a, jobs, pre, nj, nblocks, st);
+    else launch_pass_b_t<NY2, K, LREAL, double2, double2>(a, jobs, pre, nj, nblocks, st);
 }
 
 // per-K entry points (defined in apply2d_b_k*.hip); jobs == nullptr means the
@@ -256,13 +266,8 @@ inline void launch_pass_b_2d(const Apply2DArgs& a, hipStream_t st) {
                     "pass B launch of %lld blocks", (long long)nblocks);                       \
         bool ok = dispatch_fft_len(ny2, [&]<int N>() {                                         \
             if constexpr (N <= 512) {                                                          \
-                if (a.f64) {                                                                   \
-                    if (a.lam_real) launch_pass_b_t<N, KK, true, double2>(a, jobs, pre, nj, nblocks, st);  \
-                    else launch_pass_b_t<N, KK, false, double2>(a, jobs, pre, nj, nblocks, st);            \
-                } else {                                                                       \
-                    if (a.lam_real) launch_pass_b_t<N, KK, true, float2>(a, jobs, pre, nj, nblocks, st);   \
-                    else launch_pass_b_t<N, KK, false, float2>(a, jobs, pre, nj, nblocks, st);             \
-                }                                                                              \
+                if (a.lam_real) launch_pass_b_mode<N, KK, true>(a, jobs, pre, nj, nblocks, st); \
+                else launch_pass_b_mode<N, KK, false>(a, jobs, pre, nj, nblocks, st);         \
             } else {                                                                           \
                 fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Ny = %d above 512", N);             \
             }                                                                                  \

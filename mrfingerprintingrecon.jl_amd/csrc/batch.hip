@@ -68,7 +68,7 @@ msr_batch* batch_create(msr_ctx* ctx, int nops, msr_normalop* const* ops) {
         bt->gb[ib[kb]].jobs.push_back(j);
     }
     auto prefix = [&](BatchGroup& g, int64_t (*count)(const Apply2DArgs&)) {
-        for (int f = 0; f < 2; ++f) {
+        for (int f = 0; f < 3; ++f) {  // ApplyMode
             g.pre[f].assign(1, 0);
             int64_t tot = 0;
             for (int j : g.jobs) {
@@ -80,7 +80,8 @@ msr_batch* batch_create(msr_ctx* ctx, int nops, msr_normalop* const* ops) {
                 a.C = op->C;
                 a.G = op->G;
                 a.Gb = op->Gb;
-                a.f64 = f;
+                a.f64 = f != AM_F32;
+                a.compact = f == AM_F64C;
                 tot += count(a);
                 MSR_REQUIRE(tot < ((int64_t)1 << 31), MSR_ERR_UNSUPPORTED,
                             "batched launch above 2^31 blocks");
@@ -120,7 +121,7 @@ msr_batch* batch_create(msr_ctx* ctx, int nops, msr_normalop* const* ops) {
 // Apply input of every operator for one use of the tables.
 struct TableUse {
     const void* const* b = nullptr;  // per operator
-    bool f64 = false;                // apply precision
+    int am = AM_F32;                 // apply mode (ApplyMode)
     bool skip_done = false;          // gate the passes by the CG done flags
 };
 
@@ -130,12 +131,12 @@ static void fill_tables(msr_batch* bt, BatchTables& T, const TableUse& use,
     T.ensure(bt->table_bytes);
     if (T.ev) MSR_HIP(hipEventSynchronize(T.ev));  // the previous upload has left the staging
     char* h = static_cast<char*>(T.host);
-    const int f = use.f64 ? 1 : 0;
+    const int f = use.am;
     auto put_group = [&](const BatchGroup& g) {
         auto* tab = reinterpret_cast<Apply2DArgs*>(h + g.off_tab);
         for (size_t i = 0; i < g.jobs.size(); ++i) {
             const int j = g.jobs[i];
-            Apply2DArgs a = args2d(bt->ops[j], use.f64);
+            Apply2DArgs a = args2d(bt->ops[j], use.am);
             a.b = use.b[j];
             a.skip = use.skip_done ? &bt->ops[j]->cg_scal.as<CgScalars>()->done : nullptr;
             tab[i] = a;
@@ -178,7 +179,7 @@ static CgJob<VT, AT> base_job(msr_normalop* op, float2* x) {
 // the three pass sequences of every operator, in pass order across the batch
 static void batch_passes(msr_batch* bt, BatchTables& T, const TableUse& use, hipStream_t st) {
     char* base = T.dev.as<char>();
-    const int f = use.f64 ? 1 : 0;
+    const int f = use.am;
     auto tab = [&](const BatchGroup& g) { return reinterpret_cast<const Apply2DArgs*>(base + g.off_tab); };
     auto pre = [&](const BatchGroup& g) { return reinterpret_cast<const int*>(base + g.off_pre); };
     for (auto& g : bt->ga)
@@ -190,7 +191,7 @@ static void batch_passes(msr_batch* bt, BatchTables& T, const TableUse& use, hip
         launch_pass_c_2d_batch(g.key[0], f, tab(g), pre(g), (int)g.jobs.size(), g.nblocks[f], st);
     for (int j : bt->j3d) {
         msr_normalop* op = bt->ops[j];
-        op_passes(op, use.b[j], use.f64, st,
+        op_passes(op, use.b[j], use.am, st,
                   use.skip_done ? &op->cg_scal.as<CgScalars>()->done : nullptr);
     }
 }
@@ -239,16 +240,17 @@ static std::vector<const void*> cg_inputs(msr_batch* bt) {
 }
 
 template <class VT, class AT>
-static TableUse cg_use(std::vector<const void*>& u) {
+static TableUse cg_use(std::vector<const void*>& u, int am) {
     TableUse use;
     use.b = u.data();
-    use.f64 = std::is_same<AT, double2>::value;
+    use.am = am;
     use.skip_done = true;
     return use;
 }
 
 template <class VT, class AT>
-static void build_cg_tables(msr_batch* bt, BatchTables& T, const float2* const* b, hipStream_t st) {
+static void build_cg_tables(msr_batch* bt, BatchTables& T, const float2* const* b, int am,
+                            hipStream_t st) {
     const int nj = (int)bt->ops.size();
     std::vector<CgJob<VT, AT>> cj(nj);
     for (int j = 0; j < nj; ++j) {
@@ -260,13 +262,14 @@ static void build_cg_tables(msr_batch* bt, BatchTables& T, const float2* const* 
         cj[j] = J;
     }
     auto u = cg_inputs<VT, AT>(bt);
-    fill_tables(bt, T, cg_use<VT, AT>(u), cj, st);
+    fill_tables(bt, T, cg_use<VT, AT>(u, am), cj, st);
 }
 
 // The x0 tables: inputs x_j in the apply type (converted into each operator's
 // apply-input buffer first when that is FP64), epilogue target c_j.
 template <class VT, class AT>
-static void build_x0_tables(msr_batch* bt, BatchTables& T, float2* const* x, hipStream_t st) {
+static void build_x0_tables(msr_batch* bt, BatchTables& T, float2* const* x, int am,
+                            hipStream_t st) {
     const int nj = (int)bt->ops.size();
     std::vector<CgJob<VT, AT>> cj(nj);
     std::vector<const void*> xin(nj);
@@ -278,7 +281,7 @@ static void build_x0_tables(msr_batch* bt, BatchTables& T, float2* const* x, hip
     }
     TableUse use;
     use.b = xin.data();
-    use.f64 = std::is_same<AT, double2>::value;
+    use.am = am;
     fill_tables(bt, T, use, cj, st);
 }
 
@@ -291,7 +294,7 @@ __global__ void batch_f32_to_f64_kernel(const float2* __restrict__ a, double2* _
 
 template <class VT, class AT>
 static void start_seq(msr_batch* bt, BatchTables& Tcg, BatchTables& Tx0, float2* const* x,
-                      int maxiter, double reltol, double abstol, bool initially_zero,
+                      int maxiter, double reltol, double abstol, bool initially_zero, int am,
                       hipStream_t st) {
     const int nj = (int)bt->ops.size();
     if (!initially_zero) {
@@ -313,7 +316,7 @@ static void start_seq(msr_batch* bt, BatchTables& Tcg, BatchTables& Tx0, float2*
         }
         TableUse use;
         use.b = xin.data();
-        use.f64 = std::is_same<AT, double2>::value;
+        use.am = am;
         batch_passes(bt, Tx0, use, st);
         launch_epilogue_batch(dev_cgjobs<VT, AT>(bt, Tx0), dev_vpre(bt, Tx0), nj, bt->vblocks, st);
     }
@@ -322,10 +325,10 @@ static void start_seq(msr_batch* bt, BatchTables& Tcg, BatchTables& Tx0, float2*
 }
 
 template <class VT, class AT>
-static void iterate_seq(msr_batch* bt, BatchTables& T, int niter, hipStream_t st) {
+static void iterate_seq(msr_batch* bt, BatchTables& T, int niter, int am, hipStream_t st) {
     const int nj = (int)bt->ops.size();
     auto u = cg_inputs<VT, AT>(bt);
-    const TableUse use = cg_use<VT, AT>(u);
+    const TableUse use = cg_use<VT, AT>(u, am);
     CgJob<VT, AT>* cj = dev_cgjobs<VT, AT>(bt, T);
     const int* vp = dev_vpre(bt, T);
     for (int it = 0; it < niter; ++it) {
@@ -353,8 +356,7 @@ static void check_vectors(msr_batch* bt, float2* const* x, const float2* const* 
 }
 
 static void check_prec(int prec) {
-    MSR_REQUIRE(prec == MSR_PREC_F32 || prec == MSR_PREC_MIXED || prec == MSR_PREC_F64,
-                MSR_ERR_ARGUMENT, "unknown CG precision %d", prec);
+    MSR_REQUIRE(prec_valid(prec), MSR_ERR_ARGUMENT, "unknown CG precision %d", prec);
 }
 
 static void record_start(msr_batch* bt, float2* const* x, int prec) {
@@ -369,17 +371,18 @@ static void record_start(msr_batch* bt, float2* const* x, int prec) {
 // dispatch a templated step on the CG precision
 #define MSR_PREC_DISPATCH(prec, CALL)                          \
     do {                                                       \
-        if ((prec) == MSR_PREC_F64) CALL(double2, double2);    \
+        if (apply_mode(prec) != AM_F32) CALL(double2, double2); \
         else if ((prec) == MSR_PREC_MIXED) CALL(double2, float2); \
         else CALL(float2, float2);                             \
     } while (0)
 
 template <class VT, class AT>
 static void start_eager(msr_batch* bt, float2* const* x, const float2* const* b, int maxiter,
-                        double reltol, double abstol, bool initially_zero, hipStream_t st) {
-    build_cg_tables<VT, AT>(bt, bt->t_cg, b, st);
-    if (!initially_zero) build_x0_tables<VT, AT>(bt, bt->t_x0, x, st);
-    start_seq<VT, AT>(bt, bt->t_cg, bt->t_x0, x, maxiter, reltol, abstol, initially_zero, st);
+                        double reltol, double abstol, bool initially_zero, int am,
+                        hipStream_t st) {
+    build_cg_tables<VT, AT>(bt, bt->t_cg, b, am, st);
+    if (!initially_zero) build_x0_tables<VT, AT>(bt, bt->t_x0, x, am, st);
+    start_seq<VT, AT>(bt, bt->t_cg, bt->t_x0, x, maxiter, reltol, abstol, initially_zero, am, st);
 }
 
 void batch_cg_start(msr_batch* bt, float2* const* x, const float2* const* b, int maxiter,
@@ -389,7 +392,7 @@ void batch_cg_start(msr_batch* bt, float2* const* x, const float2* const* b, int
     MSR_REQUIRE(maxiter >= 0, MSR_ERR_ARGUMENT, "maxiter = %d", maxiter);
     record_start(bt, x, prec);
 #define MSR_START(VT, AT) \
-    start_eager<VT, AT>(bt, x, b, maxiter, reltol, abstol, initially_zero, st)
+    start_eager<VT, AT>(bt, x, b, maxiter, reltol, abstol, initially_zero, apply_mode(prec), st)
     MSR_PREC_DISPATCH(prec, MSR_START);
 #undef MSR_START
     bt->cg_ready = true;
@@ -409,7 +412,7 @@ static int batch_prec(msr_batch* bt) {
 
 static void ensure_cg_tables(msr_batch* bt, int prec, hipStream_t st) {
     if (bt->cg_ready) return;
-#define MSR_BUILD(VT, AT) build_cg_tables<VT, AT>(bt, bt->t_cg, nullptr, st)
+#define MSR_BUILD(VT, AT) build_cg_tables<VT, AT>(bt, bt->t_cg, nullptr, apply_mode(prec), st)
     MSR_PREC_DISPATCH(prec, MSR_BUILD);
 #undef MSR_BUILD
     bt->cg_ready = true;
@@ -419,7 +422,7 @@ void batch_cg_iterate(msr_batch* bt, int niter, hipStream_t st) {
     MSR_REQUIRE(niter >= 0, MSR_ERR_ARGUMENT, "niter = %d", niter);
     const int prec = batch_prec(bt);
     ensure_cg_tables(bt, prec, st);
-#define MSR_ITER(VT, AT) iterate_seq<VT, AT>(bt, bt->t_cg, niter, st)
+#define MSR_ITER(VT, AT) iterate_seq<VT, AT>(bt, bt->t_cg, niter, apply_mode(prec), st)
     MSR_PREC_DISPATCH(prec, MSR_ITER);
 #undef MSR_ITER
 }
@@ -434,7 +437,7 @@ void batch_cg_finish(msr_batch* bt, hipStream_t st) {
 
 template <class VT, class AT>
 static void cg_graph_t(msr_batch* bt, float2* const* x, const float2* const* b, int maxiter,
-                       double reltol, double abstol, bool initially_zero, hipStream_t st,
+                       double reltol, double abstol, bool initially_zero, int am, hipStream_t st,
                        const std::vector<const void*>& key) {
     if (bt->graph && bt->graph_key == key) {
         MSR_HIP(hipGraphLaunch(bt->graph, st));
@@ -446,13 +449,14 @@ static void cg_graph_t(msr_batch* bt, float2* const* x, const float2* const* b, 
         bt->graph = nullptr;
     }
     // tables of the graph, uploaded before capture and left untouched afterwards
-    build_cg_tables<VT, AT>(bt, bt->g_cg, b, st);
-    if (!initially_zero) build_x0_tables<VT, AT>(bt, bt->g_x0, x, st);
+    build_cg_tables<VT, AT>(bt, bt->g_cg, b, am, st);
+    if (!initially_zero) build_x0_tables<VT, AT>(bt, bt->g_x0, x, am, st);
     hipGraph_t g = nullptr;
     MSR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     try {
-        start_seq<VT, AT>(bt, bt->g_cg, bt->g_x0, x, maxiter, reltol, abstol, initially_zero, st);
-        iterate_seq<VT, AT>(bt, bt->g_cg, maxiter, st);
+        start_seq<VT, AT>(bt, bt->g_cg, bt->g_x0, x, maxiter, reltol, abstol, initially_zero, am,
+                          st);
+        iterate_seq<VT, AT>(bt, bt->g_cg, maxiter, am, st);
         finish_seq<VT, AT>(bt, bt->g_cg, st);
     } catch (...) {
         (void)hipStreamEndCapture(st, &g);
@@ -501,7 +505,7 @@ void batch_cg(msr_batch* bt, float2* const* x, const float2* const* b, int maxit
     key.push_back((const void*)(intptr_t)((initially_zero ? 1 : 0) + 2 * prec));
     key.push_back(st);
 #define MSR_GRAPH(VT, AT) \
-    cg_graph_t<VT, AT>(bt, x, b, maxiter, reltol, abstol, initially_zero, st, key)
+    cg_graph_t<VT, AT>(bt, x, b, maxiter, reltol, abstol, initially_zero, apply_mode(prec), st, key)
     MSR_PREC_DISPATCH(prec, MSR_GRAPH);
 #undef MSR_GRAPH
     bt->cg_ready = false;  // eager calls rebuild t_cg from the operators' state

@@ -25,8 +25,8 @@ namespace msr {
 struct BatchGroup {
     int key[3] = {0, 0, 0};       // A/C: {2Nx}; B: {2Ny, K, lam_real}
     std::vector<int> jobs;        // batch indices of the group's operators
-    std::vector<int> pre[2];      // block prefix (jobs.size() + 1) for FP32 / FP64 applies
-    int64_t nblocks[2] = {0, 0};
+    std::vector<int> pre[3];      // block prefix (jobs.size() + 1) per ApplyMode
+    int64_t nblocks[3] = {0, 0, 0};
     size_t off_tab = 0, off_pre = 0;  // byte offsets in a table set
 };
 

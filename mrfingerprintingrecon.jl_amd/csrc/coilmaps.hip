@@ -159,8 +159,7 @@ void reconstruct_coilwise(msr_ctx* ctx, const msr_traj* traj, int d, const int64
                 traj->d, d);
     MSR_REQUIRE(ncoils >= 1, MSR_ERR_ARGUMENT, "ncoils = %d", ncoils);
     MSR_REQUIRE(niter >= 0, MSR_ERR_ARGUMENT, "Niter_cg = %d", niter);
-    MSR_REQUIRE(prec >= MSR_PREC_F32 && prec <= MSR_PREC_F64, MSR_ERR_ARGUMENT, "precision = %d",
-                prec);
+    MSR_REQUIRE(prec_valid(prec), MSR_ERR_ARGUMENT, "precision = %d", prec);
     DevBuf ones;
     if (U == nullptr) {
         ones.alloc(sizeof(float) * (size_t)traj->nt);

@@ -92,7 +92,7 @@ static int env_int(const char* name, int dflt) {
 void op_alloc_scratch(msr_normalop* op) {
     const int64_t ny = op->N[1];
     if (op->d == 2 && !op->cart) {
-        const int tiles = tiles2d((int)ny, 0);
+        const int tiles = tiles2d((int)ny, 8);
         int G = (int)ceil_div(512, (int64_t)tiles * op->K);
         G = std::max(1, std::min(G, op->C));
         op->G = env_int("MSR_G", G);
@@ -238,13 +238,15 @@ msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real,
 }
 
 // ------------------------------------------------------------------ applies
-Apply2DArgs args2d(msr_normalop* op, bool f64) {
+Apply2DArgs args2d(msr_normalop* op, int am) {
+    const bool f64 = am != AM_F32;
     Apply2DArgs a;
     a.nx = (int)op->N[0];
     a.ny = (int)op->N[1];
     a.K = op->K;
     a.C = op->C;
     a.f64 = f64 ? 1 : 0;
+    a.compact = am == AM_F64C ? 1 : 0;
     a.cmaps = op->identity_cmaps ? nullptr : op->cmaps.as<float2>();
     a.lam = op->lam->p;
     a.lam_real = op->lam_real;
@@ -267,13 +269,17 @@ Apply2DArgs args2d(msr_normalop* op, bool f64) {
 double op_scale(const msr_normalop* op) { return 1.0 / (double)op_grid_cells(op); }
 
 // partial[g] <- unscaled group sums of A^H A b (b and partial of the apply type)
-void op_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st, const int* skip) {
+// am: ApplyMode.  The compact storage of AM_F64C applies to the 2D
+// non-Cartesian passes; the 3D and Cartesian passes keep ComplexF64
+// intermediates in that mode.
+void op_passes(msr_normalop* op, const void* b, int am, hipStream_t st, const int* skip) {
+    const bool f64 = am != AM_F32;
     MSR_REQUIRE(op->scratch_es >= (f64 ? sizeof(double2) : sizeof(float2)), MSR_ERR_ERROR,
                 "apply scratch not allocated for this precision");
     if (op->cart) {
         launch_cart_passes(op, b, f64, st, skip);
     } else if (op->d == 2) {
-        Apply2DArgs a = args2d(op, f64);
+        Apply2DArgs a = args2d(op, am);
         a.b = b;
         a.skip = skip;
         launch_apply2d_passes(a, st);
@@ -284,15 +290,15 @@ void op_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st, const 
 
 void op_apply(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
               bool beta_zero, hipStream_t st) {
-    op_passes(op, b, false, st, nullptr);
+    op_passes(op, b, AM_F32, st, nullptr);
     launch_epilogue<float2>(op->partial.as<float2>(), op->G, op->V * op->K, x, alpha, beta,
                             beta_zero, op_scale(op), st, nullptr);
 }
 
 void op_apply_f64(msr_normalop* op, double2* x, const double2* b, double2 alpha, double2 beta,
-                  bool beta_zero, hipStream_t st) {
+                  bool beta_zero, hipStream_t st, int am) {
     op_ensure_scratch(op, true);
-    op_passes(op, b, true, st, nullptr);
+    op_passes(op, b, am, st, nullptr);
     launch_epilogue<double2>(op->partial.as<double2>(), op->G, op->V * op->K, x, alpha, beta,
                              beta_zero, op_scale(op), st, nullptr);
 }
@@ -305,7 +311,7 @@ void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha
     launch_scale<float2>(x, op->V * op->K, beta, beta_zero, st);
     for (int c = 0; c < op->C; ++c) {
         if (op->d == 2) {
-            Apply2DArgs a = args2d(op, false);
+            Apply2DArgs a = args2d(op, AM_F32);
             a.C = 1;
             a.G = 1;
             a.Gb = 1;
@@ -345,7 +351,7 @@ void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hip
 // vector type, c in the apply type, and an FP32 copy of u when the vectors are
 // FP64 but the apply is FP32.  Also sizes the apply scratch.
 size_t op_footprint(const msr_normalop* op, int prec) {
-    const bool v64 = prec != MSR_PREC_F32, a64 = prec == MSR_PREC_F64;
+    const bool v64 = prec != MSR_PREC_F32, a64 = apply_mode(prec) != AM_F32;
     const size_t n = (size_t)op->V * op->K;
     const size_t es = a64 ? sizeof(double2) : sizeof(float2);  // apply intermediates
     const size_t vs = v64 ? sizeof(double2) : sizeof(float2);  // CG vectors
@@ -368,7 +374,7 @@ size_t op_kernel_bytes(const msr_normalop* op) {
 
 void op_cg_buffers(msr_normalop* op, int prec) {
     const size_t n = (size_t)op->V * op->K;
-    const bool v64 = prec != MSR_PREC_F32, a64 = prec == MSR_PREC_F64;
+    const bool v64 = prec != MSR_PREC_F32, a64 = apply_mode(prec) != AM_F32;
     const size_t es = v64 ? sizeof(double2) : sizeof(float2);
     op->cg_r.alloc(es * n);
     op->cg_u.alloc(es * n);
@@ -416,25 +422,24 @@ __global__ void f32_to_f64_kernel(const float2* __restrict__ a, double2* __restr
 // kernels skip once done is set)
 template <class VT, class AT>
 static void cg_iteration(msr_normalop* op, const CgVecs<VT, AT>& v, CgScalars* S, double* part,
-                         hipStream_t st) {
-    constexpr bool a64 = std::is_same<AT, double2>::value;
+                         int am, hipStream_t st) {
     const int64_t n = op->V * op->K;
     launch_cg_u(v, n, S, st);
-    op_passes(op, v.ua, a64, st, &S->done);
+    op_passes(op, v.ua, am, st, &S->done);
     launch_cg_reduce_dot(op->partial.as<AT>(), op->G, n, v, op_scale(op), part, S, st);
     launch_cg_xr(v, n, part, S, st);
 }
 
 // c = A x0 in the apply precision
 template <class VT, class AT>
-static void cg_apply_x0(msr_normalop* op, const CgVecs<VT, AT>& v, const float2* x,
+static void cg_apply_x0(msr_normalop* op, const CgVecs<VT, AT>& v, const float2* x, int am,
                         hipStream_t st) {
     if constexpr (std::is_same<AT, double2>::value) {
         const int64_t n = op->V * op->K;
         int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);
         hipLaunchKernelGGL(f32_to_f64_kernel, dim3(grid), dim3(256), 0, st, x, v.ua, n);
         MSR_CHECK_LAUNCH();
-        op_passes(op, v.ua, true, st, nullptr);
+        op_passes(op, v.ua, am, st, nullptr);
         launch_epilogue<double2>(op->partial.as<double2>(), op->G, n, v.c, make_double2(1.0, 0.0),
                                  make_double2(0.0, 0.0), true, op_scale(op), st, nullptr);
     } else {
@@ -445,12 +450,12 @@ static void cg_apply_x0(msr_normalop* op, const CgVecs<VT, AT>& v, const float2*
 template <class VT, class AT>
 static void cg_prologue(msr_normalop* op, const CgVecs<VT, AT>& v, CgScalars* S, double* part,
                         float2* x, const float2* b, int maxiter, double reltol, double abstol,
-                        bool initially_zero, hipStream_t st) {
+                        bool initially_zero, int am, hipStream_t st) {
     const int64_t n = op->V * op->K;
     if (initially_zero) {
         launch_cg_start(b, x, v, n, true, part, S, reltol, abstol, maxiter, false, st);
     } else {
-        cg_apply_x0(op, v, x, st);
+        cg_apply_x0(op, v, x, am, st);
         launch_cg_start(b, x, v, n, false, part, S, reltol, abstol, maxiter, true, st);
     }
 }
@@ -466,9 +471,10 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
     const CgVecs<VT, AT> v = op_cg_vecs<VT, AT>(op, x);
     auto* S = op->cg_scal.as<CgScalars>();
     double* part = op->cg_part.as<double>();
+    const int am = apply_mode(prec);
     if (mode == MSR_CG_DEVICE) {
-        cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, st);
-        for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, S, part, st);
+        cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, am, st);
+        for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, S, part, am, st);
         cg_epilogue(op, v, x, st);
     } else if (mode == MSR_CG_GRAPH) {
         // the captured kernels hold the buffer addresses: they are part of the
@@ -489,8 +495,8 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
             hipGraph_t g = nullptr;
             MSR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, st);
-                for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, S, part, st);
+                cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, am, st);
+                for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, S, part, am, st);
                 cg_epilogue(op, v, x, st);
             } catch (...) {
                 (void)hipStreamEndCapture(st, &g);
@@ -518,7 +524,7 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
         // applies all coils at once).
         if constexpr (std::is_same<AT, float2>::value) {
             const int64_t n = op->V * op->K;
-            cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, st);
+            cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, am, st);
             CgScalars hs;
             for (int it = 0; it < maxiter; ++it) {
                 MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
@@ -549,10 +555,9 @@ static void cg_run(msr_normalop* op, float2* x, const float2* b, int maxiter, do
 void op_cg(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
            double abstol, bool initially_zero, int mode, int prec, int* iters_out,
            double* res_out, hipStream_t st) {
-    MSR_REQUIRE(prec == MSR_PREC_F32 || prec == MSR_PREC_MIXED || prec == MSR_PREC_F64,
-                MSR_ERR_ARGUMENT, "unknown CG precision %d", prec);
+    MSR_REQUIRE(prec_valid(prec), MSR_ERR_ARGUMENT, "unknown CG precision %d", prec);
     op_cg_buffers(op, prec);
-    if (prec == MSR_PREC_F64)
+    if (prec == MSR_PREC_F64 || prec == MSR_PREC_F64C)
         cg_run<double2, double2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, prec, st);
     else if (prec == MSR_PREC_MIXED)
         cg_run<double2, float2>(op, x, b, maxiter, reltol, abstol, initially_zero, mode, prec, st);
@@ -605,7 +610,7 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Bytes of scratch a solve of n unknowns at precision prec takes.
 size_t cg_noalloc_bytes(int64_t n, int prec) {
-    const bool v64 = prec != MSR_PREC_F32, a64 = prec == MSR_PREC_F64;
+    const bool v64 = prec != MSR_PREC_F32, a64 = apply_mode(prec) != AM_F32;
     const size_t vs = v64 ? sizeof(double2) : sizeof(float2);
     const size_t es = a64 ? sizeof(double2) : sizeof(float2);
     size_t b = 2 * align256(vs * n) + align256(es * n);        // r, u, c
@@ -616,7 +621,7 @@ size_t cg_noalloc_bytes(int64_t n, int prec) {
 
 template <class VT, class AT>
 static void cg_noalloc_run(msr_normalop* op, unsigned char* base, float2* x, const float2* b,
-                           int maxiter, double reltol, double abstol, bool initially_zero,
+                           int maxiter, double reltol, double abstol, bool initially_zero, int am,
                            hipStream_t st, CgScalars** S_out) {
     const size_t n = (size_t)op->V * op->K;
     size_t off = 0;
@@ -638,8 +643,8 @@ static void cg_noalloc_run(msr_normalop* op, unsigned char* base, float2* x, con
     // every buffer above is written by the start kernels before it is read
     // (r, u, x and the scalars by cg_start; c, part by each iteration), so
     // nothing carried over from an earlier call is used
-    cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, st);
-    for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, S, part, st);
+    cg_prologue(op, v, S, part, x, b, maxiter, reltol, abstol, initially_zero, am, st);
+    for (int it = 0; it < maxiter; ++it) cg_iteration(op, v, S, part, am, st);
     cg_epilogue(op, v, x, st);
     *S_out = S;
 }
@@ -647,26 +652,29 @@ static void cg_noalloc_run(msr_normalop* op, unsigned char* base, float2* x, con
 void op_cg_noalloc(msr_normalop* op, float2* x, const float2* b, int maxiter, double reltol,
                    double abstol, bool initially_zero, int prec, int* iters_out, double* res_out,
                    hipStream_t st) {
-    MSR_REQUIRE(prec == MSR_PREC_F32 || prec == MSR_PREC_MIXED || prec == MSR_PREC_F64,
-                MSR_ERR_ARGUMENT, "unknown CG precision %d", prec);
+    MSR_REQUIRE(prec_valid(prec), MSR_ERR_ARGUMENT, "unknown CG precision %d", prec);
     MSR_REQUIRE(maxiter >= 0, MSR_ERR_ARGUMENT, "maxiter = %d", maxiter);
     const size_t need = cg_noalloc_bytes(op->V * op->K, prec);
     MSR_REQUIRE(need <= CG_NOALLOC_BYTES, MSR_ERR_ARGUMENT,
                 "the solve needs %zu bytes of CG state, above the %zu bytes this entry point "
                 "holds (use msr_cg)", need, (size_t)CG_NOALLOC_BYTES);
-    const size_t es = prec == MSR_PREC_F64 ? sizeof(double2) : sizeof(float2);
+    const int am = apply_mode(prec);
+    const size_t es = am != AM_F32 ? sizeof(double2) : sizeof(float2);
     MSR_REQUIRE(op->scratch_es >= es, MSR_ERR_ARGUMENT,
                 "the operator's apply scratch is not sized for this precision yet (this entry "
                 "point allocates nothing: run one msr_normalop_apply_f64 or msr_cg at this "
                 "precision first)");
     unsigned char* base = noalloc_base(op->ctx->device);
     CgScalars* S = nullptr;
-    if (prec == MSR_PREC_F64)
-        cg_noalloc_run<double2, double2>(op, base, x, b, maxiter, reltol, abstol, initially_zero, st, &S);
+    if (am != AM_F32)
+        cg_noalloc_run<double2, double2>(op, base, x, b, maxiter, reltol, abstol, initially_zero,
+                                         am, st, &S);
     else if (prec == MSR_PREC_MIXED)
-        cg_noalloc_run<double2, float2>(op, base, x, b, maxiter, reltol, abstol, initially_zero, st, &S);
+        cg_noalloc_run<double2, float2>(op, base, x, b, maxiter, reltol, abstol, initially_zero,
+                                        am, st, &S);
     else
-        cg_noalloc_run<float2, float2>(op, base, x, b, maxiter, reltol, abstol, initially_zero, st, &S);
+        cg_noalloc_run<float2, float2>(op, base, x, b, maxiter, reltol, abstol, initially_zero,
+                                       am, st, &S);
     if (iters_out || res_out) {
         CgScalars hs;
         MSR_HIP(hipMemcpyAsync(&hs, S, sizeof(CgScalars), hipMemcpyDeviceToHost, st));

@@ -19,8 +19,9 @@ void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, 
 void op_alloc_scratch(msr_normalop* op);
 void op_apply(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
               bool beta_zero, hipStream_t st);
+// ComplexF64 apply; am = AM_F64 or AM_F64C (compact intermediates, 2D only)
 void op_apply_f64(msr_normalop* op, double2* x, const double2* b, double2 alpha, double2 beta,
-                  bool beta_zero, hipStream_t st);
+                  bool beta_zero, hipStream_t st, int am = AM_F64);
 void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
                       bool beta_zero, hipStream_t st);
 void op_export(const msr_normalop* op, void* lambda_out, int64_t* kmask_out, hipStream_t st);
@@ -38,9 +39,17 @@ void op_cg_noalloc(msr_normalop* op, float2* x, const float2* b, int maxiter, do
                    hipStream_t st);
 
 // pieces used by the batch engine
-Apply2DArgs args2d(msr_normalop* op, bool f64);        // b = skip = nullptr
+// apply mode (ApplyMode) of a CG precision (MSR_PREC_*)
+inline int apply_mode(int prec) {
+    return prec == MSR_PREC_F64 ? AM_F64 : prec == MSR_PREC_F64C ? AM_F64C : AM_F32;
+}
+inline bool prec_valid(int prec) {
+    return prec == MSR_PREC_F32 || prec == MSR_PREC_MIXED || prec == MSR_PREC_F64 ||
+           prec == MSR_PREC_F64C;
+}
+Apply2DArgs args2d(msr_normalop* op, int am);          // b = skip = nullptr
 void op_ensure_scratch(msr_normalop* op, bool f64);    // grow-only; not during capture
-void op_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st, const int* skip);
+void op_passes(msr_normalop* op, const void* b, int am, hipStream_t st, const int* skip);
 double op_scale(const msr_normalop* op);               // 1 / Toeplitz grid cells
 void op_cg_buffers(msr_normalop* op, int prec);        // allocates; not during capture
 template <class VT, class AT>

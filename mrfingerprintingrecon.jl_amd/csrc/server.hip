@@ -25,8 +25,7 @@ static double now_ms() {
 
 msr_server* server_create(msr_ctx* ctx, size_t budget, int max_active, int prec) {
     MSR_REQUIRE(max_active >= 1, MSR_ERR_ARGUMENT, "max_active = %d", max_active);
-    MSR_REQUIRE(prec == MSR_PREC_F32 || prec == MSR_PREC_MIXED || prec == MSR_PREC_F64,
-                MSR_ERR_ARGUMENT, "precision = %d", prec);
+    MSR_REQUIRE(prec_valid(prec), MSR_ERR_ARGUMENT, "precision = %d", prec);
     auto* s = new msr_server;
     s->ctx = ctx;
     s->budget = budget;

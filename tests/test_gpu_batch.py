@@ -89,18 +89,21 @@ def test_batch_member_is_bit_identical_to_batch_of_one(torch, mix):
         assert torch.equal(y1, yb[i]), f"operator {i} differs alone vs in the batch"
 
 
-@pytest.mark.parametrize("mode", ["device", "graph"])
-def test_batch_cg_matches_single_cg(torch, mix, mode):
+@pytest.mark.parametrize("mode,precision", [("device", "f64"), ("graph", "f64"),
+                                            ("graph", "f64c")])
+def test_batch_cg_matches_single_cg(torch, mix, mode, precision):
     msr = load_package()
     B = msr.NFFTNormalOpBatch([j["A"] for j in mix])
     bs = [j["xd"] for j in mix]
     xs = [torch.empty_like(b) for b in bs]
     for rep in range(2):  # graph: capture, then replay
-        B.cg_(xs, bs, maxiter=12, reltol=0.0, initially_zero=True, mode=mode)
+        B.cg_(xs, bs, maxiter=12, reltol=0.0, initially_zero=True, mode=mode,
+              precision=precision)
     for j, x in zip(mix, xs):
-        xr = msr.cg(j["A"], j["xd"], maxiter=12, reltol=0.0, mode="device")
+        xr = msr.cg(j["A"], j["xd"], maxiter=12, reltol=0.0, mode="device", precision=precision)
         e = rel_err(x.cpu().numpy(), xr.cpu().numpy())
-        print(f"  {mode} N={j['N']}: batch vs single CG {e:.2e}", flush=True)
+        print(f"  {mode}/{precision} N={j['N']}: batch vs single CG {e:.2e} (bit-exact "
+              f"{torch.equal(x, xr)})", flush=True)
         assert e <= 1e-6
 
 

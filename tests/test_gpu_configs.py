@@ -88,6 +88,8 @@ def test_c2_end_to_end_vs_fp64_pipeline(torch, msr):
     A = msr.NFFTNormalOp(N, traj, U_d, cmaps=cml)
     b = msr.calculate_backprojection(data, traj, cml, U=U_d)
     x = msr.cg(A, b.reshape(-1), maxiter=iters, reltol=0.0, mode="graph").cpu().numpy()
+    xc = msr.cg(A, b.reshape(-1), maxiter=iters, reltol=0.0, mode="graph",
+                precision="f64c").cpu().numpy()
     lam = A.prod.A.Λ.cpu().numpy()
     km = A.prod.A.kmask_indcs.cpu().numpy() - 1
     b = b.cpu().numpy()
@@ -105,12 +107,15 @@ def test_c2_end_to_end_vs_fp64_pipeline(torch, msr):
     ref = ocg.cg(toeplitz.NormalOp(N, lam64, km64, cm2), b64.reshape(-1), maxiter=iters,
                  reltol=0.0)
     e2e = rel_err(x, ref)
+    e2e_c = rel_err(xc, ref)
     print(f"C2: kernel {e_lam:.2e}, backprojection {e_b:.2e}, solve on the device operator "
-          f"{e_given:.2e}, end to end {e2e:.2e}", flush=True)
+          f"{e_given:.2e} (compact intermediates {rel_err(xc, ref_given):.2e}), end to end "
+          f"{e2e:.2e} (compact intermediates {e2e_c:.2e})", flush=True)
     assert e_lam < KERNEL_TOL
     assert e_b < BP_TOL
     assert e_given < SOLVE_F64_TOL
     assert e2e < E2E_TOL
+    assert e2e_c < E2E_TOL
 
 
 # ------------------------------------------------------------------------ C3
