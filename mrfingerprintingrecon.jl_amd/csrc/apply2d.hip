@@ -15,17 +15,26 @@ constexpr int rowpass_block() {
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
-// each XCD with its own L2.  Row tiles of 64 bytes per line half-fill a
-// 128-byte line of the transposed intermediate, so neighbouring tiles should
-// run on one XCD: blocks b and b + 8 get consecutive logical indices (tile
-// fastest), when the block count divides by 8.
+// each XCD with its own L2.  Blocks b and b + 8 get consecutive logical
+// indices (when the block count divides by 8), so blocks that share inputs
+// run on one XCD close in time: pass A orders (coil, coefficient, tile) with
+// the coil fastest (the coefficient's row tile of b is read once from HBM for
+// all coils, a coil map tile once for all coefficients), pass C orders
+// (coefficient, coil group, tile) with the coefficient fastest (a group's coil
+// map tiles read once for all coefficients).  Pass A with 16-byte storage of
+// the intermediate keeps the tile fastest instead: its writes (full 128-byte
+// lines, 200 MiB per apply at 256^2 / 20 coils / K = 5) dominate its reads,
+// and neighbouring tiles of one plane written together took 74 us against
+// 93 us for the coil-fastest order on MI355X; with 8-byte storage the
+// coil-fastest order is the faster one (66 against 70 us).
 __device__ __forceinline__ int64_t xcd_order(int64_t lb, int64_t nblk) {
     if (nblk % 8 != 0) return lb;
     return (lb % 8) * (nblk / 8) + lb / 8;
 }
 
 // ----------------------------------------------------------------- pass A
-// block lb = (tile, k, c), tile fastest; CT arithmetic, ST storage of T
+// block lb = (c, k, tile), c fastest (tile fastest for 16-byte ST); CT
+// arithmetic, ST storage of T
 template <int NX2, class CT, class ST>
 __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     if (a.skip && *a.skip) return;
@@ -39,9 +48,16 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     const int nx = a.nx, ny = a.ny, K = a.K;
     const int tiles = tiles2d(ny, RT);
     lb = xcd_order(lb, (int64_t)tiles * K * a.C);
-    const int tile = (int)(lb % tiles);
-    const int k = (int)((lb / tiles) % K);
-    const int c = (int)(lb / ((int64_t)tiles * K));
+    int c, k, tile;
+    if constexpr (sizeof(ST) >= 16) {
+        tile = (int)(lb % tiles);
+        k = (int)((lb / tiles) % K);
+        c = (int)(lb / ((int64_t)tiles * K));
+    } else {
+        c = (int)(lb % a.C);
+        k = (int)((lb / a.C) % K);
+        tile = (int)(lb / ((int64_t)a.C * K));
+    }
     const int tid = threadIdx.x;
     const int y0 = tile * RT;
     load_twiddles<NX2>(s_tw, static_cast<const CT*>(a.tw_x));
@@ -84,7 +100,7 @@ pass_a_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pr
 }
 
 // ----------------------------------------------------------------- pass C
-// block lb = (tile, k, coil group), tile fastest
+// block lb = (k, coil group, tile), k fastest
 template <int NX2, class CT, class ST>
 __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     if (a.skip && *a.skip) return;
@@ -100,9 +116,9 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     const int nx = a.nx, ny = a.ny, K = a.K, C = a.C, G = a.G;
     const int tiles = tiles2d(ny, RT);
     lb = xcd_order(lb, (int64_t)tiles * K * G);
-    const int tile = (int)(lb % tiles);
-    const int k = (int)((lb / tiles) % K);
-    const int grp = (int)(lb / ((int64_t)tiles * K));
+    const int k = (int)(lb % K);
+    const int grp = (int)((lb / K) % G);
+    const int tile = (int)(lb / ((int64_t)K * G));
     const int tid = threadIdx.x;
     const int y0 = tile * RT;
     const int c0 = (int)((int64_t)C * grp / G), c1 = (int)((int64_t)C * (grp + 1) / G);
