@@ -40,7 +40,9 @@ msr_batch* batch_create(msr_ctx* ctx, int nops, msr_normalop* const* ops) {
         MSR_REQUIRE(seen.insert(ops[j]).second, MSR_ERR_ARGUMENT,
                     "ops[%d] appears twice (each operator owns one scratch and CG state)", j);
         bt->ops.push_back(ops[j]);
-        (ops[j]->d == 2 && !ops[j]->cart ? bt->j2d : bt->j3d).push_back(j);
+        // fused 2D operators share grouped launches; the others (3D, Cartesian,
+        // line-kernel apply) run their own passes in the batch's sequence
+        (ops[j]->d == 2 && !ops[j]->cart && ops[j]->fused ? bt->j2d : bt->j3d).push_back(j);
     }
     // launch groups of the 2D passes
     std::map<int, int> ia;

@@ -23,9 +23,15 @@
 namespace msr {
 
 // ----------------------------------------------------------- radix plans
-// R[]: radices in stage order; T: threads per line.  T divides 64 (a line
-// lies in one wavefront) and N/R of every stage (every thread does the same
-// number of butterflies, so a thread holds N/T values of its line).
+// R[]: radices in stage order; T: threads per line, a power of two dividing 64
+// (a line lies in one wavefront).  In an exact plan T also divides N/R of
+// every stage (every thread does the same number of butterflies, so a thread
+// holds N/T values of its line), which the register-resident transforms of
+// the fused apply need; the other plans (lengths with several factors 3 or 5)
+// serve the LDS transforms, whose stages skip the missing butterflies.  Radix
+// choice: as many 8s as the factor 2^a allows, then a 4 or a 2, then the 5s and
+// 3s; T is the largest power of two up to 64 and up to the smallest stage's
+// butterfly count.
 template <int N> struct FFTPlan;
 #define MSR_PLAN(NN, TT, ...)                                   \
     template <> struct FFTPlan<NN> {                            \
@@ -33,33 +39,145 @@ template <int N> struct FFTPlan;
         static constexpr int R[] = {__VA_ARGS__};               \
         static constexpr int NST = sizeof(R) / sizeof(int);     \
     };
+MSR_PLAN(2, 1, 2)
+MSR_PLAN(3, 1, 3)
+MSR_PLAN(4, 1, 4)
+MSR_PLAN(5, 1, 5)
+MSR_PLAN(6, 2, 2, 3)
+MSR_PLAN(8, 1, 8)
+MSR_PLAN(9, 2, 3, 3)
+MSR_PLAN(10, 2, 2, 5)
+MSR_PLAN(12, 2, 4, 3)
+MSR_PLAN(15, 2, 5, 3)
 MSR_PLAN(16, 4, 4, 4)
+MSR_PLAN(18, 4, 2, 3, 3)
+MSR_PLAN(20, 4, 4, 5)
 MSR_PLAN(24, 1, 8, 3)
+MSR_PLAN(25, 4, 5, 5)
+MSR_PLAN(27, 8, 3, 3, 3)
+MSR_PLAN(30, 4, 2, 5, 3)
 MSR_PLAN(32, 4, 8, 4)
+MSR_PLAN(36, 8, 4, 3, 3)
 MSR_PLAN(40, 1, 8, 5)
+MSR_PLAN(45, 8, 5, 3, 3)
 MSR_PLAN(48, 2, 8, 3, 2)
+MSR_PLAN(50, 8, 2, 5, 5)
+MSR_PLAN(54, 16, 2, 3, 3, 3)
+MSR_PLAN(60, 8, 4, 5, 3)
 MSR_PLAN(64, 8, 8, 8)
+MSR_PLAN(72, 8, 8, 3, 3)
+MSR_PLAN(75, 8, 5, 5, 3)
 MSR_PLAN(80, 2, 8, 5, 2)
+MSR_PLAN(81, 16, 3, 3, 3, 3)
+MSR_PLAN(90, 16, 2, 5, 3, 3)
 MSR_PLAN(96, 4, 8, 4, 3)
+MSR_PLAN(100, 16, 4, 5, 5)
+MSR_PLAN(108, 16, 4, 3, 3, 3)
+MSR_PLAN(120, 8, 8, 5, 3)
+MSR_PLAN(125, 16, 5, 5, 5)
 MSR_PLAN(128, 16, 8, 4, 4)
+MSR_PLAN(135, 16, 5, 3, 3, 3)
+MSR_PLAN(144, 16, 8, 2, 3, 3)
+MSR_PLAN(150, 16, 2, 5, 5, 3)
 MSR_PLAN(160, 8, 4, 4, 5, 2)
+MSR_PLAN(162, 32, 2, 3, 3, 3, 3)
+MSR_PLAN(180, 32, 4, 5, 3, 3)
 MSR_PLAN(192, 8, 8, 8, 3)
+MSR_PLAN(200, 16, 8, 5, 5)
+MSR_PLAN(216, 16, 8, 3, 3, 3)
+MSR_PLAN(225, 32, 5, 5, 3, 3)
+MSR_PLAN(240, 16, 8, 2, 5, 3)
+MSR_PLAN(243, 64, 3, 3, 3, 3, 3)
+MSR_PLAN(250, 32, 2, 5, 5, 5)
 MSR_PLAN(256, 32, 8, 8, 4)
+MSR_PLAN(270, 32, 2, 5, 3, 3, 3)
+MSR_PLAN(288, 32, 8, 4, 3, 3)
+MSR_PLAN(300, 32, 4, 5, 5, 3)
 MSR_PLAN(320, 16, 4, 4, 4, 5)
+MSR_PLAN(324, 64, 4, 3, 3, 3, 3)
+MSR_PLAN(360, 32, 8, 5, 3, 3)
+MSR_PLAN(375, 64, 5, 5, 5, 3)
 MSR_PLAN(384, 16, 8, 8, 3, 2)
+MSR_PLAN(400, 32, 8, 2, 5, 5)
+MSR_PLAN(405, 64, 5, 3, 3, 3, 3)
+MSR_PLAN(432, 32, 8, 2, 3, 3, 3)
+MSR_PLAN(450, 64, 2, 5, 5, 3, 3)
+MSR_PLAN(480, 32, 8, 4, 5, 3)
+MSR_PLAN(486, 64, 2, 3, 3, 3, 3, 3)
+MSR_PLAN(500, 64, 4, 5, 5, 5)
 MSR_PLAN(512, 64, 8, 8, 8)
+MSR_PLAN(540, 64, 4, 5, 3, 3, 3)
+MSR_PLAN(576, 64, 8, 8, 3, 3)
+MSR_PLAN(600, 64, 8, 5, 5, 3)
+MSR_PLAN(625, 64, 5, 5, 5, 5)
 MSR_PLAN(640, 16, 8, 8, 5, 2)
+MSR_PLAN(648, 64, 8, 3, 3, 3, 3)
+MSR_PLAN(675, 64, 5, 5, 3, 3, 3)
+MSR_PLAN(720, 64, 8, 2, 5, 3, 3)
+MSR_PLAN(729, 64, 3, 3, 3, 3, 3, 3)
+MSR_PLAN(750, 64, 2, 5, 5, 5, 3)
 MSR_PLAN(768, 32, 8, 8, 4, 3)
+MSR_PLAN(800, 64, 8, 4, 5, 5)
+MSR_PLAN(810, 64, 2, 5, 3, 3, 3, 3)
+MSR_PLAN(864, 64, 8, 4, 3, 3, 3)
+MSR_PLAN(900, 64, 4, 5, 5, 3, 3)
+MSR_PLAN(960, 64, 8, 8, 5, 3)
+MSR_PLAN(972, 64, 4, 3, 3, 3, 3, 3)
+MSR_PLAN(1000, 64, 8, 5, 5, 5)
 MSR_PLAN(1024, 64, 8, 8, 4, 4)
+MSR_PLAN(1080, 64, 8, 5, 3, 3, 3)
+MSR_PLAN(1125, 64, 5, 5, 5, 3, 3)
+MSR_PLAN(1152, 64, 8, 8, 2, 3, 3)
+MSR_PLAN(1200, 64, 8, 2, 5, 5, 3)
+MSR_PLAN(1215, 64, 5, 3, 3, 3, 3, 3)
+MSR_PLAN(1250, 64, 2, 5, 5, 5, 5)
+MSR_PLAN(1280, 64, 8, 8, 4, 5)
+MSR_PLAN(1296, 64, 8, 2, 3, 3, 3, 3)
+MSR_PLAN(1350, 64, 2, 5, 5, 3, 3, 3)
+MSR_PLAN(1440, 64, 8, 4, 5, 3, 3)
+MSR_PLAN(1458, 64, 2, 3, 3, 3, 3, 3, 3)
+MSR_PLAN(1500, 64, 4, 5, 5, 5, 3)
+MSR_PLAN(1536, 64, 8, 8, 8, 3)
+MSR_PLAN(1600, 64, 8, 8, 5, 5)
+MSR_PLAN(1620, 64, 4, 5, 3, 3, 3, 3)
+MSR_PLAN(1728, 64, 8, 8, 3, 3, 3)
+MSR_PLAN(1800, 64, 8, 5, 5, 3, 3)
+MSR_PLAN(1875, 64, 5, 5, 5, 5, 3)
+MSR_PLAN(1920, 64, 8, 8, 2, 5, 3)
+MSR_PLAN(1944, 64, 8, 3, 3, 3, 3, 3)
+MSR_PLAN(2000, 64, 8, 2, 5, 5, 5)
+MSR_PLAN(2025, 64, 5, 5, 3, 3, 3, 3)
+MSR_PLAN(2048, 64, 8, 8, 8, 4)
 #undef MSR_PLAN
 
-// Supported line lengths (each has a plan above).
+// Lengths of the fused Toeplitz apply passes (exact plans; the passes are
+// instantiated per length, coefficient count and precision).
 #define MSR_FFT_LENGTHS(X) \
     X(16) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) \
     X(512) X(640) X(768) X(1024)
 
-// Invoke F.template operator()<N>() for a runtime length n; returns false if
-// n has no plan.
+// Lengths of the line kernels (every 2,3,5-smooth length from 2 to 2048), in
+// 6 parts instantiated by separate translation units (fftlines_p*.hip).
+#define MSR_FFT_LINE_LENGTHS_P0(X) X(2) X(9) X(20) X(36) X(60) X(90) X(128) X(180) X(243) X(320) \
+    X(405) X(512) X(648) X(800) X(1000) X(1215) X(1458) X(1800) X(2048)
+#define MSR_FFT_LINE_LENGTHS_P1(X) X(3) X(10) X(24) X(40) X(64) X(96) X(135) X(192) X(250) \
+    X(324) X(432) X(540) X(675) X(810) X(1024) X(1250) X(1500) X(1875)
+#define MSR_FFT_LINE_LENGTHS_P2(X) X(4) X(12) X(25) X(45) X(72) X(100) X(144) X(200) X(256) \
+    X(360) X(450) X(576) X(720) X(864) X(1080) X(1280) X(1536) X(1920)
+#define MSR_FFT_LINE_LENGTHS_P3(X) X(5) X(15) X(27) X(48) X(75) X(108) X(150) X(216) X(270) \
+    X(375) X(480) X(600) X(729) X(900) X(1125) X(1296) X(1600) X(1944)
+#define MSR_FFT_LINE_LENGTHS_P4(X) X(6) X(16) X(30) X(50) X(80) X(120) X(160) X(225) X(288) \
+    X(384) X(486) X(625) X(750) X(960) X(1152) X(1350) X(1620) X(2000)
+#define MSR_FFT_LINE_LENGTHS_P5(X) X(8) X(18) X(32) X(54) X(81) X(125) X(162) X(240) X(300) \
+    X(400) X(500) X(640) X(768) X(972) X(1200) X(1440) X(1728) X(2025)
+#define MSR_FFT_LINE_PARTS 6
+
+#define MSR_FFT_LINE_LENGTHS(X)                                                        \
+    MSR_FFT_LINE_LENGTHS_P0(X) MSR_FFT_LINE_LENGTHS_P1(X) MSR_FFT_LINE_LENGTHS_P2(X)         \
+    MSR_FFT_LINE_LENGTHS_P3(X) MSR_FFT_LINE_LENGTHS_P4(X) MSR_FFT_LINE_LENGTHS_P5(X)
+
+// Invoke F.template operator()<N>() for a runtime length n of the fused set;
+// returns false if n is not one.
 template <class F>
 inline bool dispatch_fft_len(int n, F&& f) {
 #define MSR_CASE(NN)                     \
@@ -73,11 +191,38 @@ inline bool dispatch_fft_len(int n, F&& f) {
 #undef MSR_CASE
 }
 
-inline bool fft_len_supported(int64_t n) {
+// The same over every line length (host-side tables: twiddles).
+template <class F>
+inline bool dispatch_line_len(int n, F&& f) {
+#define MSR_CASE(NN)                     \
+    case NN:                             \
+        f.template operator()<NN>();     \
+        return true;
+    switch (n) {
+        MSR_FFT_LINE_LENGTHS(MSR_CASE)
+        default: return false;
+    }
+#undef MSR_CASE
+}
+
+// A length of the fused apply passes.
+inline bool fused_len_supported(int64_t n) {
 #define MSR_CASE(NN) \
     case NN: return true;
     switch (n) {
         MSR_FFT_LENGTHS(MSR_CASE)
+        default: return false;
+    }
+#undef MSR_CASE
+}
+
+// A length of the line kernels (backprojection, forward, kernel build,
+// Cartesian operator, the general Toeplitz apply).
+inline bool fft_len_supported(int64_t n) {
+#define MSR_CASE(NN) \
+    case NN: return true;
+    switch (n) {
+        MSR_FFT_LINE_LENGTHS(MSR_CASE)
         default: return false;
     }
 #undef MSR_CASE

@@ -1,182 +1,28 @@
-// Generic batched 1D FFT line passes (see fftlines.hpp).
-#include "fft.hpp"
+// Dispatch of the batched 1D FFT line passes (see fftlines.hpp) to the
+// translation units that instantiate each length (fftlines_p*.hip).
 #include "fftlines.hpp"
+#include "fft.hpp"
 
 namespace msr {
 
-__device__ __forceinline__ int select_bin(int j, int lout, int nf, int sel) {
-    int r;
-    if (sel == SEL_FIRST) return j;
-    if (sel == SEL_CENTER) r = j - lout / 2;
-    else r = j < (lout + 1) / 2 ? j : j - lout;
-    return r < 0 ? r + nf : r;
-}
+using LinePartFn = bool (*)(const LinePass&, const void*, bool, hipStream_t);
+static const LinePartFn kLineParts[MSR_FFT_LINE_PARTS] = {
+    line_pass_part0, line_pass_part1, line_pass_part2,
+    line_pass_part3, line_pass_part4, line_pass_part5};
 
-template <int NF, class CT>
-constexpr int rows_block() {
-    constexpr int R = sizeof(CT) == 16 ? 8 : 16;  // lines per workgroup (LDS)
-    return (R * FFTPlan<NF>::T) < 256 ? R * FFTPlan<NF>::T : 256;
-}
-
-template <int NF, int DIR, class CT>
-__global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restrict__ tw) {
-    if (p.skip && *p.skip) return;
-    using RT = real_t<CT>;
-    constexpr int TP = FFTPlan<NF>::T;
-    constexpr int BLK = rows_block<NF, CT>();
-    constexpr int F = BLK / TP;
-    constexpr int LD = lds_line<NF>() + 1;
-    __shared__ CT s_tw[NF];
-    __shared__ CT s[F * LD];
-    const CT* in = static_cast<const CT*>(p.in);
-    CT* out = static_cast<CT*>(p.out);
-    load_twiddles<NF>(s_tw, tw);
-    const int64_t l0 = (int64_t)blockIdx.x * F;
-    if (p.isel == SEL_FIRST && !p.win && !p.win64) {
-        for (int e = threadIdx.x; e < F * NF; e += BLK) {
-            const int l = e / NF, i = e - l * NF;
-            const int64_t line = l0 + l;
-            CT v = cz<CT>();
-            if (line < p.nlines && i < p.lin) v = in[line * p.in_pitch + i];
-            s[l * LD + lds_pad(i)] = v;
+static void run_line_pass(const LinePass& p, msr_ctx* ctx, bool rows, hipStream_t st) {
+    MSR_REQUIRE(fft_len_supported(p.nf), MSR_ERR_UNSUPPORTED, "no FFT kernel for length %d", p.nf);
+    const void* tw = p.f64 ? (const void*)twiddle_table64(ctx, p.nf) : (const void*)twiddle_table(ctx, p.nf);
+    for (LinePartFn f : kLineParts)
+        if (f(p, tw, rows, st)) {
+            MSR_CHECK_LAUNCH();
+            return;
         }
-    } else {
-        for (int e = threadIdx.x; e < F * NF; e += BLK) s[(e / NF) * LD + lds_pad(e % NF)] = cz<CT>();
-        __syncthreads();
-        for (int e = threadIdx.x; e < F * p.lin; e += BLK) {
-            const int l = e / p.lin, i = e - l * p.lin;
-            const int64_t line = l0 + l;
-            if (line < p.nlines) {
-                CT v = in[line * p.in_pitch + i];
-                if (p.win64) v = cscale(v, (RT)p.win64[i]);
-                else if (p.win) v = cscale(v, (RT)p.win[i]);
-                s[l * LD + lds_pad(select_bin(i, p.lin, NF, p.isel))] = v;
-            }
-        }
-    }
-    __syncthreads();
-    const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
-    fft_lds<NF, DIR>(s + g * LD, s_tw, t);
-    for (int e = threadIdx.x; e < F * p.lout; e += BLK) {
-        const int l = e / p.lout, j = e - l * p.lout;
-        const int64_t line = l0 + l;
-        if (line >= p.nlines) continue;
-        CT v = s[l * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
-        if (p.w64) v = cscale(v, (RT)p.w64[j]);
-        else if (p.w) v = cscale(v, (RT)p.w[j]);
-        out[line * p.out_pitch + j] = v;
-    }
+    fail(MSR_ERR_UNSUPPORTED, "no FFT kernel for length %d", p.nf);
 }
 
-template <int NF, class CT>
-constexpr int cols_wc() {
-    return (NF <= 512 ? 16 : 8) / (sizeof(CT) == 16 ? 2 : 1);
-}
+void run_rows(const LinePass& p, msr_ctx* ctx, hipStream_t st) { run_line_pass(p, ctx, true, st); }
 
-template <int NF, int DIR, class CT>
-__global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restrict__ tw) {
-    if (p.skip && *p.skip) return;
-    using RT = real_t<CT>;
-    constexpr int TP = FFTPlan<NF>::T;
-    constexpr int WC = cols_wc<NF, CT>();
-    constexpr int BLK = (WC * TP) < 256 ? WC * TP : 256;
-    constexpr int F = BLK / TP;
-    constexpr int LD = lds_line<NF>() + 1;
-    __shared__ CT s_tw[NF];
-    __shared__ CT s[WC * LD];
-    const CT* in = static_cast<const CT*>(p.in);
-    CT* out = static_cast<CT*>(p.out);
-    load_twiddles<NF>(s_tw, tw);
-    const int64_t col0 = (int64_t)blockIdx.x * WC;
-    const int64_t b = blockIdx.y;
-    if (p.isel == SEL_FIRST && !p.win && !p.win64) {
-        for (int e = threadIdx.x; e < WC * NF; e += BLK) {
-            const int wc = e % WC, i = e / WC;
-            const int64_t col = col0 + wc;
-            CT v = cz<CT>();
-            if (col < p.ncols && i < p.lin) v = in[b * p.in_bstride + i * p.in_stride + col];
-            s[wc * LD + lds_pad(i)] = v;
-        }
-    } else {
-        for (int e = threadIdx.x; e < WC * NF; e += BLK) s[(e % WC) * LD + lds_pad(e / WC)] = cz<CT>();
-        __syncthreads();
-        for (int e = threadIdx.x; e < WC * p.lin; e += BLK) {
-            const int wc = e % WC, i = e / WC;
-            const int64_t col = col0 + wc;
-            if (col < p.ncols) {
-                CT v = in[b * p.in_bstride + i * p.in_stride + col];
-                if (p.win64) v = cscale(v, (RT)p.win64[i]);
-                else if (p.win) v = cscale(v, (RT)p.win[i]);
-                s[wc * LD + lds_pad(select_bin(i, p.lin, NF, p.isel))] = v;
-            }
-        }
-    }
-    __syncthreads();
-    const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
-#pragma unroll
-    for (int r0 = 0; r0 < WC; r0 += F) fft_lds<NF, DIR>(s + (r0 + g) * LD, s_tw, t);
-    for (int e = threadIdx.x; e < WC * p.lout; e += BLK) {
-        const int wc = e % WC, j = e / WC;
-        const int64_t col = col0 + wc;
-        if (col >= p.ncols) continue;
-        CT v = s[wc * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
-        if (p.w64) v = cscale(v, (RT)p.w64[j]);
-        else if (p.w) v = cscale(v, (RT)p.w[j]);
-        if (!p.lam) {
-            out[b * p.out_bstride + j * p.out_stride + col] = v;
-        } else {
-            // dense Toeplitz kernel, rounded once to ComplexF32 / Float32
-            const int64_t pp = b + p.p0;
-            int64_t off, mi;
-            if (!p.lam3d) {
-                off = (col * p.P + pp) * p.lout + j;
-                mi = (int64_t)j * p.nx2 + col;
-            } else {
-                off = (col * p.P + pp) * p.lout + j;           // col = qy*nx2 + qx
-                mi = (int64_t)j * p.ny2 * p.nx2 + col;         // j = qz
-            }
-            const bool on = p.mask[mi] != 0;
-            if (p.lam_real) reinterpret_cast<float*>(p.lam)[off] = on ? (float)v.x : 0.f;
-            else reinterpret_cast<float2*>(p.lam)[off] = on ? make_float2((float)v.x, (float)v.y)
-                                                           : make_float2(0.f, 0.f);
-        }
-    }
-}
-
-template <class CT>
-static void run_rows_t(const LinePass& p, const CT* tw, hipStream_t st) {
-    bool ok = dispatch_fft_len(p.nf, [&]<int N>() {
-        constexpr int BLK = rows_block<N, CT>();
-        constexpr int F = BLK / FFTPlan<N>::T;
-        dim3 grid((unsigned)ceil_div(p.nlines, F));
-        if (p.dir < 0) hipLaunchKernelGGL((rows_kernel<N, -1, CT>), grid, dim3(BLK), 0, st, p, tw);
-        else hipLaunchKernelGGL((rows_kernel<N, 1, CT>), grid, dim3(BLK), 0, st, p, tw);
-    });
-    MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "no FFT kernel for length %d", p.nf);
-    MSR_CHECK_LAUNCH();
-}
-
-template <class CT>
-static void run_cols_t(const LinePass& p, const CT* tw, hipStream_t st) {
-    bool ok = dispatch_fft_len(p.nf, [&]<int N>() {
-        constexpr int WC = cols_wc<N, CT>();
-        constexpr int BLK = (WC * FFTPlan<N>::T) < 256 ? WC * FFTPlan<N>::T : 256;
-        dim3 grid((unsigned)ceil_div(p.ncols, WC), (unsigned)p.nb);
-        if (p.dir < 0) hipLaunchKernelGGL((cols_kernel<N, -1, CT>), grid, dim3(BLK), 0, st, p, tw);
-        else hipLaunchKernelGGL((cols_kernel<N, 1, CT>), grid, dim3(BLK), 0, st, p, tw);
-    });
-    MSR_REQUIRE(ok, MSR_ERR_UNSUPPORTED, "no FFT kernel for length %d", p.nf);
-    MSR_CHECK_LAUNCH();
-}
-
-void run_rows(const LinePass& p, msr_ctx* ctx, hipStream_t st) {
-    if (p.f64) run_rows_t<double2>(p, twiddle_table64(ctx, p.nf), st);
-    else run_rows_t<float2>(p, twiddle_table(ctx, p.nf), st);
-}
-
-void run_cols(const LinePass& p, msr_ctx* ctx, hipStream_t st) {
-    if (p.f64) run_cols_t<double2>(p, twiddle_table64(ctx, p.nf), st);
-    else run_cols_t<float2>(p, twiddle_table(ctx, p.nf), st);
-}
+void run_cols(const LinePass& p, msr_ctx* ctx, hipStream_t st) { run_line_pass(p, ctx, false, st); }
 
 }  // namespace msr

@@ -53,4 +53,14 @@ struct LinePass {
 void run_rows(const LinePass& p, msr_ctx* ctx, hipStream_t st);
 void run_cols(const LinePass& p, msr_ctx* ctx, hipStream_t st);
 
+// per-length launchers, one per translation unit (fftlines_p*.hip): launch
+// the rows (rows = true) or cols pass of p when p.nf is one of the part's
+// lengths (tw: the length's twiddle table of p's precision) and return true
+bool line_pass_part0(const LinePass& p, const void* tw, bool rows, hipStream_t st);
+bool line_pass_part1(const LinePass& p, const void* tw, bool rows, hipStream_t st);
+bool line_pass_part2(const LinePass& p, const void* tw, bool rows, hipStream_t st);
+bool line_pass_part3(const LinePass& p, const void* tw, bool rows, hipStream_t st);
+bool line_pass_part4(const LinePass& p, const void* tw, bool rows, hipStream_t st);
+bool line_pass_part5(const LinePass& p, const void* tw, bool rows, hipStream_t st);
+
 }  // namespace msr

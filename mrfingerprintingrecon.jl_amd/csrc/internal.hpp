@@ -140,6 +140,8 @@ struct msr_normalop {
     msr::DevBuf partial;              // (G, K, V)
     size_t scratch_es = 0;            // element size T / T2 / partial are allocated for
     int G = 1, Gb = 1;
+    bool fused = true;                // fused Toeplitz passes (else the line-kernel apply)
+    int Cb = 1;                       // coils per chunk of the line-kernel apply
     // CG state (allocated on first use)
     msr::DevBuf cg_r, cg_u, cg_c, cg_scal, cg_part, cg_x64, cg_u32;
     int cg_fp64 = -1;                 // precision (MSR_PREC_*) of the last started CG

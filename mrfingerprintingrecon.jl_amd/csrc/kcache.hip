@@ -284,7 +284,7 @@ static KernelFileHeader read_kernel_file(const char* path, std::vector<uint8_t>*
                 path);
     MSR_REQUIRE(h.version == 1 && h.header_bytes == sizeof(h), MSR_ERR_UNSUPPORTED,
                 "%s: kernel file version %u is not supported", path, h.version);
-    MSR_REQUIRE((h.d == 2 || h.d == 3) && h.K >= 1 && h.K <= 8 && h.P == h.K * (h.K + 1) / 2 &&
+    MSR_REQUIRE((h.d == 2 || h.d == 3) && h.K >= 1 && h.K <= (uint32_t)MAX_COEFFS && h.P == h.K * (h.K + 1) / 2 &&
                     h.lambda_real <= 1,
                 MSR_ERR_IO, "%s: inconsistent header (d %u, K %u, P %u)", path, h.d, h.K, h.P);
     int64_t G = 1;

@@ -91,7 +91,12 @@ static int env_int(const char* name, int dflt) {
 
 void op_alloc_scratch(msr_normalop* op) {
     const int64_t ny = op->N[1];
-    if (op->d == 2 && !op->cart) {
+    op->Cb = 1;
+    if (!op->cart && !op->fused) {
+        op->G = 1;
+        op->Gb = 1;
+        op->Cb = lines_chunk_coils(op);
+    } else if (op->d == 2 && !op->cart) {
         const int tiles = tiles2d((int)ny, 8);
         int G = (int)ceil_div(512, (int64_t)tiles * op->K);
         G = std::max(1, std::min(G, op->C));
@@ -123,6 +128,9 @@ void op_ensure_scratch(msr_normalop* op, bool f64) {
     if (op->cart) {  // two ping-pong grids (C, K, N...)
         op->T.alloc(es * (size_t)op->C * op->K * op->V);
         op->T2.alloc(es * (size_t)op->C * op->K * op->V);
+    } else if (!op->fused) {  // two ping-pong grids (Cb, K, 2N...)
+        op->T.alloc(lines_scratch_bytes(op, es));
+        op->T2.alloc(lines_scratch_bytes(op, es));
     } else if (op->d == 2) {
         op->T.alloc(es * (size_t)op->C * op->K * (2 * nx) * ny);
     } else {
@@ -141,6 +149,18 @@ static void check_shape_cart(int d, const int64_t* N) {
                     (long long)N[i]);
 }
 
+// The fused passes take 2N in the fused length set up to 512 along the axes
+// they transform in LDS (x and y in 2D, x and z in 3D) and K <= 8; every other
+// shape the line kernels cover runs the line-kernel apply.
+static bool fused_shape(int d, const int64_t* N, int K) {
+    if (K > 8) return false;
+    for (int i = 0; i < d; ++i) {
+        if (d == 3 && i == 1) continue;  // the 3D y pass runs on the line kernels
+        if (!fused_len_supported(2 * N[i]) || 2 * N[i] > 512) return false;
+    }
+    return true;
+}
+
 static void check_shape(int d, const int64_t* N) {
     MSR_REQUIRE(d == 2 || d == 3, MSR_ERR_ARGUMENT, "image dimension d = %d, expected 2 or 3", d);
     for (int i = 0; i < d; ++i) {
@@ -151,9 +171,9 @@ static void check_shape(int d, const int64_t* N) {
         MSR_REQUIRE(m == 1, MSR_ERR_ASSERTION,
                     "img_shape_os has to be composed of the prime factors 2, 3, and 5 "
                     "(2N[%d] = %lld)", i, (long long)(2 * N[i]));
-        MSR_REQUIRE(fft_len_supported(2 * N[i]) && 2 * N[i] <= 512, MSR_ERR_UNSUPPORTED,
-                    "Toeplitz grid size 2N[%d] = %lld has no FFT kernel in this build",
-                    i, (long long)(2 * N[i]));
+        MSR_REQUIRE(fft_len_supported(2 * N[i]), MSR_ERR_UNSUPPORTED,
+                    "Toeplitz grid size 2N[%d] = %lld has no FFT kernel in this build "
+                    "(2,3,5-smooth lengths up to 2048)", i, (long long)(2 * N[i]));
     }
 }
 
@@ -223,7 +243,9 @@ void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, 
 msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real, int cart) {
     if (cart) check_shape_cart(d, N);
     else check_shape(d, N);
-    MSR_REQUIRE(K >= 1 && K <= 8, MSR_ERR_UNSUPPORTED, "K = %d coefficients, supported 1..8", K);
+    MSR_REQUIRE(K >= 1, MSR_ERR_ARGUMENT, "K = %d coefficients", K);
+    MSR_REQUIRE(K <= (cart ? 8 : MAX_COEFFS), MSR_ERR_UNSUPPORTED,
+                "K = %d coefficients, supported 1..%d", K, cart ? 8 : MAX_COEFFS);
     auto op = std::make_unique<msr_normalop>();
     op->ctx = ctx;
     op->d = d;
@@ -234,6 +256,9 @@ msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real,
     op->P = cart ? K * K : K * (K + 1) / 2;
     op->lam_real = lam_real;
     op->cart = cart ? 1 : 0;
+    // MSR_APPLY_LINES=1 sends every non-Cartesian operator to the line-kernel
+    // apply (a cross-check of the two paths on shapes both cover)
+    op->fused = !cart && fused_shape(d, N, K) && env_int("MSR_APPLY_LINES", 0) == 0;
     return op.release();
 }
 
@@ -278,6 +303,8 @@ void op_passes(msr_normalop* op, const void* b, int am, hipStream_t st, const in
                 "apply scratch not allocated for this precision");
     if (op->cart) {
         launch_cart_passes(op, b, f64, st, skip);
+    } else if (!op->fused) {
+        launch_lines_passes(op, b, f64, st, skip);
     } else if (op->d == 2) {
         Apply2DArgs a = args2d(op, am);
         a.b = b;
@@ -310,7 +337,9 @@ void op_apply_percoil(msr_normalop* op, float2* x, const float2* b, float2 alpha
                 "the per-coil apply order exists for the non-Cartesian operator only");
     launch_scale<float2>(x, op->V * op->K, beta, beta_zero, st);
     for (int c = 0; c < op->C; ++c) {
-        if (op->d == 2) {
+        if (!op->fused) {
+            launch_lines_passes(op, b, false, st, nullptr, c, c + 1);
+        } else if (op->d == 2) {
             Apply2DArgs a = args2d(op, AM_F32);
             a.C = 1;
             a.G = 1;
@@ -358,6 +387,7 @@ size_t op_footprint(const msr_normalop* op, int prec) {
     const int64_t nx = op->N[0], ny = op->N[1], nz = op->N[2];
     size_t scratch = es * (size_t)op->G * op->K * op->V;
     if (op->cart) scratch += 2 * es * (size_t)op->C * op->K * op->V;
+    else if (!op->fused) scratch += 2 * lines_scratch_bytes(op, es);
     else if (op->d == 2) scratch += es * (size_t)op->C * op->K * (2 * nx) * ny;
     else scratch += es * (size_t)op->C * op->K * nz * ny * 2 * nx * 3;  // T1 + T2
     size_t cg = vs * n * 2 + es * n;                                   // r, u, c

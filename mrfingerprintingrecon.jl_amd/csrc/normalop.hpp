@@ -13,6 +13,16 @@ msr_normalop* op_new(msr_ctx* ctx, int d, const int64_t* N, int K, int lam_real,
 // Cartesian operator passes (cartesian.hip): partial <- unscaled sum_c conj(S_c)
 // IFFT(Λ . FFT(S_c b)) on the N grid
 void launch_cart_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st, const int* skip);
+// General Toeplitz passes on the line kernels (apply_lines.hip), for the
+// shapes and coefficient counts outside the fused passes: partial <- unscaled
+// sum over coils c_begin..c_end-1 (c_end < 0: all) of conj(S_c) crop(IFFT(
+// Lambda . FFT(pad(S_c b)))), G = 1.
+void launch_lines_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st,
+                         const int* skip, int c_begin = 0, int c_end = -1);
+int lines_chunk_coils(const msr_normalop* op);
+size_t lines_scratch_bytes(const msr_normalop* op, size_t es);
+// the largest coefficient count (the fused passes take up to 8)
+constexpr int MAX_COEFFS = 64;
 void op_set_cmaps(msr_normalop* op, int ncoils, const void* const* cmaps, hipStream_t st);
 void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, int64_t nkm,
                     hipStream_t st);

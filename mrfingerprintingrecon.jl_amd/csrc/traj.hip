@@ -50,7 +50,7 @@ static std::vector<double2> twiddles_f64(int n) {
 static std::vector<double2> stage_twiddles_f64(int n) {
     const std::vector<double2> w = twiddles_f64(n);
     std::vector<double2> h(n, make_double2(0.0, 0.0));
-    const bool ok = dispatch_fft_len(n, [&]<int N>() {
+    const bool ok = dispatch_line_len(n, [&]<int N>() {
         using Pl = FFTPlan<N>;
         int ns = Pl::R[0];
         for (int s = 1; s < Pl::NST; ++s) {

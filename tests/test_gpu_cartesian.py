@@ -205,7 +205,7 @@ def test_cartesian_operator_in_a_batch_and_errors(torch, msr):
     with pytest.raises(NotImplementedError):
         A.A.mul_(torch.empty_like(bA), bA, percoil=True)
     with pytest.raises(NotImplementedError):
-        msr.FFTNormalOp((30, 24), dev(torch, trj), dev(torch, U))  # no FFT of length 30
+        msr.FFTNormalOp((31, 24), dev(torch, trj), dev(torch, U))  # no FFT of prime length 31
 
 
 def test_cartesian_backprojection_inverts_nufft_on_the_grid(torch, msr):

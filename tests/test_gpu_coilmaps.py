@@ -76,7 +76,7 @@ def test_calibration_kspace_matches_centred_fft(torch, msr, shape):
 
 
 def test_calibration_kspace_rejects_lengths_without_fft(torch, msr):
-    x = torch.zeros((2, 27, 27), dtype=torch.complex64, device="cuda")
+    x = torch.zeros((2, 31, 31), dtype=torch.complex64, device="cuda")  # prime: no FFT
     with pytest.raises(NotImplementedError):
         msr.calibration_kspace(x)
 
