@@ -128,6 +128,9 @@ def main():
     ap.add_argument("--precision", default=PRECISION, choices=["f64", "f64c"],
                     help="CG precision of the measured solve (FP64 vectors and apply; f64c "
                          "stores the apply's FFT intermediates in 8 bytes)")
+    ap.add_argument("--tuning", metavar="FILE",
+                    help="tuned apply choices: load FILE if it exists (no tuning runs), else "
+                         "tune and write it; without the option the benchmark tunes in-process")
     ap.add_argument("--apply-only", type=int, default=0, metavar="N",
                     help="profiling mode: build the operator, run N fused applies, N per-coil "
                          "applies and a 256 MiB calibration copy, then exit")
@@ -153,6 +156,10 @@ def main():
     cml = [torch.from_numpy(c).to(dev) for c in cm]
     data_d = torch.from_numpy(data).to(dev)
 
+    loaded = 0
+    if args.tuning and os.path.exists(args.tuning):
+        loaded = msr.load_tuning(args.tuning)
+        log(f"tuning: {loaded} entries from {args.tuning}")
     log("trajectory plan + kernel build")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -160,6 +167,16 @@ def main():
     A = msr.NFFTNormalOp(N_IMG, traj, U_d, cmaps=cml)
     torch.cuda.synchronize()
     build_ms = (time.perf_counter() - t0) * 1e3
+    # measured launch choices of the apply (the counterpart of the reference's
+    # FFTW.MEASURE planning at operator construction), outside the timed solves
+    t0 = time.perf_counter()
+    tuning = {}
+    for p in ("f64c", "f64", "f32"):
+        tuning[p] = A.A.tune_(p, nrep=0 if loaded else 5)["chosen"]
+    tune_ms = (time.perf_counter() - t0) * 1e3
+    if args.tuning and not loaded:
+        msr.save_tuning(args.tuning)
+    log(f"tuning ({'loaded' if loaded else f'{tune_ms:.0f} ms'}): {tuning}")
     nvec = K * N_IMG[0] * N_IMG[1]
     if args.rows:
         rows_leg(msr, torch, dev, trj_d, traj, U_d, cml, data_d)
@@ -311,6 +328,8 @@ def main():
                       "speedup": round(solves_per_s / plain_sps, 3),
                       "apply_ms": round(t_apply_plain, 4)},
         "kernel_build_ms": round(build_ms, 1),
+        "tuning": {"chosen_Gb_orderA_orderC": tuning, "source": "file" if loaded else "measured",
+                   "ms": None if loaded else round(tune_ms, 1)},
         "roofline": roofline,
         "roofline_other_applies": roofline_other,
         "cpu_baseline": None,
@@ -510,6 +529,7 @@ def batched_leg(msr, njobs=64, iters=CG_ITERS, seed=1004):
     rng = np.random.default_rng(seed)
     jobs = []
     log(f"batched: building {njobs} jobs")
+    tuned = set()
     for i in range(njobs):
         N = int(rng.choice([128, 160, 192, 256]))
         C = int(rng.integers(8, 33))
@@ -519,6 +539,9 @@ def batched_leg(msr, njobs=64, iters=CG_ITERS, seed=1004):
         trj, U, cml, data = _job_inputs(msr, torch, dev, rng, N, C, K, spokes, nt, seed * 100 + i)
         traj = msr.plan_trajectory(trj)
         A = msr.NFFTNormalOp((N, N), traj, U, cmaps=cml)
+        if (N, C, K) not in tuned:  # later operators of the shape start with the choice
+            A.A.tune_(PRECISION, nrep=3)
+            tuned.add((N, C, K))
         b = torch.empty((K, N, N), dtype=torch.complex64, device=dev)
         jobs.append(dict(N=N, C=C, K=K, M=traj.M, traj=traj, U=U, cml=cml, data=data, A=A, b=b,
                          x1=torch.empty(K * N * N, dtype=torch.complex64, device=dev),

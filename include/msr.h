@@ -110,7 +110,12 @@ int msr_traj_destroy(msr_traj *traj);
  * the real symmetric kernel).  cmaps: host array of ncoils device pointers, or
  * NULL with ncoils = 0 for the reference default cmaps = (1,).
  * AssertionError if a grid size 2N is not 2,3,5-smooth
- * (src/NFFTNormalOp.jl:107). */
+ * (src/NFFTNormalOp.jl:107).  Supported: every 2,3,5-smooth 2N up to 2048
+ * (the kernel construction's 4N up to 2048, so N up to 512 there) and K up to
+ * 64; MSR_ERR_UNSUPPORTED beyond.  The applies run fused passes for 2N in
+ * {16, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512} along
+ * the axes they transform in LDS (x and y in 2D, x and z in 3D) and K <= 8,
+ * and the line-kernel passes otherwise (same results to rounding). */
 int msr_normalop_create_from_traj(msr_ctx *ctx, const msr_traj *traj, int d,
                                   const int64_t *img_shape, int K, const void *U, int u_real,
                                   int ncoils, const void *const *cmaps, void *stream,
@@ -234,6 +239,34 @@ int msr_normalop_apply_f64(msr_normalop *op, void *x, const void *b, double alph
  * reference (see msr_normalop_apply_f64). */
 int msr_normalop_apply_f64c(msr_normalop *op, void *x, const void *b, double alpha_re,
                             double alpha_im, double beta_re, double beta_im, void *stream);
+
+/* Measured launch configuration of the fused 2D apply at a CG precision's
+ * apply mode (MSR_PREC_F32 and MSR_PREC_MIXED share the ComplexF32 apply):
+ * the counterpart of the reference's measured FFT planning, plan_fft! with
+ * FFTW.MEASURE (src/NFFTNormalOp.jl:71-72,141).  Candidates: pass B coil
+ * groups Gb in {1, 2, 4, 8} (up to the coil count) times the two block orders of
+ * passes A and C, index 4 log2(Gb) + 2 orderA + orderC; each runs nrep
+ * applies (twice over), and must reproduce the first candidate's result bit
+ * for bit (MSR_ERR_ASSERTION otherwise).  The fastest is kept for the
+ * operator, for operators of the same shape created later on the context,
+ * and for batches made afterwards.  times_ms: MSR_TUNE_CONFIGS doubles (ms per
+ * apply, NaN for candidates not run) or NULL; *chosen: the kept index, or -1
+ * when the operator has no fused 2D passes (3D, Cartesian, line-kernel apply).
+ * nrep = 0 measures nothing and reports the current index.  Synchronises the
+ * host. */
+#define MSR_TUNE_CONFIGS 16
+int msr_normalop_tune(msr_normalop *op, int precision, int nrep, double *times_ms, int *chosen,
+                      void *stream);
+
+/* The context's tuned choices to / from a text file (the counterpart of
+ * FFTW's exported wisdom, which the reference's FFTW.MEASURE planning can
+ * reuse): save writes every (shape, apply mode) msr_normalop_tune measured on
+ * the context; load adds the file's entries to the context, so operators
+ * created afterwards start with them and no tuning runs.  load refuses a file
+ * measured on another device type (MSR_ERR_ARGUMENT) and a malformed one
+ * (MSR_ERR_IO); *nloaded (may be NULL) = the number of entries. */
+int msr_tune_save(msr_ctx *ctx, const char *path);
+int msr_tune_load(msr_ctx *ctx, const char *path, int *nloaded);
 
 /* -------------------------------------------------------------- batches */
 /* Many reconstructions (uneven image size, coil count and K, 2D and 3D)

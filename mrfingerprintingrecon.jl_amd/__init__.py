@@ -31,7 +31,8 @@ __all__ = ["NFFTNormalOp", "calculate_backprojection", "forward", "cg", "cg_", "
            "LinearOperator", "plan_trajectory", "ArgumentError", "ErrorException", "synth",
            "kernel_key", "save_kernel", "load_kernel", "kernel_file_info", "KernelCache",
            "reconstruct_coilwise", "calibration_region", "calibration_kspace",
-           "calculate_coil_maps", "default_calib_size", "FFTNormalOp"]
+           "calculate_coil_maps", "default_calib_size", "FFTNormalOp", "save_tuning",
+           "load_tuning"]
 
 _ctxs = {}
 
@@ -241,6 +242,25 @@ class _NFFTNormalOp:
         _lib.call("msr_normalop_apply_f64c" if compact else "msr_normalop_apply_f64", self.h,
                   _ptr(x), _ptr(b), a.real, a.imag, bt.real, bt.imag, _stream_ptr(self.device))
         return x
+
+    def tune_(self, precision="f32", nrep=5):
+        """Measured launch configuration of the fused 2D apply for a CG
+        precision (the counterpart of plan_fft! with FFTW.MEASURE,
+        src/NFFTNormalOp.jl:71-72): every candidate (pass B coil groups
+        1/2/4/8, pass A and pass C block orders) runs nrep applies and must
+        reproduce the first candidate's result bit for bit; the fastest is
+        kept.  Returns {"times": ms per apply per candidate (NaN: not run),
+        "chosen": (Gb, order_a, order_c) or None when the operator has no
+        fused 2D passes}.  nrep = 0 reports the current choice only."""
+        if precision not in _lib.CG_PRECISIONS:
+            raise ArgumentError(f"precision must be one of {sorted(_lib.CG_PRECISIONS)}")
+        times = (ctypes.c_double * _lib.TUNE_CONFIGS)()
+        chosen = ctypes.c_int(-1)
+        _lib.call("msr_normalop_tune", self.h, _lib.CG_PRECISIONS[precision], int(nrep), times,
+                  ctypes.byref(chosen), _stream_ptr(self.device))
+        c = chosen.value
+        knobs = None if c < 0 else (1 << (c // 4), (c // 2) % 2, c % 2)
+        return {"times": [float(t) for t in times], "chosen": knobs}
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None and _lib._lib is not None:
@@ -489,6 +509,26 @@ def load_kernel(path, cmaps=None, key=None, device=None):
     _lib.call("msr_kernel_load", _ctx(dev.index), os.fsencode(path), _key_arg(key), len(cl), carr,
               _stream_ptr(dev), ctypes.byref(h))
     return LinearOperator(_NFFTNormalOp(h, info["img_shape"], dev, cl))
+
+
+def _dev_index(device):
+    torch = _torch()
+    return torch.cuda.current_device() if device is None else (torch.device(device).index or 0)
+
+
+def save_tuning(path, device=None):
+    """Writes the tuned apply choices measured on the device's context
+    (msr_tune_save), the counterpart of FFTW's exported wisdom."""
+    _lib.call("msr_tune_save", _ctx(_dev_index(device)), os.fsencode(path))
+
+
+def load_tuning(path, device=None):
+    """Adds a tuning file's choices to the device's context (msr_tune_load):
+    operators created afterwards start with them.  Returns the number of
+    entries; ArgumentError for a file measured on another device type."""
+    n = ctypes.c_int()
+    _lib.call("msr_tune_load", _ctx(_dev_index(device)), os.fsencode(path), ctypes.byref(n))
+    return n.value
 
 
 def kernel_file_info(path, verify=True):

@@ -22,6 +22,7 @@ DCF_MODES = {"none": 0, "radial_2D": 1, "radial_3D": 2}
 DCF_ARRAY = 3
 CG_MODES = {"device": 0, "host": 1, "graph": 2}
 CG_PRECISIONS = {"f32": 0, "mixed": 1, "f64": 2, "f64c": 3}
+TUNE_CONFIGS = 16
 
 
 class ArgumentError(ValueError):
@@ -73,6 +74,9 @@ SIGNATURES = {
     "msr_cg_noalloc": (_I, [_P, _P, _P, _I, _D, _D, _I, _I, _P, _P, _P]),
     "msr_normalop_apply_f64": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),
     "msr_normalop_apply_f64c": (_I, [_P, _P, _P, _D, _D, _D, _D, _P]),
+    "msr_normalop_tune": (_I, [_P, _I, _I, _P, _P, _P]),
+    "msr_tune_save": (_I, [_P, ctypes.c_char_p]),
+    "msr_tune_load": (_I, [_P, ctypes.c_char_p, _P]),
     "msr_normalop_create_shared": (_I, [_P, _P, _I, _P, _P, _P]),
     "msr_normalop_footprint": (_I, [_P, _I, _P, _P]),
     "msr_kernel_key": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, _P]),

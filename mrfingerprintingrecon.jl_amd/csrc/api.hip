@@ -344,6 +344,34 @@ int msr_normalop_apply_f64c(msr_normalop* op, void* x, const void* b, double alp
     });
 }
 
+int msr_normalop_tune(msr_normalop* op, int precision, int nrep, double* times_ms, int* chosen,
+                      void* stream) {
+    return guarded([&] {
+        MSR_NOTNULL(op);
+        MSR_REQUIRE(prec_valid(precision), MSR_ERR_ARGUMENT, "precision = %d", precision);
+        static_assert(TUNE_CONFIGS == MSR_TUNE_CONFIGS, "tuner candidate count");
+        ensure_device(op->ctx);
+        op_tune(op, apply_mode(precision), nrep, times_ms, chosen, (hipStream_t)stream);
+    });
+}
+
+int msr_tune_save(msr_ctx* ctx, const char* path) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(path);
+        tune_save(ctx, path);
+    });
+}
+
+int msr_tune_load(msr_ctx* ctx, const char* path, int* nloaded) {
+    return guarded([&] {
+        MSR_NOTNULL(ctx);
+        MSR_NOTNULL(path);
+        const int n = tune_load(ctx, path);
+        if (nloaded) *nloaded = n;
+    });
+}
+
 int msr_cg(msr_normalop* op, void* x, const void* b, int maxiter, double reltol, double abstol,
            int initially_zero, int mode, int precision, int* iters_out, double* residual_out,
            void* stream) {

@@ -16,25 +16,27 @@ constexpr int rowpass_block() {
 
 // Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
 // each XCD with its own L2.  Blocks b and b + 8 get consecutive logical
-// indices (when the block count divides by 8), so blocks that share inputs
-// run on one XCD close in time: pass A orders (coil, coefficient, tile) with
-// the coil fastest (the coefficient's row tile of b is read once from HBM for
-// all coils, a coil map tile once for all coefficients), pass C orders
-// (coefficient, coil group, tile) with the coefficient fastest (a group's coil
-// map tiles read once for all coefficients).  Pass A with 16-byte storage of
-// the intermediate keeps the tile fastest instead: its writes (full 128-byte
-// lines, 200 MiB per apply at 256^2 / 20 coils / K = 5) dominate its reads,
-// and neighbouring tiles of one plane written together took 74 us against
-// 93 us for the coil-fastest order on MI355X; with 8-byte storage the
-// coil-fastest order is the faster one (66 against 70 us).
+// indices (when the block count divides by 8), so that consecutive logical
+// blocks run on one XCD close in time.  Two logical orders per pass, the same
+// blocks and arithmetic in either (the result is bit-identical): pass A
+// (tile, k, c) with the tile fastest (neighbouring tiles of one plane written
+// together) or (c, k, tile) with the coil fastest (a coefficient's row tile of
+// b read once from HBM for all coils, a coil map tile once for all
+// coefficients); pass C (tile, k, group) with the tile fastest or (k, group,
+// tile) with the coefficient fastest (a group's coil map tiles read once for
+// all coefficients).  Which is faster depends on the storage type and the
+// shape (at 256^2 / 20 coils / K = 5 on MI355X: pass A 74 against 93 us
+// tile-fastest with 16-byte storage, 66 against 70 us coil-fastest with 8-byte
+// storage; pass C 45 against 64 us coefficient-fastest in ComplexF32); the
+// operator's knobs carry the choice, measured by the tuner.
 __device__ __forceinline__ int64_t xcd_order(int64_t lb, int64_t nblk) {
     if (nblk % 8 != 0) return lb;
     return (lb % 8) * (nblk / 8) + lb / 8;
 }
 
 // ----------------------------------------------------------------- pass A
-// block lb = (c, k, tile), c fastest (tile fastest for 16-byte ST); CT
-// arithmetic, ST storage of T
+// block lb = (tile, k, c) or (c, k, tile) (a.order_a); CT arithmetic, ST
+// storage of T
 template <int NX2, class CT, class ST>
 __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     if (a.skip && *a.skip) return;
@@ -49,7 +51,7 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     const int tiles = tiles2d(ny, RT);
     lb = xcd_order(lb, (int64_t)tiles * K * a.C);
     int c, k, tile;
-    if constexpr (sizeof(ST) >= 16) {
+    if (a.order_a == 0) {
         tile = (int)(lb % tiles);
         k = (int)((lb / tiles) % K);
         c = (int)(lb / ((int64_t)tiles * K));
@@ -100,7 +102,7 @@ pass_a_2d_batch(const Apply2DArgs* __restrict__ jobs, const int* __restrict__ pr
 }
 
 // ----------------------------------------------------------------- pass C
-// block lb = (k, coil group, tile), k fastest
+// block lb = (tile, k, coil group) or (k, coil group, tile) (a.order_c)
 template <int NX2, class CT, class ST>
 __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     if (a.skip && *a.skip) return;
@@ -116,9 +118,16 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     const int nx = a.nx, ny = a.ny, K = a.K, C = a.C, G = a.G;
     const int tiles = tiles2d(ny, RT);
     lb = xcd_order(lb, (int64_t)tiles * K * G);
-    const int k = (int)(lb % K);
-    const int grp = (int)((lb / K) % G);
-    const int tile = (int)(lb / ((int64_t)K * G));
+    int k, grp, tile;
+    if (a.order_c == 0) {
+        tile = (int)(lb % tiles);
+        k = (int)((lb / tiles) % K);
+        grp = (int)(lb / ((int64_t)tiles * K));
+    } else {
+        k = (int)(lb % K);
+        grp = (int)((lb / K) % G);
+        tile = (int)(lb / ((int64_t)K * G));
+    }
     const int tid = threadIdx.x;
     const int y0 = tile * RT;
     const int c0 = (int)((int64_t)C * grp / G), c1 = (int)((int64_t)C * (grp + 1) / G);

@@ -52,6 +52,8 @@ struct Apply2DArgs {
     void* partial;        // scratch (G, K, ny, nx)
     int G;                // coil groups of pass C (>= 1)
     int Gb;               // coil groups of pass B (>= 1)
+    int order_a;          // pass A block order: 0 tile fastest, 1 coil fastest
+    int order_c;          // pass C block order: 0 tile fastest, 1 coefficient fastest
     const void* tw_x;     // twiddles for 2nx
     const void* tw_y;     // twiddles for 2ny
     const int* skip;      // when non-null and *skip != 0 the passes do nothing

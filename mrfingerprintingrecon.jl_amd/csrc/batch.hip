@@ -40,6 +40,7 @@ msr_batch* batch_create(msr_ctx* ctx, int nops, msr_normalop* const* ops) {
         MSR_REQUIRE(seen.insert(ops[j]).second, MSR_ERR_ARGUMENT,
                     "ops[%d] appears twice (each operator owns one scratch and CG state)", j);
         bt->ops.push_back(ops[j]);
+        bt->knobs.push_back({ops[j]->knobs[0], ops[j]->knobs[1], ops[j]->knobs[2]});
         // fused 2D operators share grouped launches; the others (3D, Cartesian,
         // line-kernel apply) run their own passes in the batch's sequence
         (ops[j]->d == 2 && !ops[j]->cart && ops[j]->fused ? bt->j2d : bt->j3d).push_back(j);
@@ -81,7 +82,7 @@ msr_batch* batch_create(msr_ctx* ctx, int nops, msr_normalop* const* ops) {
                 a.K = op->K;
                 a.C = op->C;
                 a.G = op->G;
-                a.Gb = op->Gb;
+                a.Gb = bt->knobs[j][f].Gb;
                 a.f64 = f != AM_F32;
                 a.compact = f == AM_F64C;
                 tot += count(a);
@@ -139,6 +140,10 @@ static void fill_tables(msr_batch* bt, BatchTables& T, const TableUse& use,
         for (size_t i = 0; i < g.jobs.size(); ++i) {
             const int j = g.jobs[i];
             Apply2DArgs a = args2d(bt->ops[j], use.am);
+            const ApplyKnobs& kn = bt->knobs[j][use.am];
+            a.Gb = kn.Gb;
+            a.order_a = kn.order_a;
+            a.order_c = kn.order_c;
             a.b = use.b[j];
             a.skip = use.skip_done ? &bt->ops[j]->cg_scal.as<CgScalars>()->done : nullptr;
             tab[i] = a;

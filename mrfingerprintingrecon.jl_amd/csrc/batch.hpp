@@ -47,6 +47,10 @@ struct msr_batch {
     msr_ctx* ctx = nullptr;
     std::vector<msr_normalop*> ops;
     std::vector<int> j2d, j3d;
+    // the fused 2D operators' knobs per apply mode when the batch was made:
+    // the block prefixes count blocks with them, so later tuning of an
+    // operator does not reach this batch
+    std::vector<std::array<msr::ApplyKnobs, 3>> knobs;
     std::vector<msr::BatchGroup> ga, gb, gc;
     std::vector<int> vpre;       // vector-launch block prefix over all operators
     int vblocks = 0;

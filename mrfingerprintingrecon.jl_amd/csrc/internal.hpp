@@ -1,6 +1,7 @@
 // Internal object definitions behind the opaque C handles.
 #pragma once
 
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -8,11 +9,22 @@
 
 #include "common.hpp"
 
-#include <array>
 #include <tuple>
+
+namespace msr {
+// Launch choices of the fused 2D passes that leave the result unchanged bit
+// for bit: pass B coil groups and the pass A / pass C block orders (apply2d.hip).
+struct ApplyKnobs {
+    int Gb = 1;
+    int order_a = 1;
+    int order_c = 1;
+};
+}  // namespace msr
 
 struct msr_ctx {
     int device = 0;
+    // tuned knobs per (nx, ny, K, C, kernel real, identity maps, apply mode)
+    std::map<std::array<int64_t, 7>, msr::ApplyKnobs> tuned;
     std::map<int, float2*> twiddles;     // N -> device table exp(-2 pi i m/N), ComplexF32
     std::map<int, double2*> twiddles64;  // the same in ComplexF64
     std::map<std::tuple<int, int, int>, float*> deapod;  // (n_os, n_out, sel) -> 1/phihat
@@ -139,7 +151,8 @@ struct msr_normalop {
     msr::DevBuf T2;                   // second intermediate (3D)
     msr::DevBuf partial;              // (G, K, V)
     size_t scratch_es = 0;            // element size T / T2 / partial are allocated for
-    int G = 1, Gb = 1;
+    int G = 1, Gb = 1;                // coil groups of pass C / of pass B (3D)
+    msr::ApplyKnobs knobs[3];         // fused 2D launch choices per apply mode
     bool fused = true;                // fused Toeplitz passes (else the line-kernel apply)
     int Cb = 1;                       // coils per chunk of the line-kernel apply
     // CG state (allocated on first use)

@@ -89,6 +89,23 @@ static int env_int(const char* name, int dflt) {
     return s ? atoi(s) : dflt;
 }
 
+static std::array<int64_t, 7> tune_key(const msr_normalop* op, int am) {
+    return {op->N[0], op->N[1], op->K, op->C, op->lam_real, op->identity_cmaps ? 1 : 0, am};
+}
+
+bool tuned_knobs(const msr_normalop* op, int am, ApplyKnobs* out) {
+    std::lock_guard<std::mutex> guard(op->ctx->lock);
+    auto it = op->ctx->tuned.find(tune_key(op, am));
+    if (it == op->ctx->tuned.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+void remember_knobs(const msr_normalop* op, int am, const ApplyKnobs& k) {
+    std::lock_guard<std::mutex> guard(op->ctx->lock);
+    op->ctx->tuned[tune_key(op, am)] = k;
+}
+
 void op_alloc_scratch(msr_normalop* op) {
     const int64_t ny = op->N[1];
     op->Cb = 1;
@@ -105,6 +122,16 @@ void op_alloc_scratch(msr_normalop* op) {
         const int64_t colblocks = 2 * op->N[0];
         int Gb = colblocks >= 1024 ? 1 : 2;
         op->Gb = std::max(1, std::min(env_int("MSR_GB", Gb), op->C));
+        // untuned knobs: the orders measured fastest at 256^2 / 20 coils / K = 5
+        // (pass A tile-fastest only with 16-byte intermediates); a tuned choice
+        // for the same shape on this context replaces them
+        for (int am = 0; am < 3; ++am) {
+            op->knobs[am].Gb = op->Gb;
+            op->knobs[am].order_a = am == AM_F64 ? 0 : 1;
+            op->knobs[am].order_c = 1;
+            ApplyKnobs t;
+            if (tuned_knobs(op, am, &t)) op->knobs[am] = t;
+        }
     } else {
         op->G = 1;
         op->Gb = 1;
@@ -278,7 +305,9 @@ Apply2DArgs args2d(msr_normalop* op, int am) {
     a.T = op->T.p;
     a.partial = op->partial.p;
     a.G = op->G;
-    a.Gb = op->Gb;
+    a.Gb = op->knobs[am].Gb;
+    a.order_a = op->knobs[am].order_a;
+    a.order_c = op->knobs[am].order_c;
     if (f64) {
         a.tw_x = twiddle_table64(op->ctx, 2 * a.nx);
         a.tw_y = twiddle_table64(op->ctx, 2 * a.ny);

@@ -27,6 +27,22 @@ void op_set_cmaps(msr_normalop* op, int ncoils, const void* const* cmaps, hipStr
 void op_load_packed(msr_normalop* op, const void* lambda, const int64_t* kmask, int64_t nkm,
                     hipStream_t st);
 void op_alloc_scratch(msr_normalop* op);
+// tuned fused-2D knobs of the operator's shape on its context (tune.hip)
+bool tuned_knobs(const msr_normalop* op, int am, ApplyKnobs* out);
+void remember_knobs(const msr_normalop* op, int am, const ApplyKnobs& k);
+// Measures the fused 2D apply of mode am under every knob set (Gb in 1, 2, 4,
+// 8 up to C; both block orders of passes A and C), nrep applies each, checks
+// that every set gives the first one's partial sums bit for bit, and keeps
+// the fastest (also for later operators of the same shape on the context).
+// times[TUNE_CONFIGS] (ms per apply, NaN where Gb > C; may be null), *chosen
+// = 4 g + 2 order_a + order_c with Gb = 1 << g, or -1 for an operator without
+// fused 2D passes.  nrep = 0 only reports the current knobs.
+constexpr int TUNE_CONFIGS = 16;
+void op_tune(msr_normalop* op, int am, int nrep, double* times, int* chosen, hipStream_t st);
+// the context's tuned knobs to / from a text file (tune.hip); load returns
+// the number of entries and refuses a file measured on another device type
+void tune_save(msr_ctx* ctx, const char* path);
+int tune_load(msr_ctx* ctx, const char* path);
 void op_apply(msr_normalop* op, float2* x, const float2* b, float2 alpha, float2 beta,
               bool beta_zero, hipStream_t st);
 // ComplexF64 apply; am = AM_F64 or AM_F64C (compact intermediates, 2D only)
