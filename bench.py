@@ -582,22 +582,32 @@ def batched_leg(msr, njobs=64, iters=CG_ITERS, seed=1004):
     return res
 
 
-def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, niter_step=5,
-               max_active=64, policy="fifo"):
-    """Served stream: Poisson arrivals (rate jobs/s) of 2D jobs drawn as in the
-    batched leg over a set of protocols whose kernels stay resident; each job
-    brings its own coil maps (an operator sharing its protocol's kernel) and
-    data.  On arrival a job's backprojection runs and its CG is submitted to
-    the server (continuous batching, admission by device memory); latency is
-    arrival to solution.  The same arrivals served one at a time (FIFO, each
-    job's backprojection and single-operator graph CG) are the baseline; the
-    server also runs them under shortest-work-first admission with one job in
-    flight (lower median latency, longer tail)."""
+SERVER_CONFIGS = (  # (policy, max jobs in flight, CG iterations per server step)
+    ("fifo", 64, 5), ("shortest_first", 1, CG_ITERS), ("shortest_first", 2, 5),
+    ("shortest_first", 4, 5), ("shortest_first", 8, 5))
+
+
+def served_leg(msr, njobs=80, load=1.25, iters=CG_ITERS, iters3d=15, seed=1005, nproto=5,
+               configs=SERVER_CONFIGS, frac3d=0.15, n3d=96):
+    """Served stream at saturating load: Poisson arrivals of jobs drawn over
+    resident protocols whose kernels stay on the device: 2D protocols as in
+    the batched leg (N in {128,160,192,256}, K 3-6) and a 3D koosh-ball
+    protocol at 96^3 (K 4, 4 spokes x 500 frames) that draws frac3d of the
+    jobs.  Each job brings its own coil maps (8-32 coils in 2D, 12-16 in 3D;
+    an operator sharing its protocol's kernel) and data.  The arrival rate is
+    `load` times the one-at-a-time capacity measured on the same jobs (the
+    mean of their backprojection + graph CG service times), so the queue
+    grows under the one-at-a-time baseline.  On arrival a job's
+    backprojection runs and its CG (25 iterations in 2D, 15 in 3D) is
+    submitted to the server (continuous batching, admission by device
+    memory); latency is arrival to solution.  The same arrivals served one at
+    a time (FIFO) are the baseline; the server also runs them under
+    shortest-work-first admission with one job in flight."""
     import numpy as np
     import torch
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(seed)
-    log(f"served: {nproto} protocols, {njobs} jobs at {rate}/s")
+    log(f"served: {nproto} 2D protocols + one 3D protocol at {n3d}^3, {njobs} jobs")
     protos = []
     for p in range(nproto):
         N = int(rng.choice([128, 160, 192, 256]))
@@ -607,35 +617,57 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
         trj, U, cml, _ = _job_inputs(msr, torch, dev, rng, N, 8, K, spokes, nt, seed * 100 + p)
         traj = msr.plan_trajectory(trj)
         A = msr.NFFTNormalOp((N, N), traj, U, cmaps=cml)
-        protos.append(dict(N=N, K=K, spokes=spokes, nt=nt, traj=traj, U=U, A=A))
+        protos.append(dict(shape=(N, N), K=K, nsamp=spokes * 2 * N, nt=nt, traj=traj, U=U, A=A,
+                           iters=iters, coils=(8, 33)))
+    nt3, sp3, K3 = 500, 4, 4
+    trj3 = msr.synth.traj_kooshball_goldenratio(2 * n3d, sp3, nt3).astype("float32")
+    U3 = torch.from_numpy(msr.synth.t2_basis(nt3, K3, seed=seed)).to(dev)
+    traj3 = msr.plan_trajectory(torch.from_numpy(trj3).to(dev))
+    cm3 = [torch.from_numpy(c).to(dev) for c in msr.synth.gaussian_coil_maps((n3d,) * 3, 2, seed=seed)]
+    A3 = msr.NFFTNormalOp((n3d,) * 3, traj3, U3, cmaps=cm3)
+    p3 = dict(shape=(n3d,) * 3, K=K3, nsamp=trj3.shape[1], nt=nt3, traj=traj3, U=U3, A=A3,
+              iters=iters3d, coils=(12, 17))
     jobs = []
-    t = 0.0
     for i in range(njobs):
-        t += float(rng.exponential(1.0 / rate))
-        pr = protos[int(rng.integers(0, nproto))]
-        C = int(rng.integers(8, 33))
-        N, K = pr["N"], pr["K"]
-        cml = [torch.from_numpy(c).to(dev) for c in msr.synth.gaussian_coil_maps((N, N), C, seed=i)]
+        pr = p3 if rng.uniform() < frac3d else protos[int(rng.integers(0, nproto))]
+        C = int(rng.integers(*pr["coils"]))
+        shape, K = pr["shape"], pr["K"]
+        cml = [torch.from_numpy(c).to(dev) for c in msr.synth.gaussian_coil_maps(shape, C, seed=i)]
         g = torch.Generator(device=dev)
         g.manual_seed(seed * 1000 + i)
-        data = torch.randn((C, pr["nt"], pr["spokes"] * 2 * N), dtype=torch.complex64, device=dev,
+        data = torch.randn((C, pr["nt"], pr["nsamp"]), dtype=torch.complex64, device=dev,
                            generator=g)
         A = msr.NFFTNormalOp_shared(pr["A"], cmaps=cml)
-        jobs.append(dict(t=t, pr=pr, cml=cml, data=data, A=A,
-                         b=torch.empty((K, N, N), dtype=torch.complex64, device=dev),
-                         x=torch.empty(K * N * N, dtype=torch.complex64, device=dev),
-                         x1=torch.empty(K * N * N, dtype=torch.complex64, device=dev)))
-    srv = msr.NFFTServer(max_active=max_active, precision=PRECISION, policy=policy)
-
+        n = K * int(np.prod(shape))
+        jobs.append(dict(pr=pr, cml=cml, data=data, A=A, three_d=len(shape) == 3,
+                         b=torch.empty((K,) + shape[::-1], dtype=torch.complex64, device=dev),
+                         x=torch.empty(n, dtype=torch.complex64, device=dev),
+                         x1=torch.empty(n, dtype=torch.complex64, device=dev)))
     def bp(j):
         msr.calculate_backprojection(j["data"], j["pr"]["traj"], j["cml"], U=j["pr"]["U"], out=j["b"])
 
-    # warm-up (graph captures of the single-operator baseline, first launches)
-    for j in jobs:
+    def solve_alone(j):
         bp(j)
-        msr.cg_(j["x1"], j["A"], j["b"].reshape(-1), maxiter=iters, reltol=0.0,
+        msr.cg_(j["x1"], j["A"], j["b"].reshape(-1), maxiter=j["pr"]["iters"], reltol=0.0,
                 initially_zero=True, mode="graph", precision=PRECISION)
+
+    # warm-up (graph captures of the single-operator baseline, first launches),
+    # then each job's one-at-a-time service time
+    for j in jobs:
+        solve_alone(j)
     torch.cuda.synchronize()
+    service = []
+    for j in jobs:
+        t0 = time.perf_counter()
+        solve_alone(j)
+        torch.cuda.synchronize()
+        service.append(time.perf_counter() - t0)
+    capacity = len(jobs) / sum(service)
+    rate = load * capacity
+    t = 0.0
+    for j in jobs:
+        t += float(rng.exponential(1.0 / rate))
+        j["t"] = t
 
     def run_fifo():
         lat = []
@@ -643,16 +675,13 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
         for j in jobs:
             while time.perf_counter() - t0 < j["t"]:
                 pass
-            bp(j)
-            msr.cg_(j["x1"], j["A"], j["b"].reshape(-1), maxiter=iters, reltol=0.0,
-                    initially_zero=True, mode="graph", precision=PRECISION)
+            solve_alone(j)
             torch.cuda.synchronize()
             lat.append(time.perf_counter() - t0 - j["t"])
         return time.perf_counter() - t0, lat
 
-    step_iters = [niter_step]
-
-    def run_server():
+    def run_server(policy, max_active, niter):
+        srv = msr.NFFTServer(max_active=max_active, precision=PRECISION, policy=policy)
         lat = [None] * len(jobs)
         ids = {}
         nxt = 0
@@ -662,10 +691,10 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
             while nxt < len(jobs) and jobs[nxt]["t"] <= now:
                 j = jobs[nxt]
                 bp(j)
-                ids[srv.submit(j["A"], j["x"], j["b"].reshape(-1), maxiter=iters)] = nxt
+                ids[srv.submit(j["A"], j["x"], j["b"].reshape(-1), maxiter=j["pr"]["iters"])] = nxt
                 nxt += 1
             if ids:
-                srv.step(step_iters[0])
+                srv.step(niter)
                 tdone = time.perf_counter() - t0
                 for jid in [k for k in ids if srv.poll(k)["state"] == srv.DONE]:
                     i = ids.pop(jid)
@@ -674,30 +703,36 @@ def served_leg(msr, njobs=80, rate=40.0, iters=CG_ITERS, seed=1005, nproto=6, ni
         return time.perf_counter() - t0, lat
 
     t_fifo, lat_fifo = run_fifo()
-    t_srv, lat_srv = run_server()
-    # the same arrivals under shortest-work-first admission, one job in flight
-    srv = msr.NFFTServer(max_active=1, precision=PRECISION, policy="shortest_first")
-    step_iters[0] = iters
-    t_sjf, lat_sjf = run_server()
-    err = max(float(torch.linalg.norm(j["x"] - j["x1"]) / torch.linalg.norm(j["x1"])) for j in jobs)
     ms = lambda v: round(1e3 * v, 3)
-    res = {"workload": f"{njobs} 2D recons, Poisson arrivals at {rate}/s over {nproto} resident "
-                       f"protocols (N in {{128,160,192,256}}, K 3-6, 1-4 spokes x 500-1000 frames), "
-                       f"per-job coil maps (8-32 coils) and data; BP + {iters}-iteration CG each",
-           "throughput_solves_per_s": round(njobs / t_srv, 2),
-           "latency_ms": {"p50": ms(percentile(lat_srv, 50)), "p99": ms(percentile(lat_srv, 99))},
-           "one_at_a_time": {"throughput_solves_per_s": round(njobs / t_fifo, 2),
-                             "latency_ms": {"p50": ms(percentile(lat_fifo, 50)),
-                                            "p99": ms(percentile(lat_fifo, 99))}},
-           "shortest_first": {"max_active": 1, "iterations_per_step": iters,
-                              "throughput_solves_per_s": round(njobs / t_sjf, 2),
-                              "latency_ms": {"p50": ms(percentile(lat_sjf, 50)),
-                                             "p99": ms(percentile(lat_sjf, 99))}},
+    n3 = sum(j["three_d"] for j in jobs)
+    lat2 = lambda lat: [v for v, j in zip(lat, jobs) if not j["three_d"]]
+    summary = lambda t, lat: {"throughput_solves_per_s": round(njobs / t, 2),
+                              "latency_ms": {"p50": ms(percentile(lat, 50)),
+                                             "p99": ms(percentile(lat, 99)),
+                                             "p50_2d": ms(percentile(lat2(lat), 50))}}
+    runs = []
+    err = 0.0
+    for policy, max_active, niter in configs:
+        t_srv, lat_srv = run_server(policy, max_active, niter)
+        err = max(err, max(float(torch.linalg.norm(j["x"] - j["x1"]) / torch.linalg.norm(j["x1"]))
+                           for j in jobs))
+        runs.append({"policy": policy, "max_active": max_active, "iterations_per_step": niter,
+                     **summary(t_srv, lat_srv)})
+        log(f"served {policy}/{max_active}/{niter}: {runs[-1]}")
+    res = {"workload": f"{njobs} recons ({njobs - n3} 2D, {n3} 3D at {n3d}^3), Poisson arrivals "
+                       f"at {load} x the one-at-a-time capacity over {nproto} resident 2D "
+                       f"protocols (N in {{128,160,192,256}}, K 3-6, 1-4 spokes x 500-1000 "
+                       f"frames) and one 3D koosh-ball protocol (K 4, {sp3} spokes x {nt3} "
+                       f"frames); per-job coil maps (2D 8-32, 3D 12-16 coils) and data; BP + "
+                       f"CG ({iters} iterations 2D, {iters3d} 3D) each",
+           "arrival_rate_per_s": round(rate, 2),
+           "one_at_a_time_capacity_per_s": round(capacity, 2),
            "arrival_span_s": round(jobs[-1]["t"], 3),
-           "max_active": max_active, "iterations_per_step": niter_step, "policy": policy,
+           "one_at_a_time": summary(t_fifo, lat_fifo),
+           "server": runs,
            "max_rel_diff_served_vs_single": err}
     log(f"served: {res}")
-    del srv, jobs, protos
+    del jobs, protos
     torch.cuda.synchronize()
     return res
 
