@@ -83,6 +83,8 @@ def julia_ccalls():
 def c_to_julia_ok(ctype, jtype):
     c = ctype.replace("const", "").strip()
     if "*" in c:
+        if c.replace(" ", "") == "char*":  # a NUL-terminated string
+            return jtype in ("Cstring", "Ptr{UInt8}", "Ptr{Cchar}")
         return jtype.startswith("Ptr") or jtype.startswith("Ref")
     return {"int": "Cint", "int64_t": "Int64", "double": "Cdouble", "size_t": "Csize_t",
             "uint64_t": "UInt64"}.get(c) == jtype
