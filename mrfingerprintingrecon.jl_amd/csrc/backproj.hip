@@ -99,7 +99,9 @@ void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
     // stream-ordered; a handle is not re-entrant)
     DevBuf& work = traj->bp_work;
     const size_t gbytes = sizeof(float2) * (size_t)nch * Gos;
-    const size_t sbytes = d == 2 ? spread_bp_workspace_bytes(traj, plan, nch, ncoils, st) : 0;
+    const bool gather3 = d == 3 && spread_bp3_supported(traj);
+    const size_t sbytes = d == 2 ? spread_bp_workspace_bytes(traj, plan, nch, ncoils, st)
+                          : gather3 ? spread_bp3_workspace_bytes(traj, plan, nch, st) : 0;
     const size_t hbytes = std::max(sizeof(float2) * (size_t)nch * (Gos / 2), sbytes);
     if (work.bytes < gbytes + hbytes) {
         MSR_HIP(hipStreamSynchronize(st));  // earlier calls may still use the old buffer
@@ -108,6 +110,7 @@ void backprojection(msr_ctx* ctx, const msr_traj* traj, int d, const int64_t* N,
     float2* G = work.as<float2>();
     float2* H = reinterpret_cast<float2*>(reinterpret_cast<char*>(work.p) + gbytes);
     if (d == 2) spread_bp(traj, plan, src, nch, G, H, st);
+    else if (gather3) spread_bp3(traj, plan, src, nch, G, H, st);
     else spread(traj, plan, src, nch, G, H, false, st);
 
     const bool combine = cmaps != nullptr;

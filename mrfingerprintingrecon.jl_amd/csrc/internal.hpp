@@ -72,18 +72,21 @@ struct SpreadPlan {
     int64_t nbins = 0;
     DevBuf perm;                // (M) int32: points sorted by tile
     DevBuf offs;                // (nbins + 1) int32
-    // 2D output-tile streams of the ComplexF32 spreading (built on first
-    // use): per sorted point its cell, frame and data index; per tile the
-    // sorted points whose support reaches it (pair list) with their kernel
-    // weights over the tile's columns and rows, cut into bounded work items
+    // output-block streams of the ComplexF32 backprojection spreading (built
+    // on first use; 8 x 8 blocks in 2D, 4 x 4 x 4 in 3D): per sorted point its
+    // cell, frame and data index; per block the sorted points whose support
+    // reaches it (pair list) with their kernel weights over the block's
+    // columns, rows (and planes), cut into bounded work items
     bool streams_ready = false;
     int64_t npairs = 0, nitems = 0, nsplit = 0, nslots = 0;
-    DevBuf pw;                  // (npairs, 32) float: weights over the tile's 16 columns, 16 rows
-    DevBuf cell;                // (M) int2: floor cell
+    DevBuf pw;                  // (npairs, 16) float: weights over the block's 8 columns and 8 rows
+                                // (2D); (npairs, 12): 4 columns, rows and planes (3D)
+    DevBuf cell;                // (M) int2 (2D) or int4 (3D): floor cell
     DevBuf frame_s;             // (M) int32
     DevBuf jsel_s;              // (M) int64
-    DevBuf kc_s;                // (M) float2
-    DevBuf pairs;               // (npairs) int32: sorted point index | wave mask << 28, grouped by tile
+    DevBuf kc_s;                // (M) float2 (2D)
+    DevBuf r2_s;                // (M) float: |k|^2 of the sample (3D)
+    DevBuf pairs;               // (npairs) int32: sorted point index, grouped by output block
     DevBuf items;               // (nitems) int4 {tile, q0, q1, slab slot or -1}
     DevBuf split_tiles;         // (nsplit) int2 {tile, first item}; items of a split tile are consecutive
     DevBuf split_nitems;        // (nsplit) int32

@@ -58,6 +58,14 @@ size_t spread_bp_workspace_bytes(const msr_traj* traj, const SpreadPlan& plan, i
                                  hipStream_t st);
 void spread_bp(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src, int nch,
                float2* grid, float2* work, hipStream_t st);
+// The 3D form on 4 x 4 x 4 output blocks (spread_bp3.hip); grid (nch, n2,
+// n1, n0) is overwritten.  Supported up to 2^31 / 64 samples (int pair
+// offsets); larger trajectories take the tile spreading.
+bool spread_bp3_supported(const msr_traj* traj);
+size_t spread_bp3_workspace_bytes(const msr_traj* traj, const SpreadPlan& plan, int nch,
+                                  hipStream_t st);
+void spread_bp3(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src, int nch,
+                float2* grid, float2* work, hipStream_t st);
 
 // Device table 1/phihat(r) for the output frequencies of a type-1 NUFFT of
 // size n_out (sel: 1 centred r = j - n_out/2, 2 FFT order) on an n_os grid.
