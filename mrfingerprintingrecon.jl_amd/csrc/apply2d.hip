@@ -44,7 +44,7 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rowpass_block<NX2, CT, ST>();
     constexpr int F = BLK / TP;              // rows transformed concurrently
-    constexpr int LD = lds_line<NX2>() + 1;  // LDS row pitch (odd: transposed access)
+    constexpr int LD = lds_line<NX2, CT>() + 1;  // LDS row pitch (odd: transposed access)
     __shared__ CT s_tw[NX2];
     __shared__ CT s_rows[RT * LD];
     const int nx = a.nx, ny = a.ny, K = a.K;
@@ -74,7 +74,7 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
             v = bk[(size_t)y * nx + xx];
             if (cm) v = cmul(cto<CT>(cm[(size_t)y * nx + xx]), v);
         }
-        s_rows[yl * LD + lds_pad(xx)] = v;
+        s_rows[yl * LD + lds_pad<CT>(xx)] = v;
     }
     __syncthreads();
     const int g = tid / TP, t = tid - g * TP;
@@ -85,7 +85,7 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     for (int e = tid; e < RT * NX2; e += BLK) {
         const int yl = e % RT, qx = e / RT;
         const int y = y0 + yl;
-        if (y < ny) st_store<CT, ST>(Tk + (size_t)qx * ny + y, s_rows[yl * LD + lds_pad(qx)]);
+        if (y < ny) st_store<CT, ST>(Tk + (size_t)qx * ny + y, s_rows[yl * LD + lds_pad<CT>(qx)]);
     }
 }
 
@@ -110,7 +110,7 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rowpass_block<NX2, CT, ST>();
     constexpr int F = BLK / TP;
-    constexpr int LD = lds_line<NX2>() + 1;
+    constexpr int LD = lds_line<NX2, CT>() + 1;
     constexpr int NXC = NX2 / 2;                 // = nx
     constexpr int EA = (RT * NXC + BLK - 1) / BLK;
     __shared__ CT s_tw[NX2];
@@ -156,7 +156,7 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
 #pragma unroll
         for (int r = 0; r < NLE; ++r) {
             const int e = tid + r * BLK;
-            if (e < RT * NX2) s_rows[(e % RT) * LD + lds_pad(e / RT)] = st_conv<CT, ST>(pre[r]);
+            if (e < RT * NX2) s_rows[(e % RT) * LD + lds_pad<CT>(e / RT)] = st_conv<CT, ST>(pre[r]);
         }
         __syncthreads();
         if (c + 1 < c1) prefetch(c + 1);  // in flight during this coil's transforms
@@ -170,7 +170,7 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
                 const int yl = e / NXC, xx = e - yl * NXC;
                 const int y = y0 + yl;
                 if (y < ny) {
-                    CT v = s_rows[yl * LD + lds_pad(xx)];
+                    CT v = s_rows[yl * LD + lds_pad<CT>(xx)];
                     if (cm) cfmac(acc[i], cto<CT>(cm[(size_t)y * nx + xx]), v);
                     else acc[i] = cadd(acc[i], v);
                 }

@@ -81,7 +81,7 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     using RT = real_t<CT>;
     constexpr int TP = S::TP, WC = S::WC, BLK = S::BLK, NV = S::NV, EV = S::EV, P = S::P;
     constexpr int FT = S::FT;
-    constexpr int LL = lds_line<NY2>();
+    constexpr int LL = lds_line<NY2, CT>();
     __shared__ CT s_tw[NY2];
     __shared__ CT s_buf[WC * K * LL];
     const LT* lam = reinterpret_cast<const LT*>(a.lam);
@@ -163,7 +163,7 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
             const int v = tid + e * BLK;
             if (v < NV) {
                 const int wc = v / NY2, qy = v - wc * NY2;
-                CT* col = s_buf + (size_t)wc * K * LL + lds_pad(qy);
+                CT* col = s_buf + (size_t)wc * K * LL + lds_pad<CT>(qy);
                 CT in[K], acc[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -205,7 +205,7 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
 #pragma unroll
                 for (int m = 0; m < NR / 2; ++m) {
                     const int i = t + m * TP;
-                    if (col_ok) st_store<CT, ST>(dst + i, line[lds_pad(i)]);
+                    if (col_ok) st_store<CT, ST>(dst + i, line[lds_pad<CT>(i)]);
                 }
             }
         }

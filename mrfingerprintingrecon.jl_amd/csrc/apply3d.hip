@@ -26,7 +26,7 @@ pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, CT* __rest
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rows3_block<NX2, CT>();
     constexpr int F = BLK / TP;
-    constexpr int LD = lds_line<NX2>() + 1;
+    constexpr int LD = lds_line<NX2, CT>() + 1;
     __shared__ CT s_tw[NX2];
     __shared__ CT s[F * LD];
     load_twiddles<NX2>(s_tw, tw);
@@ -43,7 +43,7 @@ pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, CT* __rest
             v = bk[row * nx + xx];
             if (cm) v = cmul(cto<CT>(cm[row * nx + xx]), v);
         }
-        s[l * LD + lds_pad(xx)] = v;
+        s[l * LD + lds_pad<CT>(xx)] = v;
     }
     __syncthreads();
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
@@ -52,7 +52,7 @@ pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, CT* __rest
     for (int e = threadIdx.x; e < F * NX2; e += BLK) {
         const int l = e / NX2, q = e - l * NX2;
         const int64_t row = r0 + l;
-        if (row < nrows) out[row * NX2 + q] = s[l * LD + lds_pad(q)];
+        if (row < nrows) out[row * NX2 + q] = s[l * LD + lds_pad<CT>(q)];
     }
 }
 
@@ -65,7 +65,7 @@ pass_e_3d(const CT* __restrict__ T1, const float2* __restrict__ cmaps,
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rows3_block<NX2, CT>();
     constexpr int F = BLK / TP;
-    constexpr int LD = lds_line<NX2>() + 1;
+    constexpr int LD = lds_line<NX2, CT>() + 1;
     constexpr int NXC = NX2 / 2;
     constexpr int EA = (F * NXC + BLK - 1) / BLK;
     __shared__ CT s_tw[NX2];
@@ -84,7 +84,7 @@ pass_e_3d(const CT* __restrict__ T1, const float2* __restrict__ cmaps,
         for (int e = threadIdx.x; e < F * NX2; e += BLK) {
             const int l = e / NX2, q = e - l * NX2;
             const int64_t row = r0 + l;
-            s[l * LD + lds_pad(q)] = row < nrows ? in[row * NX2 + q] : cz<CT>();
+            s[l * LD + lds_pad<CT>(q)] = row < nrows ? in[row * NX2 + q] : cz<CT>();
         }
         __syncthreads();
         fft_lds<NX2, +1>(s + g * LD, s_tw, t);
@@ -96,7 +96,7 @@ pass_e_3d(const CT* __restrict__ T1, const float2* __restrict__ cmaps,
                 const int l = e / NXC, xx = e - l * NXC;
                 const int64_t row = r0 + l;
                 if (row < nrows) {
-                    const CT v = s[l * LD + lds_pad(xx)];
+                    const CT v = s[l * LD + lds_pad<CT>(xx)];
                     if (cm) cfmac(acc[i], cto<CT>(cm[row * nx + xx]), v);
                     else acc[i] = cadd(acc[i], v);
                 }

@@ -56,7 +56,7 @@ pass_c_3d(CT* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
     using RT = real_t<CT>;
     constexpr int TP = S::TP, WC = S::WC, BLK = S::BLK, NV = S::NV, EV = S::EV, P = S::P;
     constexpr int FT = S::FT;
-    constexpr int LL = lds_line<NZ2>();
+    constexpr int LL = lds_line<NZ2, CT>();
     __shared__ CT s_tw[NZ2];
     __shared__ CT s_buf[WC * K * LL];
     const LT* lam = reinterpret_cast<const LT*>(lam_v);
@@ -103,7 +103,7 @@ pass_c_3d(CT* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
             CT v = cz<CT>();
             if (z < nz && qx < nx2)
                 v = T2[(((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx];
-            s_buf[(wc * K + k) * LL + lds_pad(z)] = v;
+            s_buf[(wc * K + k) * LL + lds_pad<CT>(z)] = v;
         }
         __syncthreads();
         fft_lds<NZ2, -1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
@@ -112,7 +112,7 @@ pass_c_3d(CT* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
             const int v = tid + e * BLK;
             if (v < NV) {
                 const int wc = v / NZ2, qz = v - wc * NZ2;
-                CT* col = s_buf + (size_t)wc * K * LL + lds_pad(qz);
+                CT* col = s_buf + (size_t)wc * K * LL + lds_pad<CT>(qz);
                 CT in[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) in[k] = col[k * LL];
@@ -143,7 +143,7 @@ pass_c_3d(CT* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
             const int qx = qx0 + wc;
             if (z < nz && qx < nx2)
                 T2[(((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx] =
-                    s_buf[(wc * K + k) * LL + lds_pad(z)];
+                    s_buf[(wc * K + k) * LL + lds_pad<CT>(z)];
         }
         __syncthreads();
     }

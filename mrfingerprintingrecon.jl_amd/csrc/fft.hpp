@@ -338,12 +338,20 @@ template <int DIR> struct Butterfly<5, DIR> {
 };
 
 // ----------------------------------------------------------- LDS layout
-// Element i of a line lives at lds_pad(i): one complex of padding after every
-// 8 elements, so the stride-R stores of the first Stockham stages and the
-// strided reads of the transposing passes spread over the 64 LDS banks.
-__host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 3); }
-template <int N>
-constexpr int lds_line() { return N + N / 8; }
+// Element i of a line lives at lds_pad<CT>(i): one complex of padding after
+// every 8 (ComplexF64) or 32 (ComplexF32) elements, so the stride-R stores of the
+// first Stockham stages and the strided reads of the transposing passes spread
+// over the LDS banks.
+// The padding period depends on the element size, measured on MI355X at
+// 256^2 / 20 coils / K = 5: every 8 elements for 16-byte ComplexF64 (periods
+// of 16 and 32 took the apply from 0.305 to 0.310 / 0.315 ms), every 32 for
+// 8-byte ComplexF32 (0.217 ms with a period of 8, 0.193 with 16, 0.190 with 32).
+template <class CT>
+constexpr int pad_shift() { return sizeof(CT) == 16 ? 3 : 5; }
+template <class CT>
+__host__ __device__ constexpr int lds_pad(int i) { return i + (i >> pad_shift<CT>()); }
+template <int N, class CT>
+constexpr int lds_line() { return N + (N >> pad_shift<CT>()); }
 
 // ----------------------------------------------------------- stages
 // The T threads of a line always lie in one wavefront (every plan's T divides
@@ -384,7 +392,7 @@ __device__ __forceinline__ void fft_stage(CT* buf, const CT* tw, int t, bool act
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if constexpr (IN_REG) v[p][r] = reg[p * R + r];
-                else v[p][r] = buf[lds_pad(j + r * NB)];
+                else v[p][r] = buf[lds_pad<CT>(j + r * NB)];
             }
         }
     }
@@ -429,7 +437,7 @@ __device__ __forceinline__ void fft_stage(CT* buf, const CT* tw, int t, bool act
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if constexpr (OUT_REG) reg[p * R + r] = v[p][r];
-                else buf[lds_pad(base + r * NS)] = v[p][r];
+                else buf[lds_pad<CT>(base + r * NS)] = v[p][r];
             }
         }
     }
@@ -447,7 +455,7 @@ __device__ __forceinline__ void fft_stages(CT* buf, const CT* tw, int t, bool ac
     }
 }
 
-// In-place N-point FFT of a line (element i at buf[lds_pad(i)]) by the
+// In-place N-point FFT of a line (element i at buf[lds_pad<CT>(i)]) by the
 // T = FFTPlan<N>::T threads with local index t.  Requires a barrier before the
 // call (data in LDS); ends with a barrier.  All threads of the workgroup must
 // call it; threads with active = false only take part in the final barrier

@@ -25,7 +25,7 @@ template <int NF, class CT>
 constexpr int rows_lines() {
     int f = sizeof(CT) == 16 ? 8 : 16;
     if (f * FFTPlan<NF>::T > 256) f = 256 / FFTPlan<NF>::T;
-    while (f > 1 && (size_t)(f * (lds_line<NF>() + 1) + NF) * sizeof(CT) > LINE_LDS_BYTES) f /= 2;
+    while (f > 1 && (size_t)(f * (lds_line<NF, CT>() + 1) + NF) * sizeof(CT) > LINE_LDS_BYTES) f /= 2;
     return f;
 }
 
@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
     constexpr int TP = FFTPlan<NF>::T;
     constexpr int BLK = rows_block<NF, CT>();
     constexpr int F = BLK / TP;
-    constexpr int LD = lds_line<NF>() + 1;
+    constexpr int LD = lds_line<NF, CT>() + 1;
     __shared__ CT s_tw[NF];
     __shared__ CT s[F * LD];
     const CT* in = static_cast<const CT*>(p.in);
@@ -54,10 +54,10 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
             const int64_t line = l0 + l;
             CT v = cz<CT>();
             if (line < p.nlines && i < p.lin) v = in[line * p.in_pitch + i];
-            s[l * LD + lds_pad(i)] = v;
+            s[l * LD + lds_pad<CT>(i)] = v;
         }
     } else {
-        for (int e = threadIdx.x; e < F * NF; e += BLK) s[(e / NF) * LD + lds_pad(e % NF)] = cz<CT>();
+        for (int e = threadIdx.x; e < F * NF; e += BLK) s[(e / NF) * LD + lds_pad<CT>(e % NF)] = cz<CT>();
         __syncthreads();
         for (int e = threadIdx.x; e < F * p.lin; e += BLK) {
             const int l = e / p.lin, i = e - l * p.lin;
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
                 CT v = in[line * p.in_pitch + i];
                 if (p.win64) v = cscale(v, (RT)p.win64[i]);
                 else if (p.win) v = cscale(v, (RT)p.win[i]);
-                s[l * LD + lds_pad(select_bin(i, p.lin, NF, p.isel))] = v;
+                s[l * LD + lds_pad<CT>(select_bin(i, p.lin, NF, p.isel))] = v;
             }
         }
     }
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
         const int l = e / p.lout, j = e - l * p.lout;
         const int64_t line = l0 + l;
         if (line >= p.nlines) continue;
-        CT v = s[l * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
+        CT v = s[l * LD + lds_pad<CT>(select_bin(j, p.lout, NF, p.sel))];
         if (p.w64) v = cscale(v, (RT)p.w64[j]);
         else if (p.w) v = cscale(v, (RT)p.w[j]);
         out[line * p.out_pitch + j] = v;
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
 template <int NF, class CT>
 constexpr int cols_wc() {
     int w = (NF <= 512 ? 16 : 8) / (sizeof(CT) == 16 ? 2 : 1);
-    while (w > 1 && (size_t)(w * (lds_line<NF>() + 1) + NF) * sizeof(CT) > LINE_LDS_BYTES) w /= 2;
+    while (w > 1 && (size_t)(w * (lds_line<NF, CT>() + 1) + NF) * sizeof(CT) > LINE_LDS_BYTES) w /= 2;
     return w;
 }
 
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
     constexpr int WC = cols_wc<NF, CT>();
     constexpr int BLK = (WC * TP) < 256 ? WC * TP : 256;
     constexpr int F = BLK / TP;
-    constexpr int LD = lds_line<NF>() + 1;
+    constexpr int LD = lds_line<NF, CT>() + 1;
     __shared__ CT s_tw[NF];
     __shared__ CT s[WC * LD];
     const CT* in = static_cast<const CT*>(p.in);
@@ -114,10 +114,10 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
             const int64_t col = col0 + wc;
             CT v = cz<CT>();
             if (col < p.ncols && i < p.lin) v = in[b * p.in_bstride + i * p.in_stride + col];
-            s[wc * LD + lds_pad(i)] = v;
+            s[wc * LD + lds_pad<CT>(i)] = v;
         }
     } else {
-        for (int e = threadIdx.x; e < WC * NF; e += BLK) s[(e % WC) * LD + lds_pad(e / WC)] = cz<CT>();
+        for (int e = threadIdx.x; e < WC * NF; e += BLK) s[(e % WC) * LD + lds_pad<CT>(e / WC)] = cz<CT>();
         __syncthreads();
         for (int e = threadIdx.x; e < WC * p.lin; e += BLK) {
             const int wc = e % WC, i = e / WC;
@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
                 CT v = in[b * p.in_bstride + i * p.in_stride + col];
                 if (p.win64) v = cscale(v, (RT)p.win64[i]);
                 else if (p.win) v = cscale(v, (RT)p.win[i]);
-                s[wc * LD + lds_pad(select_bin(i, p.lin, NF, p.isel))] = v;
+                s[wc * LD + lds_pad<CT>(select_bin(i, p.lin, NF, p.isel))] = v;
             }
         }
     }
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
         const int wc = e % WC, j = e / WC;
         const int64_t col = col0 + wc;
         if (col >= p.ncols) continue;
-        CT v = s[wc * LD + lds_pad(select_bin(j, p.lout, NF, p.sel))];
+        CT v = s[wc * LD + lds_pad<CT>(select_bin(j, p.lout, NF, p.sel))];
         if (p.w64) v = cscale(v, (RT)p.w64[j]);
         else if (p.w) v = cscale(v, (RT)p.w[j]);
         if (!p.lam) {
