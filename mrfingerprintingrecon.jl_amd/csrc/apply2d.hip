@@ -45,7 +45,10 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     constexpr int BLK = rowpass_block<NX2, CT, ST>();
     constexpr int F = BLK / TP;              // rows transformed concurrently
     constexpr int LD = lds_line<NX2, CT>() + 1;  // LDS row pitch (odd: transposed access)
-    __shared__ CT s_tw[NX2];
+    // ComplexF64 twiddles are read from the global table (L1 / L2), which frees
+    // LDS for one more workgroup per CU; ComplexF32 ones from an LDS copy
+    __shared__ CT s_tw_lds[sizeof(CT) == 16 ? 1 : NX2];
+    const CT* s_tw = sizeof(CT) == 16 ? static_cast<const CT*>(a.tw_x) : s_tw_lds;
     __shared__ CT s_rows[RT * LD];
     const int nx = a.nx, ny = a.ny, K = a.K;
     const int tiles = tiles2d(ny, RT);
@@ -62,7 +65,8 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     }
     const int tid = threadIdx.x;
     const int y0 = tile * RT;
-    load_twiddles<NX2>(s_tw, static_cast<const CT*>(a.tw_x));
+    if constexpr (sizeof(CT) != 16)
+        load_twiddles<NX2>(s_tw_lds, static_cast<const CT*>(a.tw_x));
     const CT* bk = static_cast<const CT*>(a.b) + (size_t)k * nx * ny;
     const float2* cm = a.cmaps ? a.cmaps + (size_t)c * nx * ny : nullptr;
     // load RT rows, multiply by the coil map, zero-pad to NX2
@@ -113,7 +117,10 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     constexpr int LD = lds_line<NX2, CT>() + 1;
     constexpr int NXC = NX2 / 2;                 // = nx
     constexpr int EA = (RT * NXC + BLK - 1) / BLK;
-    __shared__ CT s_tw[NX2];
+    // ComplexF64 twiddles are read from the global table (L1 / L2), which frees
+    // LDS for one more workgroup per CU; ComplexF32 ones from an LDS copy
+    __shared__ CT s_tw_lds[sizeof(CT) == 16 ? 1 : NX2];
+    const CT* s_tw = sizeof(CT) == 16 ? static_cast<const CT*>(a.tw_x) : s_tw_lds;
     __shared__ CT s_rows[RT * LD];
     const int nx = a.nx, ny = a.ny, K = a.K, C = a.C, G = a.G;
     const int tiles = tiles2d(ny, RT);
@@ -131,7 +138,8 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
     const int tid = threadIdx.x;
     const int y0 = tile * RT;
     const int c0 = (int)((int64_t)C * grp / G), c1 = (int)((int64_t)C * (grp + 1) / G);
-    load_twiddles<NX2>(s_tw, static_cast<const CT*>(a.tw_x));
+    if constexpr (sizeof(CT) != 16)
+        load_twiddles<NX2>(s_tw_lds, static_cast<const CT*>(a.tw_x));
     CT acc[EA];
 #pragma unroll
     for (int i = 0; i < EA; ++i) acc[i] = cz<CT>();

@@ -82,7 +82,10 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     constexpr int TP = S::TP, WC = S::WC, BLK = S::BLK, NV = S::NV, EV = S::EV, P = S::P;
     constexpr int FT = S::FT;
     constexpr int LL = lds_line<NY2, CT>();
-    __shared__ CT s_tw[NY2];
+    // ComplexF64 twiddles are read from the global table (L1 / L2), which frees
+    // LDS for one more workgroup per CU; ComplexF32 ones from an LDS copy
+    __shared__ CT s_tw_lds[sizeof(CT) == 16 ? 1 : NY2];
+    const CT* s_tw = sizeof(CT) == 16 ? static_cast<const CT*>(a.tw_y) : s_tw_lds;
     __shared__ CT s_buf[WC * K * LL];
     const LT* lam = reinterpret_cast<const LT*>(a.lam);
     ST* T = static_cast<ST*>(a.T);
@@ -92,7 +95,8 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     const int qx0 = (int)(lb % ncb) * WC;
     const int grp = (int)(lb / ncb);
     const int c0 = (int)((int64_t)C * grp / Gb), c1 = (int)((int64_t)C * (grp + 1) / Gb);
-    load_twiddles<NY2>(s_tw, static_cast<const CT*>(a.tw_y));
+    if constexpr (sizeof(CT) != 16)
+        load_twiddles<NY2>(s_tw_lds, static_cast<const CT*>(a.tw_y));
 
     // kernel values of the voxels this thread owns: kept in registers for all
     // coils, or (LAMG) read again from global memory for every coil

@@ -344,8 +344,9 @@ template <int DIR> struct Butterfly<5, DIR> {
 // over the LDS banks.
 // The padding period depends on the element size, measured on MI355X at
 // 256^2 / 20 coils / K = 5: every 8 elements for 16-byte ComplexF64 (periods
-// of 16 and 32 took the apply from 0.305 to 0.310 / 0.315 ms), every 32 for
-// 8-byte ComplexF32 (0.217 ms with a period of 8, 0.193 with 16, 0.190 with 32).
+// of 2, 4, 16 and 32 took the apply from 0.306 to 0.37, 0.33, 0.310 and 0.315
+// ms), every 32 for 8-byte ComplexF32 (0.217 ms with a period of 8, 0.193 with
+// 16, 0.190 with 32).
 template <class CT>
 constexpr int pad_shift() { return sizeof(CT) == 16 ? 3 : 5; }
 template <class CT>
