@@ -384,7 +384,9 @@ def volume_leg(msr, torch, dev, args, N=128, C=16, K3=4, nt=2000, spokes=4, iter
     wall = (time.perf_counter() - t0) / steps
     print(json.dumps({"workload": f"3D recon {N}^3 (grid {2 * N}^3), {C} coils, K={K3}, koosh-ball "
                                   f"{nt} frames x {spokes} spokes x {2 * N} samples, BP + {iters}-"
-                                  f"iteration FP64 CG", "solves_per_s": round(1.0 / wall, 3),
+                                  f"iteration CG (FP64 arithmetic, {'8-byte' if PRECISION == 'f64c' else 'ComplexF64'} "
+                                  f"intermediates)", "precision": PRECISION,
+                                  "solves_per_s": round(1.0 / wall, 3),
                       "s_per_solve": round(wall, 4), "steps": steps,
                       "kernel_build_s": round(build_s, 3),
                       "device_state_bytes": list(msr.footprint(A, PRECISION))}), flush=True)

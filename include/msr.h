@@ -67,10 +67,11 @@ extern "C" {
 #define MSR_PREC_MIXED 1 /* FP64 vectors, ComplexF32 apply on an FP32 copy of the search vector */
 #define MSR_PREC_F64 2   /* FP64 vectors and ComplexF64 apply (FFTs, kernel multiply and
                             intermediates in FP64; kernel and coil maps stored as ComplexF32) */
-#define MSR_PREC_F64C 3  /* as MSR_PREC_F64, with the 2D apply's FFT intermediates stored in
-                            8 bytes per complex value (a shared exponent and two 28-bit
-                            mantissas: relative rounding 2^-27, 8x below ComplexF32 storage);
-                            3D and Cartesian operators keep ComplexF64 intermediates */
+#define MSR_PREC_F64C 3  /* as MSR_PREC_F64, with the fused 2D and 3D applies' FFT
+                            intermediates stored in 8 bytes per complex value (a shared
+                            exponent and two 28-bit mantissas: relative rounding 2^-27, 8x
+                            below ComplexF32 storage); Cartesian operators and the
+                            line-kernel apply keep ComplexF64 intermediates */
 
 typedef struct msr_ctx msr_ctx;
 typedef struct msr_traj msr_traj;

@@ -2,6 +2,7 @@
 // 2N grid, each intermediate written once and read once, the kernel read once
 // per apply for all coils (ext/MRISubspaceReconCUDAExt/NFFTNormalOp.jl:193-218
 // runs per coil: fill, pad, batched cuFFT, kernel_mul!, cuFFT, crop).
+// T1 and T2 hold ComplexF32, ComplexF64 or (the f64c mode) cx64 values.
 //   A  rows (y<Ny, z<Nz) of S_c b_k, zero-padded to 2Nx, FFT along x  -> T1[c][k][z][y][qx]
 //   B  columns along y, zero-padded to 2Ny, FFT                       -> T2[c][k][z][qy][qx]
 //   C  lines along z: pad to 2Nz, FFT, K x K kernel multiply, inverse FFT, crop to Nz (in place)
@@ -18,9 +19,9 @@ constexpr int rows3_block() {
     return (R * FFTPlan<NX2>::T) < 256 ? R * FFTPlan<NX2>::T : 256;
 }
 
-template <int NX2, class CT>
+template <int NX2, class CT, class ST>
 __global__ void __launch_bounds__(256)
-pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, CT* __restrict__ T1,
+pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, ST* __restrict__ T1,
           const CT* __restrict__ tw, int nx, int64_t nrows, int K, const int* __restrict__ skip) {
     if (skip && *skip) return;
     constexpr int TP = FFTPlan<NX2>::T;
@@ -48,17 +49,17 @@ pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, CT* __rest
     __syncthreads();
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
     fft_lds<NX2, -1>(s + g * LD, s_tw, t);
-    CT* out = T1 + ((int64_t)c * K + k) * nrows * NX2;
+    ST* out = T1 + ((int64_t)c * K + k) * nrows * NX2;
     for (int e = threadIdx.x; e < F * NX2; e += BLK) {
         const int l = e / NX2, q = e - l * NX2;
         const int64_t row = r0 + l;
-        if (row < nrows) out[row * NX2 + q] = s[l * LD + lds_pad<CT>(q)];
+        if (row < nrows) st_store<CT, ST>(out + row * NX2 + q, s[l * LD + lds_pad<CT>(q)]);
     }
 }
 
-template <int NX2, class CT>
+template <int NX2, class CT, class ST>
 __global__ void __launch_bounds__(256)
-pass_e_3d(const CT* __restrict__ T1, const float2* __restrict__ cmaps,
+pass_e_3d(const ST* __restrict__ T1, const float2* __restrict__ cmaps,
           CT* __restrict__ partial, const CT* __restrict__ tw, int nx, int64_t nrows,
           int K, int C, int G, const int* __restrict__ skip) {
     if (skip && *skip) return;
@@ -80,11 +81,11 @@ pass_e_3d(const CT* __restrict__ T1, const float2* __restrict__ cmaps,
     for (int i = 0; i < EA; ++i) acc[i] = cz<CT>();
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
     for (int c = c0; c < c1; ++c) {
-        const CT* in = T1 + ((int64_t)c * K + k) * nrows * NX2;
+        const ST* in = T1 + ((int64_t)c * K + k) * nrows * NX2;
         for (int e = threadIdx.x; e < F * NX2; e += BLK) {
             const int l = e / NX2, q = e - l * NX2;
             const int64_t row = r0 + l;
-            s[l * LD + lds_pad<CT>(q)] = row < nrows ? in[row * NX2 + q] : cz<CT>();
+            s[l * LD + lds_pad<CT>(q)] = row < nrows ? st_load<CT, ST>(in + row * NX2 + q) : cz<CT>();
         }
         __syncthreads();
         fft_lds<NX2, +1>(s + g * LD, s_tw, t);
@@ -131,7 +132,7 @@ static void launch_pass_c_3d(const Apply3DArgs& a, hipStream_t st, const int* sk
     MSR_CHECK_LAUNCH();
 }
 
-template <class CT>
+template <class CT, class ST>
 static void run_row_passes_a(const Apply3DArgs& a, const void* b, hipStream_t st, const int* skip) {
     const int NX2 = 2 * a.nx;
     const int64_t nrows = (int64_t)a.ny * a.nz;
@@ -139,8 +140,8 @@ static void run_row_passes_a(const Apply3DArgs& a, const void* b, hipStream_t st
         if constexpr (N <= 512) {
             constexpr int F = rows3_block<N, CT>() / FFTPlan<N>::T;
             dim3 grid((unsigned)ceil_div(nrows, F), a.K, a.C);
-            hipLaunchKernelGGL((pass_a_3d<N, CT>), grid, dim3(rows3_block<N, CT>()), 0, st,
-                               static_cast<const CT*>(b), a.cmaps, static_cast<CT*>(a.T1),
+            hipLaunchKernelGGL((pass_a_3d<N, CT, ST>), grid, dim3(rows3_block<N, CT>()), 0, st,
+                               static_cast<const CT*>(b), a.cmaps, static_cast<ST*>(a.T1),
                                static_cast<const CT*>(a.tw_x), a.nx, nrows, a.K, skip);
         } else {
             fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Nx = %d above 512", N);
@@ -150,15 +151,15 @@ static void run_row_passes_a(const Apply3DArgs& a, const void* b, hipStream_t st
     MSR_CHECK_LAUNCH();
 }
 
-template <class CT>
+template <class CT, class ST>
 static void run_row_passes_e(const Apply3DArgs& a, hipStream_t st, const int* skip) {
     const int64_t nrows = (int64_t)a.ny * a.nz;
     dispatch_fft_len(2 * a.nx, [&]<int N>() {
         if constexpr (N <= 512) {
             constexpr int F = rows3_block<N, CT>() / FFTPlan<N>::T;
             dim3 grid((unsigned)ceil_div(nrows, F), a.K, a.G);
-            hipLaunchKernelGGL((pass_e_3d<N, CT>), grid, dim3(rows3_block<N, CT>()), 0, st,
-                               static_cast<const CT*>(a.T1), a.cmaps, static_cast<CT*>(a.partial),
+            hipLaunchKernelGGL((pass_e_3d<N, CT, ST>), grid, dim3(rows3_block<N, CT>()), 0, st,
+                               static_cast<const ST*>(a.T1), a.cmaps, static_cast<CT*>(a.partial),
                                static_cast<const CT*>(a.tw_x), a.nx, nrows, a.K, a.C, a.G, skip);
         }
     });
@@ -168,8 +169,9 @@ static void run_row_passes_e(const Apply3DArgs& a, hipStream_t st, const int* sk
 static void run_passes(const Apply3DArgs& a, msr_ctx* ctx, const void* b, hipStream_t st,
                        const int* skip) {
     const int NX2 = 2 * a.nx;
-    if (a.f64) run_row_passes_a<double2>(a, b, st, skip);
-    else run_row_passes_a<float2>(a, b, st, skip);
+    if (!a.f64) run_row_passes_a<float2, float2>(a, b, st, skip);
+    else if (a.compact) run_row_passes_a<double2, cx64>(a, b, st, skip);
+    else run_row_passes_a<double2, double2>(a, b, st, skip);
     // B: y columns of T1 [(c,k,z)][y][qx] -> T2 [(c,k,z)][qy][qx]
     LinePass pb;
     pb.in = a.T1;
@@ -187,6 +189,7 @@ static void run_passes(const Apply3DArgs& a, msr_ctx* ctx, const void* b, hipStr
     pb.out_bstride = (int64_t)2 * a.ny * NX2;
     pb.out_stride = NX2;
     pb.skip = skip;
+    pb.in_compact = pb.out_compact = a.compact;
     run_cols(pb, ctx, st);
     launch_pass_c_3d(a, st, skip);
     // D: inverse along y, crop to Ny: T2 -> T1
@@ -199,11 +202,13 @@ static void run_passes(const Apply3DArgs& a, msr_ctx* ctx, const void* b, hipStr
     pd.in_bstride = (int64_t)2 * a.ny * NX2;
     pd.out_bstride = (int64_t)a.ny * NX2;
     run_cols(pd, ctx, st);
-    if (a.f64) run_row_passes_e<double2>(a, st, skip);
-    else run_row_passes_e<float2>(a, st, skip);
+    if (!a.f64) run_row_passes_e<float2, float2>(a, st, skip);
+    else if (a.compact) run_row_passes_e<double2, cx64>(a, st, skip);
+    else run_row_passes_e<double2, double2>(a, st, skip);
 }
 
-static Apply3DArgs args3d(msr_normalop* op, bool f64) {
+static Apply3DArgs args3d(msr_normalop* op, int am) {
+    const bool f64 = am != AM_F32;
     Apply3DArgs a;
     a.nx = (int)op->N[0];
     a.ny = (int)op->N[1];
@@ -211,6 +216,7 @@ static Apply3DArgs args3d(msr_normalop* op, bool f64) {
     a.K = op->K;
     a.C = op->C;
     a.f64 = f64 ? 1 : 0;
+    a.compact = am == AM_F64C ? 1 : 0;
     a.cmaps = op->identity_cmaps ? nullptr : op->cmaps.as<float2>();
     a.lam = op->lam->p;
     a.lam_real = op->lam_real;
@@ -231,13 +237,13 @@ static Apply3DArgs args3d(msr_normalop* op, bool f64) {
     return a;
 }
 
-void launch_apply3d_passes(msr_normalop* op, const void* b, bool f64, hipStream_t st,
+void launch_apply3d_passes(msr_normalop* op, const void* b, int am, hipStream_t st,
                            const int* skip) {
-    run_passes(args3d(op, f64), op->ctx, b, st, skip);
+    run_passes(args3d(op, am), op->ctx, b, st, skip);
 }
 
 void launch_apply3d_passes_coil(msr_normalop* op, const float2* b, int coil, hipStream_t st) {
-    Apply3DArgs a = args3d(op, false);
+    Apply3DArgs a = args3d(op, AM_F32);
     a.C = 1;
     a.G = 1;
     a.Gb = 1;

@@ -17,6 +17,7 @@ namespace msr {
 struct Apply3DArgs {
     int nx, ny, nz, K, C;
     int f64;
+    int compact;          // T1, T2 stored as cx64 (with f64)
     const float2* cmaps;  // (C, nz, ny, nx) or nullptr
     const void* lam;
     int lam_real;
@@ -46,9 +47,9 @@ struct PassC3Shape {
     static constexpr int P = K * (K + 1) / 2;
 };
 
-template <int NZ2, int K, bool LREAL, class CT>
+template <int NZ2, int K, bool LREAL, class CT, class ST>
 __global__ void __launch_bounds__(512)
-pass_c_3d(CT* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restrict__ tw,
+pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restrict__ tw,
           int nx2, int ny2, int nz, int C, int Gb, const int* __restrict__ skip) {
     if (skip && *skip) return;
     using S = PassC3Shape<NZ2, K, CT>;
@@ -102,7 +103,8 @@ pass_c_3d(CT* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
             const int qx = qx0 + wc;
             CT v = cz<CT>();
             if (z < nz && qx < nx2)
-                v = T2[(((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx];
+                v = st_load<CT, ST>(T2 + (((int64_t)c * K + k) * nz + z) * plane +
+                                    (int64_t)qy * nx2 + qx);
             s_buf[(wc * K + k) * LL + lds_pad<CT>(z)] = v;
         }
         __syncthreads();
@@ -142,21 +144,28 @@ pass_c_3d(CT* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
             const int k = r % K, z = r / K;
             const int qx = qx0 + wc;
             if (z < nz && qx < nx2)
-                T2[(((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx] =
-                    s_buf[(wc * K + k) * LL + lds_pad<CT>(z)];
+                st_store<CT, ST>(T2 + (((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx,
+                                 s_buf[(wc * K + k) * LL + lds_pad<CT>(z)]);
         }
         __syncthreads();
     }
 }
 
-template <int NZ2, int K, bool LREAL, class CT>
+template <int NZ2, int K, bool LREAL, class CT, class ST>
 void launch_pass_c_3d_t(const Apply3DArgs& a, hipStream_t st, const int* skip) {
     using S = PassC3Shape<NZ2, K, CT>;
     const int nx2 = 2 * a.nx, ny2 = 2 * a.ny;
     dim3 grid((nx2 + S::WC - 1) / S::WC, ny2, a.Gb);
-    hipLaunchKernelGGL((pass_c_3d<NZ2, K, LREAL, CT>), grid, dim3(S::BLK), 0, st,
-                       static_cast<CT*>(a.T2), a.lam, static_cast<const CT*>(a.tw_z), nx2, ny2,
+    hipLaunchKernelGGL((pass_c_3d<NZ2, K, LREAL, CT, ST>), grid, dim3(S::BLK), 0, st,
+                       static_cast<ST*>(a.T2), a.lam, static_cast<const CT*>(a.tw_z), nx2, ny2,
                        a.nz, a.C, a.Gb, skip);
+}
+
+template <int NZ2, int K, bool LREAL>
+void launch_pass_c_3d_mode(const Apply3DArgs& a, hipStream_t st, const int* skip) {
+    if (!a.f64) launch_pass_c_3d_t<NZ2, K, LREAL, float2, float2>(a, st, skip);
+    else if (a.compact) launch_pass_c_3d_t<NZ2, K, LREAL, double2, cx64>(a, st, skip);
+    else launch_pass_c_3d_t<NZ2, K, LREAL, double2, double2>(a, st, skip);
 }
 
 #define MSR_DECL_PASSC3(KK) void launch_pass_c_3d_k##KK(const Apply3DArgs& a, hipStream_t st, const int* skip);
@@ -169,13 +178,8 @@ MSR_DECL_PASSC3(5) MSR_DECL_PASSC3(6) MSR_DECL_PASSC3(7) MSR_DECL_PASSC3(8)
         const int nz2 = 2 * a.nz;                                                              \
         bool ok = dispatch_fft_len(nz2, [&]<int N>() {                                         \
             if constexpr (N <= 512) {                                                          \
-                if (a.f64) {                                                                   \
-                    if (a.lam_real) launch_pass_c_3d_t<N, KK, true, double2>(a, st, skip);     \
-                    else launch_pass_c_3d_t<N, KK, false, double2>(a, st, skip);               \
-                } else {                                                                       \
-                    if (a.lam_real) launch_pass_c_3d_t<N, KK, true, float2>(a, st, skip);      \
-                    else launch_pass_c_3d_t<N, KK, false, float2>(a, st, skip);                \
-                }                                                                              \
+                if (a.lam_real) launch_pass_c_3d_mode<N, KK, true>(a, st, skip);               \
+                else launch_pass_c_3d_mode<N, KK, false>(a, st, skip);                         \
             } else {                                                                           \
                 fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Nz = %d above 512", N);             \
             }                                                                                  \

@@ -48,6 +48,9 @@ struct LinePass {
     int64_t ny2 = 0;
     // device flag: the pass does nothing when *skip != 0 (early-stopped CG)
     const int* skip = nullptr;
+    // cols passes with f64: input / output stored as cx64 (8 bytes per value,
+    // common.hpp) instead of ComplexF64 (the 3D apply's compact intermediates)
+    int in_compact = 0, out_compact = 0;
 };
 
 void run_rows(const LinePass& p, msr_ctx* ctx, hipStream_t st);

@@ -105,6 +105,20 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
     __shared__ CT s[WC * LD];
     const CT* in = static_cast<const CT*>(p.in);
     CT* out = static_cast<CT*>(p.out);
+    // element idx of the input / output, ComplexF64 or (f64 only) cx64
+    auto load_in = [&](int64_t idx) -> CT {
+        if constexpr (sizeof(CT) == 16)
+            if (p.in_compact) return st_load<CT, cx64>(static_cast<const cx64*>(p.in) + idx);
+        return in[idx];
+    };
+    auto store_out = [&](int64_t idx, CT v) {
+        if constexpr (sizeof(CT) == 16)
+            if (p.out_compact) {
+                st_store<CT, cx64>(static_cast<cx64*>(p.out) + idx, v);
+                return;
+            }
+        out[idx] = v;
+    };
     load_twiddles<NF>(s_tw, tw);
     const int64_t col0 = (int64_t)blockIdx.x * WC;
     const int64_t b = blockIdx.y;
@@ -113,7 +127,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
             const int wc = e % WC, i = e / WC;
             const int64_t col = col0 + wc;
             CT v = cz<CT>();
-            if (col < p.ncols && i < p.lin) v = in[b * p.in_bstride + i * p.in_stride + col];
+            if (col < p.ncols && i < p.lin) v = load_in(b * p.in_bstride + i * p.in_stride + col);
             s[wc * LD + lds_pad<CT>(i)] = v;
         }
     } else {
@@ -123,7 +137,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
             const int wc = e % WC, i = e / WC;
             const int64_t col = col0 + wc;
             if (col < p.ncols) {
-                CT v = in[b * p.in_bstride + i * p.in_stride + col];
+                CT v = load_in(b * p.in_bstride + i * p.in_stride + col);
                 if (p.win64) v = cscale(v, (RT)p.win64[i]);
                 else if (p.win) v = cscale(v, (RT)p.win[i]);
                 s[wc * LD + lds_pad<CT>(select_bin(i, p.lin, NF, p.isel))] = v;
@@ -142,7 +156,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
         if (p.w64) v = cscale(v, (RT)p.w64[j]);
         else if (p.w) v = cscale(v, (RT)p.w[j]);
         if (!p.lam) {
-            out[b * p.out_bstride + j * p.out_stride + col] = v;
+            store_out(b * p.out_bstride + j * p.out_stride + col, v);
         } else {
             // dense Toeplitz kernel, rounded once to ComplexF32 / Float32
             const int64_t pp = b + p.p0;

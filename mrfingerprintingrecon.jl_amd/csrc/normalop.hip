@@ -323,8 +323,8 @@ Apply2DArgs args2d(msr_normalop* op, int am) {
 double op_scale(const msr_normalop* op) { return 1.0 / (double)op_grid_cells(op); }
 
 // partial[g] <- unscaled group sums of A^H A b (b and partial of the apply type)
-// am: ApplyMode.  The compact storage of AM_F64C applies to the 2D
-// non-Cartesian passes; the 3D and Cartesian passes keep ComplexF64
+// am: ApplyMode.  The compact storage of AM_F64C applies to the fused 2D and
+// 3D passes; the Cartesian and line-kernel passes keep ComplexF64
 // intermediates in that mode.
 void op_passes(msr_normalop* op, const void* b, int am, hipStream_t st, const int* skip) {
     const bool f64 = am != AM_F32;
@@ -340,7 +340,7 @@ void op_passes(msr_normalop* op, const void* b, int am, hipStream_t st, const in
         a.skip = skip;
         launch_apply2d_passes(a, st);
     } else {
-        launch_apply3d_passes(op, b, f64, st, skip);
+        launch_apply3d_passes(op, b, am, st, skip);
     }
 }
 

@@ -178,3 +178,29 @@ def test_3d_spreading_atomic_and_slab_paths_agree(torch, msr, monkeypatch):
           f"vs oracle {rel_err(lam_a, lam_ref):.1e} {rel_err(b_a, b_ref):.1e}", flush=True)
     assert rel_err(lam_s, lam_a) < 1e-7 and rel_err(b_s, b_a) < 1e-6
     assert rel_err(lam_a, lam_ref) < 3e-7 and rel_err(b_a, b_ref) < 2e-6
+
+
+def test_recon3d_compact_solve(torch, msr):
+    """The f64c precision in 3D: FP64 arithmetic with the apply's T1 / T2
+    intermediates stored in 8 bytes (four storage roundings of 2^-27 per
+    apply) against the FP64 oracle's CG on the same operator, 15 iterations."""
+    N = (16, 16, 16)
+    trj, U, cm, x = kooshball_case(N, 2, 4, 10, 8, seed=5)
+    cml = [dev(torch, c) for c in cm]
+    A = msr.NFFTNormalOp(N, dev(torch, trj), dev(torch, U), cmaps=cml)
+    rng = np.random.default_rng(6)
+    data = (rng.normal(size=(4,) + trj.shape[:2]) + 1j * rng.normal(size=(4,) + trj.shape[:2])).astype(np.complex64)
+    b = msr.calculate_backprojection(dev(torch, data), dev(torch, trj), cml, U=dev(torch, U))
+    bd = b.reshape(-1)
+    lam = A.prod.A.Λ.cpu().numpy().astype(np.complex128)
+    km = A.prod.A.kmask_indcs.cpu().numpy() - 1
+    bv = b.cpu().numpy()
+    Af = lambda v: toeplitz.apply_normal(N, lam, km, cm.astype(np.complex128),
+                                         v.reshape(bv.shape)).reshape(-1)
+    ref = ocg.cg(Af, bv.reshape(-1).astype(np.complex128), maxiter=15, reltol=0.0)
+    for mode in ("device", "graph"):
+        xg = msr.cg(A, bd, maxiter=15, reltol=0.0, mode=mode, precision="f64c")
+        e = rel_err(xg.cpu().numpy(), ref)
+        print(f"3d solve f64c ({mode}) rel err {e:.3e}", flush=True)
+        assert e < 2e-5
+

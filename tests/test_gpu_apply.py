@@ -308,11 +308,13 @@ def test_cg_warm_start(torch, msr):
 
 @pytest.mark.parametrize("N,K,C,real", [((16, 16), 3, 4, False), ((48, 80), 5, 3, False),
                                         ((96, 48), 6, 2, True), ((256, 256), 5, 20, False),
-                                        ((160, 32), 7, 2, False)])
+                                        ((160, 32), 7, 2, False), ((16, 16, 16), 3, 3, False),
+                                        ((24, 20, 16), 2, 2, True), ((32, 32, 32), 4, 4, False)])
 def test_compact_storage_apply_matches_oracle(torch, msr, N, K, C, real):
     """The FP64 apply with 8-byte intermediates (mul_f64_(compact=True), the
-    apply of the "f64c" precision) against the FP64 oracle: the two storage
-    roundings of each value (2^-27 of the larger part) bound the error."""
+    apply of the "f64c" precision) against the FP64 oracle: the storage
+    roundings of each value (2^-27 of the larger part; two per apply in 2D, four
+    in 3D) bound the error."""
     print(f"compact apply N={N} K={K} C={C} real={real}", flush=True)
     rng = np.random.default_rng(int(np.prod(N)) + K)
     lam, km, cm, x = make_case(rng, N, K, C, real)
