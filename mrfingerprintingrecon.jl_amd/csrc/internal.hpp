@@ -90,6 +90,9 @@ struct SpreadPlan {
     DevBuf items;               // (nitems) int4 {tile, q0, q1, slab slot or -1}
     DevBuf split_tiles;         // (nsplit) int2 {tile, first item}; items of a split tile are consecutive
     DevBuf split_nitems;        // (nsplit) int32
+    DevBuf order;               // (8, order_len) int32: dispatch table of the gathers (item
+                                // per XCD and position, -1 past an XCD's last item)
+    int64_t order_len = 0;
 };
 
 // Cell sort of an integer (Cartesian) trajectory's samples for an N grid

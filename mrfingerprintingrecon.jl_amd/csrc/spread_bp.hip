@@ -5,16 +5,18 @@
 // blocks; every block has the list of the (tile-sorted) points whose W = 8
 // support reaches it, in increasing point order, cut into work items of at
 // most ITEM_PAIRS points.  One wavefront per (work item, group of CG
-// channels), one lane per cell of the block: a lane keeps its cell's CG
-// channel sums in registers and walks the item's points, adding w_x w_y v.
-// A point's 2 CG value floats are loaded 16 per VGPR, the same 16 in each of
-// the four 16-lane rows, and each FMA takes value j from lane j of its row
-// through DPP row_newbcast, so a point costs 2 CG FMAs per cell and no other
-// arithmetic.  Its kernel weights over the block's columns and rows were laid
-// out per (point, block) pair when the streams were built.  All loads are
-// vector loads issued a few points ahead of their use (they complete in
-// order).  Each cell's sum runs over the
-// points in a fixed order (bit-reproducible).  A block with one work item
+// channels) walks the item's points and sums w_x w_y v into the block's
+// 64 cells.  Groups of 16 channels run on the matrix cores (mfma_gather16:
+// the block's sums are the product of the points' values and their weights
+// at the cells, two points per v_mfma_f32_32x32x2f32 pair).  Groups of 1 or 8
+// channels keep one lane per cell: a point's 2 CG value floats are loaded 16
+// per VGPR, the same 16 in each of the four 16-lane rows, and each FMA takes
+// value j from lane j of its row through DPP row_newbcast, so a point costs
+// 2 CG FMAs per cell and no other arithmetic.  A point's kernel weights over
+// the block's columns and rows were laid out per (point, block) pair when the
+// streams were built.  All loads are vector loads issued a few points ahead
+// of their use (they complete in order).  Each cell's sum runs over the
+// points in a fixed order (bit-reproducible; both forms give the same bits).  A block with one work item
 // writes its cells straight into the grid; the items of a split block (the
 // dense centre of a radial trajectory) write partial blocks that a second
 // kernel sums in item order.
@@ -187,7 +189,10 @@ void spread_streams(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
     keys.alloc(sizeof(int) * P);
     keys_sorted.alloc(sizeof(int) * P);
     vals.alloc(sizeof(int) * P);
-    plan.pairs.alloc(sizeof(int) * P);
+    // zero pair records past the last pair: the gathers' prefetch reads up to
+    // BP_PAD_PAIRS of them
+    plan.pairs.alloc(sizeof(int) * (P + BP_PAD_PAIRS));
+    fill_floats(plan.pairs.as<int>() + total, P - total + BP_PAD_PAIRS, 0.f, st);
     if (M > 0) {
         hipLaunchKernelGGL(bp_emit_pairs_kernel, dim3(grid), dim3(256), 0, st, plan.cell.as<int2>(),
                            ppos.as<int>(), M, g, keys.as<int>(), vals.as<int>());
@@ -207,7 +212,8 @@ void spread_streams(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
                                                    bits, st));
     }
     // pair weights (keys_sorted: the block of each pair)
-    plan.pw.alloc(sizeof(float) * 16 * P);
+    plan.pw.alloc(sizeof(float) * 16 * (P + BP_PAD_PAIRS));
+    fill_floats(plan.pw.as<float>() + 16 * (int64_t)total, 16 * (P - total + BP_PAD_PAIRS), 0.f, st);
     if (total > 0) {
         const int g2 = (int)std::min<int64_t>((total + 255) / 256, 8192);
         hipLaunchKernelGGL(bp_pair_weights_kernel, dim3(g2), dim3(256), 0, st, keys_sorted.as<int>(),
@@ -258,12 +264,24 @@ void spread_streams(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
                                hipMemcpyHostToDevice, st));
     }
     plan.nslots = slot;
+    std::vector<uint64_t> key(items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+        const int c[2] = {items[i].x % g.nb[0], items[i].x / g.nb[0]};
+        key[i] = bp_morton_key(c, 2);
+    }
+    const std::vector<int> order = bp_dispatch_order(items, key, plan.order_len);
+    plan.order.alloc(sizeof(int) * order.size());
+    MSR_HIP(hipMemcpyAsync(plan.order.p, order.data(), sizeof(int) * order.size(),
+                           hipMemcpyHostToDevice, st));
     MSR_HIP(hipStreamSynchronize(st));  // host staging and temporaries go out of scope
     plan.streams_ready = true;
 }
 
 struct BpGatherArgs {
     const int4* items;
+    const int* order;     // dispatch table (bp_dispatch_order)
+    int64_t order_len;
+    int ngp;              // channel groups in this launch
     const int* pairs;
     const float* pw;
     const float2* vals;   // (ngp, M, CG)
@@ -286,9 +304,11 @@ __global__ void __launch_bounds__(64) bp_gather_kernel(BpGatherArgs a) {
     constexpr int NV = CG == 1 ? 1 : 2 * CG / 16;  // value VGPRs per point
     constexpr int LV = CG == 1 ? 2 : 16;          // value floats per VGPR (per 16-lane row)
     constexpr int D = BP_DEPTH;
-    const int4 it = a.items[blockIdx.x];
+    int gl;
+    const int ii = bp_item_of(a.order, a.order_len, a.ngp, gl);
+    if (ii < 0) return;
+    const int4 it = a.items[ii];
     const int tile = it.x, q0 = it.y, q1 = it.z, slot = it.w;
-    const int gl = blockIdx.y;
     const int ch0 = (a.g0 + gl) * CG;
     const int n0 = (int)a.g.n[0], n1 = (int)a.g.n[1];
     const int t0x = (tile % a.g.nb[0]) * BP_TB, t0y = (tile / a.g.nb[0]) * BP_TB;
@@ -304,6 +324,7 @@ __global__ void __launch_bounds__(64) bp_gather_kernel(BpGatherArgs a) {
     for (int j = 0; j < 2 * CG; ++j) acc[j] = 0.f;
     const int n = q1 - q0;
     const int last = q1 - 1;
+    __builtin_assume(n >= 1);
     int wd[D];            // pair words of points p + D .. p + 2D - 1
     float vv[D][NV];      // values of points p .. p + D - 1
     float wx[D], wy[D];   // weights of points p .. p + D - 1
@@ -321,6 +342,8 @@ __global__ void __launch_bounds__(64) bp_gather_kernel(BpGatherArgs a) {
     }
 #pragma unroll
     for (int i = 0; i < D; ++i) wd[i] = pq[min(q0 + D + i, last)];
+    asm volatile("" ::: "memory");  // prologue loads complete before the loop
+    __builtin_amdgcn_s_waitcnt(BP_VMCNT0);
     for (int p = 0; p < n; p += D) {
 #pragma unroll
         for (int i = 0; i < D; ++i) {
@@ -334,6 +357,7 @@ __global__ void __launch_bounds__(64) bp_gather_kernel(BpGatherArgs a) {
                 }
             }
             issue(i, p + i + D, wd[i]);
+            __builtin_amdgcn_sched_barrier(0);  // next word into the same register
             wd[i] = pq[min(q0 + p + i + 2 * D, last)];
         }
     }
@@ -348,6 +372,72 @@ __global__ void __launch_bounds__(64) bp_gather_kernel(BpGatherArgs a) {
             a.grid[(int64_t)(ch0 + j) * G + (int64_t)(t0y + ly) * n0 + t0x + lx] = r;
         else
             a.slab[((int64_t)slot * a.nch + ch0 + j) * (BP_TB * BP_TB) + lane] = r;
+    }
+    (void)n1;
+}
+
+// kernel weight of cell (cx, cy) (acc0) and (cx, cy + 4) (acc1) of the 8 x 8
+// block, from the pair's weights over the block's columns and rows
+struct Bp2Cells {
+    static constexpr uint32_t PW_BYTES = 16 * sizeof(float);
+    static constexpr int NW = 3;
+    uint32_t ox, oy;  // lane's byte offsets in a pair's weights: column cx, row cy
+    // weights of the pair whose record starts at p + rec
+    __device__ __forceinline__ void load(const char* __restrict__ p, uint32_t rec,
+                                         float (&w)[NW]) const {
+        w[0] = bp_ldf(p, rec + ox);
+        w[1] = bp_ldf(p, rec + oy);
+        w[2] = bp_ldf(p + 4 * sizeof(float), rec + oy);
+    }
+    __device__ __forceinline__ void product(const float (&w)[NW], float& b0, float& b1) const {
+        b0 = w[0] * w[1];
+        b1 = w[0] * w[2];
+    }
+};
+
+// The 16-channel group on the matrix cores (mfma_gather16): one wavefront per
+// (work item, group); lane l ends with cell (l & 31) of rows 0..3 in acc0 and
+// cell (l & 31) + 32 in acc1, eight of the sixteen channels each
+// (mfma_channel).
+template <bool WIDE>
+__global__ void __launch_bounds__(64) bp_gather_mfma_kernel(BpGatherArgs a) {
+    int gl;
+    const int ii = bp_item_of(a.order, a.order_len, a.ngp, gl);
+    if (ii < 0) return;
+    const int4 it = a.items[ii];
+    const int tile = it.x, q0 = it.y, q1 = it.z, slot = it.w;
+    const int ch0 = (a.g0 + gl) * BP_CGMAX;
+    const int n0 = (int)a.g.n[0], n1 = (int)a.g.n[1];
+    const int t0x = (tile % a.g.nb[0]) * BP_TB, t0y = (tile / a.g.nb[0]) * BP_TB;
+    const int lane = threadIdx.x;
+    const int j = lane & 31, h = lane >> 5;
+    const int cx = j & 7, cy = j >> 3;
+    const float* __restrict__ vg = reinterpret_cast<const float*>(a.vals + (size_t)gl * a.M * BP_CGMAX);
+    const Bp2Cells cells{(uint32_t)(cx * sizeof(float)), (uint32_t)((BP_TB + cy) * sizeof(float))};
+    f32x16 acc0, acc1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        acc0[r] = 0.f;
+        acc1[r] = 0.f;
+    }
+    mfma_gather16<WIDE>(vg, a.pairs, a.pw, q0, q1, cells, acc0, acc1);
+    const int ex = min(BP_TB, n0 - t0x), ey = min(BP_TB, n1 - t0y);
+    const int64_t G = a.g.n[0] * a.g.n[1];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const f32x16& acc = half ? acc1 : acc0;
+        const int ly = cy + 4 * half;
+        if (cx >= ex || ly >= ey) continue;
+#pragma unroll
+        for (int r = 0; r < 16; r += 2) {
+            const int c = ch0 + mfma_channel(r, h);
+            if (c >= a.nch) continue;
+            const float2 v = make_float2(acc[r], acc[r + 1]);
+            if (slot < 0)
+                a.grid[(int64_t)c * G + (int64_t)(t0y + ly) * n0 + t0x + cx] = v;
+            else
+                a.slab[((int64_t)slot * a.nch + c) * (BP_TB * BP_TB) + ly * BP_TB + cx] = v;
+        }
     }
     (void)n1;
 }
@@ -421,12 +511,17 @@ void spread_bp(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src
         else MSR_BP_VALUES(BP_CGMAX);
 #undef MSR_BP_VALUES
         MSR_CHECK_LAUNCH();
-        BpGatherArgs a{plan.items.as<int4>(), plan.pairs.as<int>(), plan.pw.as<float>(), vals, M,
-                       nch, g0, g, grid, slab};
-        dim3 grd((unsigned)plan.nitems, (unsigned)ngp);
+        BpGatherArgs a{plan.items.as<int4>(), plan.order.as<int>(), plan.order_len, ngp,
+                       plan.pairs.as<int>(), plan.pw.as<float>(), vals, M, nch, g0, g, grid, slab};
+        const int64_t nwg = (int64_t)BP_XCDS * plan.order_len * ngp;
+        MSR_REQUIRE(nwg < ((int64_t)1 << 31), MSR_ERR_UNSUPPORTED, "too many work items");
+        dim3 grd((unsigned)nwg);
         if (CG == 1) hipLaunchKernelGGL(bp_gather_kernel<1>, grd, dim3(64), 0, st, a);
         else if (CG == 8) hipLaunchKernelGGL(bp_gather_kernel<8>, grd, dim3(64), 0, st, a);
-        else hipLaunchKernelGGL(bp_gather_kernel<BP_CGMAX>, grd, dim3(64), 0, st, a);
+        else if (M <= BP_NARROW_POINTS)
+            hipLaunchKernelGGL(bp_gather_mfma_kernel<false>, grd, dim3(64), 0, st, a);
+        else
+            hipLaunchKernelGGL(bp_gather_mfma_kernel<true>, grd, dim3(64), 0, st, a);
         MSR_CHECK_LAUNCH();
     }
     if (plan.nsplit > 0) {

@@ -3,11 +3,11 @@
 // ext/MRISubspaceReconCUDAExt/BackProjection.jl:55-71): the 2D gather of
 // spread_bp.hip on 4 x 4 x 4 output blocks.
 //
-// One wavefront per (work item, group of CG channels), one lane per cell
-// (lx, ly, lz) of the block: the lane keeps its cell's CG channel sums in
-// registers and walks the item's points, adding wx[lx] wy[ly] wz[lz] v with
-// the point's values broadcast from lanes by DPP (two CG multiply-adds per
-// cell and point).  The kernel weights of a point over the block's four
+// One wavefront per (work item, group of CG channels) walks the item's points
+// and sums wx wy wz v into the block's 64 cells: on the matrix cores for
+// groups of 16 channels (mfma_gather16), else one lane per cell (lx, ly, lz)
+// holding its cell's CG channel sums, with the point's values broadcast from
+// lanes by DPP (two CG multiply-adds per cell and point).  The kernel weights of a point over the block's four
 // columns, rows and planes (12 floats per (block, point) pair) are laid out
 // when the streams are built.  A point's W = 8 support covers two or three
 // blocks per dimension, about 21 pairs per point on average, and 512 of the
@@ -207,7 +207,10 @@ void spread_streams3(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
     keys.alloc(sizeof(int) * P);
     keys_sorted.alloc(sizeof(int) * P);
     vals.alloc(sizeof(int) * P);
-    plan.pairs.alloc(sizeof(int) * P);
+    // zero pair records past the last pair: the gathers' prefetch reads up to
+    // BP_PAD_PAIRS of them
+    plan.pairs.alloc(sizeof(int) * (P + BP_PAD_PAIRS));
+    fill_floats(plan.pairs.as<int>() + total, P - total + BP_PAD_PAIRS, 0.f, st);
     if (M > 0) {
         hipLaunchKernelGGL(bp3_emit_pairs_kernel, dim3(grid), dim3(256), 0, st, plan.cell.as<int4>(),
                            ppos.as<int>(), M, g, keys.as<int>(), vals.as<int>());
@@ -228,7 +231,8 @@ void spread_streams3(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
     }
     vals.release();
     keys.release();
-    plan.pw.alloc(sizeof(float) * 12 * P);
+    plan.pw.alloc(sizeof(float) * 12 * (P + BP_PAD_PAIRS));
+    fill_floats(plan.pw.as<float>() + 12 * (int64_t)total, 12 * (P - total + BP_PAD_PAIRS), 0.f, st);
     if (total > 0) {
         const int g2 = (int)std::min<int64_t>((total + 255) / 256, 8192);
         hipLaunchKernelGGL(bp3_pair_weights_kernel, dim3(g2), dim3(256), 0, st,
@@ -279,12 +283,25 @@ void spread_streams3(const msr_traj* traj, SpreadPlan& plan, hipStream_t st) {
                                hipMemcpyHostToDevice, st));
     }
     plan.nslots = slot;
+    std::vector<uint64_t> key(items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+        const int b = items[i].x;
+        const int c[3] = {b % g.nb[0], (b / g.nb[0]) % g.nb[1], b / (g.nb[0] * g.nb[1])};
+        key[i] = bp_morton_key(c, 3);
+    }
+    const std::vector<int> order = bp_dispatch_order(items, key, plan.order_len);
+    plan.order.alloc(sizeof(int) * order.size());
+    MSR_HIP(hipMemcpyAsync(plan.order.p, order.data(), sizeof(int) * order.size(),
+                           hipMemcpyHostToDevice, st));
     MSR_HIP(hipStreamSynchronize(st));  // host staging and temporaries go out of scope
     plan.streams_ready = true;
 }
 
 struct Bp3GatherArgs {
     const int4* items;
+    const int* order;     // dispatch table (bp_dispatch_order)
+    int64_t order_len;
+    int ngp;              // channel groups in this launch
     const int* pairs;
     const float* pw;
     const float2* vals;   // (ngp, M, CG)
@@ -303,9 +320,11 @@ __global__ void __launch_bounds__(64) bp3_gather_kernel(Bp3GatherArgs a) {
     constexpr int NV = CG == 1 ? 1 : 2 * CG / 16;  // value VGPRs per point
     constexpr int LV = CG == 1 ? 2 : 16;          // value floats per VGPR (per 16-lane row)
     constexpr int D = BP_DEPTH;
-    const int4 it = a.items[blockIdx.x];
+    int gl;
+    const int ii = bp_item_of(a.order, a.order_len, a.ngp, gl);
+    if (ii < 0) return;
+    const int4 it = a.items[ii];
     const int blk = it.x, q0 = it.y, q1 = it.z, slot = it.w;
-    const int gl = blockIdx.y;
     const int ch0 = (a.g0 + gl) * CG;
     const int n0 = (int)a.g.n[0], n1 = (int)a.g.n[1], n2 = (int)a.g.n[2];
     const int bx = blk % a.g.nb[0], br = blk / a.g.nb[0];
@@ -323,6 +342,7 @@ __global__ void __launch_bounds__(64) bp3_gather_kernel(Bp3GatherArgs a) {
     for (int j = 0; j < 2 * CG; ++j) acc[j] = 0.f;
     const int n = q1 - q0;
     const int last = q1 - 1;
+    __builtin_assume(n >= 1);
     int wd[D];               // pair words of points p + D .. p + 2D - 1
     float vv[D][NV];         // values of points p .. p + D - 1
     float wx[D], wy[D], wz[D];
@@ -338,6 +358,8 @@ __global__ void __launch_bounds__(64) bp3_gather_kernel(Bp3GatherArgs a) {
     for (int i = 0; i < D; ++i) issue(i, i, pq[min(q0 + i, last)]);
 #pragma unroll
     for (int i = 0; i < D; ++i) wd[i] = pq[min(q0 + D + i, last)];
+    asm volatile("" ::: "memory");  // prologue loads complete before the loop
+    __builtin_amdgcn_s_waitcnt(BP_VMCNT0);
     for (int p = 0; p < n; p += D) {
 #pragma unroll
         for (int i = 0; i < D; ++i) {
@@ -351,6 +373,7 @@ __global__ void __launch_bounds__(64) bp3_gather_kernel(Bp3GatherArgs a) {
                 }
             }
             issue(i, p + i + D, wd[i]);
+            __builtin_amdgcn_sched_barrier(0);  // next word into the same register
             wd[i] = pq[min(q0 + p + i + 2 * D, last)];
         }
     }
@@ -365,6 +388,74 @@ __global__ void __launch_bounds__(64) bp3_gather_kernel(Bp3GatherArgs a) {
             a.grid[(int64_t)(ch0 + j) * G + ((int64_t)z * n1 + y) * n0 + x] = r;
         else
             a.slab[((int64_t)slot * a.nch + ch0 + j) * 64 + lane] = r;
+    }
+}
+
+// kernel weight of cell (cx, cy, cz) (acc0) and (cx, cy, cz + 2) (acc1) of the
+// 4 x 4 x 4 block, from the pair's weights over the block's x, y and z lines
+struct Bp3Cells {
+    static constexpr uint32_t PW_BYTES = 12 * sizeof(float);
+    static constexpr int NW = 4;
+    uint32_t ox, oy, oz;  // lane's byte offsets in a pair's weights: cx, cy, cz
+    // weights of the pair whose record starts at p + rec
+    __device__ __forceinline__ void load(const char* __restrict__ p, uint32_t rec,
+                                         float (&w)[NW]) const {
+        w[0] = bp_ldf(p, rec + ox);
+        w[1] = bp_ldf(p, rec + oy);
+        w[2] = bp_ldf(p, rec + oz);
+        w[3] = bp_ldf(p + 2 * sizeof(float), rec + oz);
+    }
+    __device__ __forceinline__ void product(const float (&w)[NW], float& b0, float& b1) const {
+        const float xy = w[0] * w[1];
+        b0 = xy * w[2];
+        b1 = xy * w[3];
+    }
+};
+
+// The 16-channel group on the matrix cores (mfma_gather16): lane l ends with
+// cell (l & 31) of z planes 0..1 in acc0 and cell (l & 31) + 32 in acc1, eight
+// of the sixteen channels each (mfma_channel).
+template <bool WIDE>
+__global__ void __launch_bounds__(64) bp3_gather_mfma_kernel(Bp3GatherArgs a) {
+    int gl;
+    const int ii = bp_item_of(a.order, a.order_len, a.ngp, gl);
+    if (ii < 0) return;
+    const int4 it = a.items[ii];
+    const int blk = it.x, q0 = it.y, q1 = it.z, slot = it.w;
+    const int ch0 = (a.g0 + gl) * BP_CGMAX;
+    const int n0 = (int)a.g.n[0], n1 = (int)a.g.n[1], n2 = (int)a.g.n[2];
+    const int bx = blk % a.g.nb[0], br = blk / a.g.nb[0];
+    const int t0x = bx * BP3_TB, t0y = (br % a.g.nb[1]) * BP3_TB, t0z = (br / a.g.nb[1]) * BP3_TB;
+    const int lane = threadIdx.x;
+    const int j = lane & 31, h = lane >> 5;
+    const int cx = j & 3, cy = (j >> 2) & 3, cz = j >> 4;
+    const float* __restrict__ vg = reinterpret_cast<const float*>(a.vals + (size_t)gl * a.M * BP_CGMAX);
+    const Bp3Cells cells{(uint32_t)(cx * sizeof(float)), (uint32_t)((BP3_TB + cy) * sizeof(float)),
+                         (uint32_t)((2 * BP3_TB + cz) * sizeof(float))};
+    f32x16 acc0, acc1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        acc0[r] = 0.f;
+        acc1[r] = 0.f;
+    }
+    mfma_gather16<WIDE>(vg, a.pairs, a.pw, q0, q1, cells, acc0, acc1);
+    const int64_t G = a.g.n[0] * a.g.n[1] * a.g.n[2];
+    const int x = t0x + cx, y = t0y + cy;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const f32x16& acc = half ? acc1 : acc0;
+        const int z = t0z + cz + 2 * half;
+        if (x >= n0 || y >= n1 || z >= n2) continue;
+#pragma unroll
+        for (int r = 0; r < 16; r += 2) {
+            const int c = ch0 + mfma_channel(r, h);
+            if (c >= a.nch) continue;
+            const float2 v = make_float2(acc[r], acc[r + 1]);
+            if (slot < 0)
+                a.grid[(int64_t)c * G + ((int64_t)z * n1 + y) * n0 + x] = v;
+            else
+                a.slab[((int64_t)slot * a.nch + c) * 64 + j + 32 * half] = v;
+        }
     }
 }
 
@@ -429,12 +520,17 @@ void spread_bp3(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& sr
         else MSR_BP3_VALUES(BP_CGMAX);
 #undef MSR_BP3_VALUES
         MSR_CHECK_LAUNCH();
-        Bp3GatherArgs a{plan.items.as<int4>(), plan.pairs.as<int>(), plan.pw.as<float>(), vals, M,
-                        nch, g0, g, grid, slab};
-        dim3 grd((unsigned)plan.nitems, (unsigned)ngp);
+        Bp3GatherArgs a{plan.items.as<int4>(), plan.order.as<int>(), plan.order_len, ngp,
+                        plan.pairs.as<int>(), plan.pw.as<float>(), vals, M, nch, g0, g, grid, slab};
+        const int64_t nwg = (int64_t)BP_XCDS * plan.order_len * ngp;
+        MSR_REQUIRE(nwg < ((int64_t)1 << 31), MSR_ERR_UNSUPPORTED, "too many work items");
+        dim3 grd((unsigned)nwg);
         if (CG == 1) hipLaunchKernelGGL(bp3_gather_kernel<1>, grd, dim3(64), 0, st, a);
         else if (CG == 8) hipLaunchKernelGGL(bp3_gather_kernel<8>, grd, dim3(64), 0, st, a);
-        else hipLaunchKernelGGL(bp3_gather_kernel<BP_CGMAX>, grd, dim3(64), 0, st, a);
+        else if (M <= BP_NARROW_POINTS)
+            hipLaunchKernelGGL(bp3_gather_mfma_kernel<false>, grd, dim3(64), 0, st, a);
+        else
+            hipLaunchKernelGGL(bp3_gather_mfma_kernel<true>, grd, dim3(64), 0, st, a);
         MSR_CHECK_LAUNCH();
     }
     if (plan.nsplit > 0) {

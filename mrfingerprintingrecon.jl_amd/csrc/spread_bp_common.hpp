@@ -1,7 +1,11 @@
 // Device pieces shared by the 2D and 3D output-owned backprojection gathers
 // (spread_bp.hip, spread_bp3.hip): the spreading kernel, the per-point
-// channel values and the DPP-broadcast multiply-adds.
+// channel values, the DPP-broadcast multiply-adds and the matrix-core gather.
 #pragma once
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
 
 #include "spread.hpp"
 
@@ -9,7 +13,7 @@ namespace msr {
 
 constexpr int ITEM_PAIRS = 1024;   // (block, point) pairs per work item
 constexpr size_t BP_VALS_BUDGET = (size_t)1 << 30;  // bytes of channel values per pass
-constexpr int BP_CGMAX = 16;       // largest channel group (2 CG value floats in two VGPRs)
+constexpr int BP_CGMAX = 16;       // largest channel group (32 value floats: one MFMA column block)
 constexpr int BP_DEPTH = 4;        // points whose loads are in flight per wavefront
 
 __device__ __forceinline__ float es_phi_f(float t) {
@@ -126,6 +130,188 @@ __device__ __forceinline__ void fma_bcast2(float (&acc)[2], float v, float w) {
         "v_fmac_f32_dpp %1, %2, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf"
         : "+v"(acc[0]), "+v"(acc[1])
         : "v"(v), "v"(w));
+}
+
+// ---------------------------------------------------------------- matrix cores
+// A 16-channel group of an output block of 64 cells is the product
+// S = V W over the block's points: V (32 x P) holds each point's 32 value
+// floats (16 complex channels), W (P x 64) its kernel weight at each cell.
+// v_mfma_f32_32x32x2f32 multiplies 32 x 2 by 2 x 32: lane l supplies row
+// (l & 31) of A for point (l >> 5) and column (l & 31) of B for the same point,
+// and holds S[i][l & 31] in register r for i = 8 (r / 4) + 4 (l >> 5) + r % 4.
+// So a wavefront takes two points per step, with two instructions: one for
+// cells 0..31 (acc0) and one for cells 32..63 (acc1), built in registers from
+// the pair's per-dimension weights (Cells::load and Cells::product).  Compared
+// with the broadcast FMAs above (2 CG FMAs per cell and point on the vector
+// ALU), the matrix cores do the same multiply-adds at twice the rate of
+// unpacked FP32 FMAs, and each lane loads one value float per step instead of
+// two value VGPRs per point.
+using f32x16 = float __attribute__((ext_vector_type(16)));
+
+// s_waitcnt immediate (gfx9 encoding) that waits for every vector memory load:
+// vmcnt 0, expcnt 7 and lgkmcnt 15 (no wait on those)
+constexpr int BP_VMCNT0 = 0x0F70;
+
+// channel of register pair (r, r + 1) (real, imaginary part) in lane half h
+__host__ __device__ constexpr int mfma_channel(int r, int h) {
+    return 4 * (r >> 2) + 2 * h + ((r & 3) >> 1);
+}
+
+// pair records past the last pair (zero weights, pair word 0) that the
+// gathers' prefetch may read
+constexpr int BP_PAD_PAIRS = 8 * BP_DEPTH;
+// groups of at most this many points address their values with 32-bit offsets
+constexpr int64_t BP_NARROW_POINTS = (int64_t)1 << 25;
+
+__device__ __forceinline__ float bp_ldf(const char* __restrict__ base, uint32_t off) {
+    return *reinterpret_cast<const float*>(base + (size_t)off);
+}
+__device__ __forceinline__ int bp_ldi(const char* __restrict__ base, uint32_t off) {
+    return *reinterpret_cast<const int*>(base + (size_t)off);
+}
+
+// Walks the pairs q0 .. q1 - 1 (q1 > q0) of a block, two per step (lane
+// half h = lane >> 5 takes pair q0 + 2 s + h), the loads of a step issued
+// BP_DEPTH steps ahead of its use and the pair word BP_DEPTH steps before that
+// (vector loads complete in order).  vals points at value float 0 of point 0
+// of the group (32 floats per point).  All addresses are a uniform base plus a
+// 32-bit lane offset (plus a constant), so a load costs no address arithmetic
+// beyond one add per iteration and stage group: the pair words and weights
+// are addressed from the item's first pair (the prefetch runs up to
+// BP_PAD_PAIRS past the item's last pair, into the next items or the zero
+// padding, and never uses what it reads there), the values from the group's
+// point 0 (WIDE: 64-bit offsets, for groups above 4 GiB).  In the last step
+// of an odd count the upper half's value is zero.  The sum of each cell runs
+// over the pairs in a fixed order.
+template <bool WIDE, class Cells>
+__device__ __forceinline__ void mfma_gather16(const float* __restrict__ vals,
+                                              const int* __restrict__ pairs,
+                                              const float* __restrict__ pw, int q0, int q1,
+                                              const Cells& cells, f32x16& acc0, f32x16& acc1) {
+    constexpr int D = BP_DEPTH;
+    constexpr uint32_t PSTEP = 2 * Cells::PW_BYTES;   // weight bytes per step
+    const int lane = (int)threadIdx.x;
+    const int h = lane >> 5;
+    const int n = q1 - q0;
+    __builtin_assume(n >= 1);
+    const int steps = (n + 1) >> 1;
+    const char* __restrict__ pwi =
+        reinterpret_cast<const char*>(pw) + (size_t)q0 * Cells::PW_BYTES;
+    const char* __restrict__ pqi = reinterpret_cast<const char*>(pairs + q0);
+    const char* __restrict__ vb = reinterpret_cast<const char*>(vals);
+    const uint32_t vlane = (uint32_t)(lane & 31) * 4;
+    int wd[D];                    // pair words of steps s + D .. s + 2D - 1
+    float av[D];                  // value floats of steps s .. s + D - 1
+    float w[D][Cells::NW];        // weights of steps s .. s + D - 1
+    auto value = [&](int word) -> float {
+        if constexpr (WIDE)
+            return vals[(size_t)(uint32_t)word * 32 + (lane & 31)];
+        else
+            return bp_ldf(vb, ((uint32_t)word << 7) + vlane);
+    };
+    // lane offsets: pair record of step 0, pair word of step 0
+    const uint32_t orec0 = (uint32_t)h * Cells::PW_BYTES;
+    const uint32_t oq0 = (uint32_t)h * 4;
+#pragma unroll
+    for (int i = 0; i < D; ++i) wd[i] = bp_ldi(pqi + 8 * i, oq0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        av[i] = value(wd[i]);
+        cells.load(pwi + i * PSTEP, orec0, w[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) wd[i] = bp_ldi(pqi + 8 * (D + i), oq0);
+    // The value loads of the first steps wait for their pair words, so they
+    // leave the prologue in another order than the loop issues its loads; with
+    // the prologue's loads complete, the waits at the loop head follow the
+    // loop's own order (loads of the next D - 1 steps stay in flight).
+    asm volatile("" ::: "memory");  // the prologue's loads stay above the wait
+    __builtin_amdgcn_s_waitcnt(BP_VMCNT0);
+    uint32_t orec = orec0;        // lane's record offset of step s (from pwi + D PSTEP)
+    uint32_t oq = oq0;            // lane's word offset of step s (from pqi + 16 D)
+    for (int s = 0; s < steps; s += D) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            if (s + i < steps) {
+                float b0, b1;
+                cells.product(w[i], b0, b1);
+                float v = av[i];
+                if ((n & 1) && s + i == steps - 1) v = h ? 0.f : v;
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(v, b0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(v, b1, acc1, 0, 0, 0);
+            }
+            av[i] = value(wd[i]);
+            cells.load(pwi + (D + i) * PSTEP, orec, w[i]);
+            // the next pair word of stage i goes into the register of the
+            // one just used (loaded any earlier, it takes another register and
+            // a copy at the loop's end that waits for every load in flight)
+            __builtin_amdgcn_sched_barrier(0);
+            wd[i] = bp_ldi(pqi + 8 * (2 * D + i), oq);
+        }
+        orec += D * PSTEP;
+        oq += 8 * D;
+    }
+}
+
+// ------------------------------------------------------------ dispatch order
+// A launch's workgroup i runs on XCD i mod BP_XCDS, and each XCD has its own
+// L2.  In block order the neighbouring blocks that read the same points'
+// values land on different XCDs, and the channel groups of a block (which
+// read the same pair words and weights) run a whole pass apart.  The gathers
+// therefore take their work from a dispatch table: the work items in Morton
+// order of their blocks, cut into BP_XCDS * BP_SEGMENTS segments of about
+// equal work (pairs + BP_ITEM_COST per item), segment s on XCD s mod BP_XCDS,
+// the channel groups of an item dispatched one after the other.  Each XCD
+// then works through compact regions of blocks.  The result does not depend
+// on the order (each item writes its own cells or partial block).
+constexpr int BP_XCDS = 8;
+constexpr int BP_SEGMENTS = 32;
+constexpr int BP_ITEM_COST = 16;
+
+// Morton key (bit interleave) of d block coordinates
+inline uint64_t bp_morton_key(const int* c, int d) {
+    uint64_t k = 0;
+    for (int b = 0; b < 21; ++b)
+        for (int a = 0; a < d; ++a) k |= (uint64_t)((c[a] >> b) & 1) << (b * d + a);
+    return k;
+}
+
+// order[x * len + p]: the item at dispatch position p on XCD x, -1 past the
+// XCD's last item; key[i]: the Morton key of item i's block
+inline std::vector<int> bp_dispatch_order(const std::vector<int4>& items,
+                                          const std::vector<uint64_t>& key, int64_t& len) {
+    const int64_t n = (int64_t)items.size();
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return key[a] < key[b]; });
+    double total = 0.0;
+    for (const int4& it : items) total += it.z - it.y + BP_ITEM_COST;
+    const int64_t nseg = std::max<int64_t>(1, std::min<int64_t>(BP_XCDS * BP_SEGMENTS, n));
+    std::vector<std::vector<int>> lists(BP_XCDS);
+    double acc = 0.0;
+    int64_t seg = 0;
+    for (const int i : idx) {
+        lists[seg % BP_XCDS].push_back(i);
+        acc += items[i].z - items[i].y + BP_ITEM_COST;
+        while (seg + 1 < nseg && acc >= total * (double)(seg + 1) / (double)nseg) ++seg;
+    }
+    len = 0;
+    for (const auto& l : lists) len = std::max<int64_t>(len, (int64_t)l.size());
+    std::vector<int> order((size_t)(BP_XCDS * std::max<int64_t>(len, 1)), -1);
+    for (int x = 0; x < BP_XCDS; ++x)
+        for (size_t p = 0; p < lists[x].size(); ++p) order[(size_t)x * len + p] = lists[x][p];
+    return order;
+}
+
+// the item (-1: a padding workgroup) and channel group (gl) of this workgroup
+// in a launch of BP_XCDS * len * ngp workgroups
+__device__ __forceinline__ int bp_item_of(const int* __restrict__ order, int64_t len, int ngp,
+                                          int& gl) {
+    const int64_t L = blockIdx.x;
+    const int64_t x = L % BP_XCDS, seq = L / BP_XCDS;
+    const int64_t p = seq / ngp;
+    gl = (int)(seq - p * ngp);
+    return order[x * len + p];
 }
 
 // channel group size: the fewest groups of at most BP_CGMAX channels, each
