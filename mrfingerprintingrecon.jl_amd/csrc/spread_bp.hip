@@ -500,7 +500,7 @@ void spread_bp(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src
     float2* vals = work + (size_t)plan.nslots * nch * BP_TB * BP_TB;
     for (int g0 = 0; g0 < ng; g0 += gp) {
         const int ngp = std::min(gp, ng - g0);
-        const int gr = (int)std::min<int64_t>(ceil_div(M, 256), 8192);
+        const int gr = bp_values_grid(M, CG);
 #define MSR_BP_VALUES(CGV)                                                                         \
     hipLaunchKernelGGL(bp_values_kernel<CGV>, dim3(gr, ngp), dim3(256), 0, st, src,                \
                        plan.jsel_s.as<int64_t>(), plan.kc_s.as<float2>(), nullptr,                 \

@@ -30,19 +30,28 @@ __device__ __forceinline__ float2 basis_val(const ValueSrc& s, int t, int k) {
 
 // vals[gl][s][j] = conj(U[t_s, k]) y[jsel(s), c] dcf(s) for channel
 // ch = (g0 + gl) CG + j = c K + k (zero beyond the last channel): the channel
-// values of sorted point s, CG per group; one thread per (point, group),
-// blockIdx.y = gl, each coil sample read once per group.
+// values of sorted point s, CG per group; blockIdx.y = gl.  For even CG, CG / 2
+// threads per point, each forming two channels and storing them as one 16-byte
+// word, so a wavefront's stores cover consecutive bytes; for CG = 1 one thread
+// per point.
+constexpr int bp_values_lanes(int CG) { return CG % 2 == 0 ? CG / 2 : 1; }
+
 template <int CG>
 __global__ void bp_values_kernel(ValueSrc src, const int64_t* __restrict__ jsel_s,
                                  const float2* __restrict__ kc_s, const float* __restrict__ r2_s,
                                  const int* __restrict__ perm,
                                  const int* __restrict__ frame_s, int64_t M, int nch, int g0,
                                  float2* __restrict__ vals) {
+    constexpr int LP = bp_values_lanes(CG);   // threads per point
+    constexpr int CL = CG / LP;               // channels per thread
     const int gl = blockIdx.y;
-    const int chb = (g0 + gl) * CG;
     const int K = src.K;
-    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < M;
-         s += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t total = M * LP;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = e / LP;
+        const int j0 = (int)(e - s * LP) * CL;
+        const int chb = (g0 + gl) * CG + j0;
         const int64_t js = jsel_s[s];
         const int t = frame_s[s];
         float dcf = 1.f;
@@ -59,12 +68,12 @@ __global__ void bp_values_kernel(ValueSrc src, const int64_t* __restrict__ jsel_
         } else if (src.dcf_mode == 3) {
             dcf = src.dcf[perm[s]];
         }
-        float2 out[CG];
+        float2 out[CL];
         int c = chb / K, k = chb - c * K;
         float2 y = make_float2(0.f, 0.f);
         int cy = -1;
 #pragma unroll
-        for (int j = 0; j < CG; ++j) {
+        for (int j = 0; j < CL; ++j) {
             out[j] = make_float2(0.f, 0.f);
             if (chb + j < nch) {
                 if (c != cy) {
@@ -78,16 +87,17 @@ __global__ void bp_values_kernel(ValueSrc src, const int64_t* __restrict__ jsel_
                 ++c;
             }
         }
-        if constexpr (CG % 2 == 0) {
-            float4* dst = reinterpret_cast<float4*>(vals + ((int64_t)gl * M + s) * CG);
-#pragma unroll
-            for (int j = 0; j < CG / 2; ++j)
-                dst[j] = make_float4(out[2 * j].x, out[2 * j].y, out[2 * j + 1].x, out[2 * j + 1].y);
-        } else {
-#pragma unroll
-            for (int j = 0; j < CG; ++j) vals[((int64_t)gl * M + s) * CG + j] = out[j];
-        }
+        float2* dst = vals + ((int64_t)gl * M + s) * CG + j0;
+        if constexpr (CL == 2)
+            *reinterpret_cast<float4*>(dst) = make_float4(out[0].x, out[0].y, out[1].x, out[1].y);
+        else
+            dst[0] = out[0];
     }
+}
+
+// grid of bp_values_kernel for M points (blockIdx.y: the groups)
+inline int bp_values_grid(int64_t M, int CG) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(M * bp_values_lanes(CG), 256), 32768));
 }
 
 // acc[J0 + j] += v_j w for j < 16, where v_j is lane j of the 16-lane row of v
@@ -259,10 +269,10 @@ __device__ __forceinline__ void mfma_gather16(const float* __restrict__ vals,
 // values land on different XCDs, and the channel groups of a block (which
 // read the same pair words and weights) run a whole pass apart.  The gathers
 // therefore take their work from a dispatch table: the work items in Morton
-// order of their blocks, cut into BP_XCDS * BP_SEGMENTS segments of about
-// equal work (pairs + BP_ITEM_COST per item), segment s on XCD s mod BP_XCDS,
-// the channel groups of an item dispatched one after the other.  Each XCD
-// then works through compact regions of blocks.  The result does not depend
+// order of their blocks (the long items first), cut into BP_XCDS * BP_SEGMENTS
+// segments of about equal work (pairs + BP_ITEM_COST per item), segment s on
+// XCD s mod BP_XCDS, the channel groups of an item dispatched one after the
+// other.  Each XCD then works through compact regions of blocks.  The result does not depend
 // on the order (each item writes its own cells or partial block).
 constexpr int BP_XCDS = 8;
 constexpr int BP_SEGMENTS = 32;
@@ -283,7 +293,13 @@ inline std::vector<int> bp_dispatch_order(const std::vector<int4>& items,
     const int64_t n = (int64_t)items.size();
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return key[a] < key[b]; });
+    // items of at least ITEM_PAIRS / 2 pairs (the dense centre of a radial
+    // trajectory) first, so that no long wavefront starts near the launch's end
+    auto heavy = [&](int a) { return items[a].z - items[a].y >= ITEM_PAIRS / 2; };
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+        if (heavy(a) != heavy(b)) return heavy(a);
+        return key[a] < key[b];
+    });
     double total = 0.0;
     for (const int4& it : items) total += it.z - it.y + BP_ITEM_COST;
     const int64_t nseg = std::max<int64_t>(1, std::min<int64_t>(BP_XCDS * BP_SEGMENTS, n));
