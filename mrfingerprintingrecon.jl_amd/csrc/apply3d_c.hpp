@@ -47,8 +47,9 @@ struct PassC3Shape {
     static constexpr int P = K * (K + 1) / 2;
 };
 
+// at most 128 VGPRs: four waves per SIMD, two 512-thread workgroups per CU
 template <int NZ2, int K, bool LREAL, class CT, class ST>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512, 4)
 pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restrict__ tw,
           int nx2, int ny2, int nz, int C, int Gb, const int* __restrict__ skip) {
     if (skip && *skip) return;
@@ -84,12 +85,109 @@ pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
         const int wc = v / NZ2, qz = v - wc * NZ2;
         const int qx = qx0 + wc;
         const bool ok = (v < NV) && (qx < nx2);
+        // branch-free, so the P loads issue together: an out-of-range voxel
+        // reads column 0 and keeps zero
+        const LT* src = lam + (ok ? ((int64_t)qy * nx2 + qx) * P * NZ2 + qz : 0);
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            if (ok) lr[e][p] = lam[(((int64_t)qy * nx2 + qx) * P + p) * NZ2 + qz];
-            else if constexpr (LREAL) lr[e][p] = 0.f;
-            else lr[e][p] = make_float2(0.f, 0.f);
+            const LT x = src[(int64_t)p * NZ2];
+            if constexpr (LREAL) lr[e][p] = ok ? x : 0.f;
+            else lr[e][p] = ok ? x : make_float2(0.f, 0.f);
         }
+    }
+    // per-voxel Hermitian (or real symmetric) K x K multiply of the lines in
+    // LDS, line (k, wc) at col_base(wc) + k * kstride; the kernel entry is
+    // promoted on use (opaque: the registers keep the Float32 table across
+    // coils instead of a Float64 copy)
+    auto multiply = [&](int kstride, auto col_base) {
+#pragma unroll
+        for (int e = 0; e < EV; ++e) {
+            const int v = tid + e * BLK;
+            if (v < NV) {
+                const int wc = v / NZ2, qz = v - wc * NZ2;
+                CT* col = col_base(wc) + lds_pad<CT>(qz);
+                CT in[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) in[k] = col[k * kstride];
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    CT acc = cz<CT>();
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const LT lv = lr[e][j <= k ? packed_index(j, k) : packed_index(k, j)];
+                        const LT lf = sizeof(CT) == 16 ? opaque(lv) : lv;
+                        if constexpr (LREAL) {
+                            const RT l = (RT)lf;
+                            acc.x = fma(l, in[k].x, acc.x);
+                            acc.y = fma(l, in[k].y, acc.y);
+                        } else {
+                            if (j <= k) cfma(acc, cto<CT>(lf), in[k]);
+                            else cfmac(acc, cto<CT>(lf), in[k]);
+                        }
+                    }
+                    col[j * kstride] = acc;
+                }
+            }
+        }
+    };
+
+    if constexpr (fft_plan_exact<NZ2>()) {
+        // Register form (exact plans): line (k, wc) = g = k WC + wc, its TP
+        // threads in one wavefront (adjacent columns share a wavefront, so its
+        // accesses pair up like the staged form's).  A thread loads its line's
+        // elements straight into the first stage's layout (the padding half
+        // the constant zero), the forward transform leaves the line in LDS, the
+        // multiply runs between two barriers, and the inverse transform leaves
+        // the cropped outputs in registers, stored from there: two workgroup
+        // barriers per coil, no LDS staging of the stored values.
+        constexpr int R0 = FFTPlan<NZ2>::R[0], NB0 = NZ2 / R0;
+        constexpr int RL = FFTPlan<NZ2>::R[FFTPlan<NZ2>::NST - 1], NBL = NZ2 / RL;
+        constexpr int NR = NZ2 / TP;
+        const int g = tid / TP, t = tid - g * TP;
+        const bool fft_thread = tid < FT;
+        const int gl = fft_thread ? g : 0;
+        const int kl = gl / WC, wcl = gl - kl * WC;
+        const bool col_ok = fft_thread && qx0 + wcl < nx2;
+        CT* line = s_buf + gl * LL;
+        // branch-free: a thread without a column reads column qx0 and keeps zero
+        const int64_t colofs = (int64_t)qy * nx2 + (col_ok ? qx0 + wcl : qx0);
+        __syncthreads();  // twiddles
+        for (int c = c0; c < c1; ++c) {
+            const int64_t lofs = ((int64_t)c * K + kl) * nz * plane + colofs;
+            if (fft_thread) {
+                CT xr[NR];
+#pragma unroll
+                for (int p = 0; p < NR / R0; ++p)
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) {
+                        const int i = t + p * TP + r * NB0;
+                        CT x = cz<CT>();
+                        if (R0 % 2 != 0 || r < R0 / 2) {  // i < NZ2 / 2 = nz for even R0
+                            const int ic = i < nz ? i : 0;
+                            const CT y = st_load<CT, ST>(T2 + lofs + (int64_t)ic * plane);
+                            x = (i < nz && col_ok) ? y : cz<CT>();
+                        }
+                        xr[p * R0 + r] = x;
+                    }
+                fft_wave<NZ2, -1, true, false>(line, s_tw, t, true, xr);
+            }
+            __syncthreads();
+            multiply(WC * LL, [&](int wc) { return s_buf + wc * LL; });
+            __syncthreads();
+            if (fft_thread) {
+                CT xr[NR];
+                fft_wave<NZ2, +1, false, true>(line, s_tw, t, true, xr);
+#pragma unroll
+                for (int p = 0; p < NBL / TP; ++p)
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) {
+                        const int i = t + p * TP + r * NBL;
+                        if ((RL % 2 != 0 || r < RL / 2) && i < nz && col_ok)
+                            st_store<CT, ST>(T2 + lofs + (int64_t)i * plane, xr[p * RL + r]);
+                    }
+            }
+        }
+        return;
     }
 
     const int g = tid / TP, t = tid - g * TP;
@@ -109,33 +207,7 @@ pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
         }
         __syncthreads();
         fft_lds<NZ2, -1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
-#pragma unroll
-        for (int e = 0; e < EV; ++e) {
-            const int v = tid + e * BLK;
-            if (v < NV) {
-                const int wc = v / NZ2, qz = v - wc * NZ2;
-                CT* col = s_buf + (size_t)wc * K * LL + lds_pad<CT>(qz);
-                CT in[K];
-#pragma unroll
-                for (int k = 0; k < K; ++k) in[k] = col[k * LL];
-#pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    CT acc = cz<CT>();
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        if constexpr (LREAL) {
-                            const RT l = (RT)lr[e][j <= k ? packed_index(j, k) : packed_index(k, j)];
-                            acc.x = fma(l, in[k].x, acc.x);
-                            acc.y = fma(l, in[k].y, acc.y);
-                        } else {
-                            if (j <= k) cfma(acc, cto<CT>(lr[e][packed_index(j, k)]), in[k]);
-                            else cfmac(acc, cto<CT>(lr[e][packed_index(k, j)]), in[k]);
-                        }
-                    }
-                    col[j * LL] = acc;
-                }
-            }
-        }
+        multiply(LL, [&](int wc) { return s_buf + (size_t)wc * K * LL; });
         __syncthreads();
         fft_lds<NZ2, +1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
         for (int el = tid; el < WC * K * NZ2; el += BLK) {
