@@ -42,11 +42,14 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
     constexpr int BLK = rows_block<NF, CT>();
     constexpr int F = BLK / TP;
     constexpr int LD = lds_line<NF, CT>() + 1;
-    __shared__ CT s_tw[NF];
+    // ComplexF64 twiddles from the global table (L1 / L2), ComplexF32 ones
+    // from an LDS copy: the smaller LDS footprint admits more workgroups per CU
+    __shared__ CT s_tw_lds[(sizeof(CT) == 16) ? 1 : NF];
+    const CT* s_tw = (sizeof(CT) == 16) ? tw : s_tw_lds;
     __shared__ CT s[F * LD];
     const CT* in = static_cast<const CT*>(p.in);
     CT* out = static_cast<CT*>(p.out);
-    load_twiddles<NF>(s_tw, tw);
+    if constexpr (!(sizeof(CT) == 16)) load_twiddles<NF>(s_tw_lds, tw);
     const int64_t l0 = (int64_t)blockIdx.x * F;
     if (p.isel == SEL_FIRST && !p.win && !p.win64) {
         for (int e = threadIdx.x; e < F * NF; e += BLK) {
@@ -101,7 +104,10 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
     constexpr int BLK = (WC * TP) < 256 ? WC * TP : 256;
     constexpr int F = BLK / TP;
     constexpr int LD = lds_line<NF, CT>() + 1;
-    __shared__ CT s_tw[NF];
+    // ComplexF64 twiddles from the global table (L1 / L2), ComplexF32 ones
+    // from an LDS copy: the smaller LDS footprint admits more workgroups per CU
+    __shared__ CT s_tw_lds[(sizeof(CT) == 16) ? 1 : NF];
+    const CT* s_tw = (sizeof(CT) == 16) ? tw : s_tw_lds;
     __shared__ CT s[WC * LD];
     const CT* in = static_cast<const CT*>(p.in);
     CT* out = static_cast<CT*>(p.out);
@@ -119,7 +125,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
             }
         out[idx] = v;
     };
-    load_twiddles<NF>(s_tw, tw);
+    if constexpr (!(sizeof(CT) == 16)) load_twiddles<NF>(s_tw_lds, tw);
     const int64_t col0 = (int64_t)blockIdx.x * WC;
     const int64_t b = blockIdx.y;
     if (p.isel == SEL_FIRST && !p.win && !p.win64) {
