@@ -121,7 +121,7 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
                 v = bk[(size_t)y * nx + xx];
                 if (cm) v = cmul(cto<CT>(cm[(size_t)y * nx + xx]), v);
             }
-            s_rows[yl * LD + lds_pad<CT>(xx)] = v;
+            s_rows[yl * LD + lds_pad<NX2, CT>(xx)] = v;
         }
         __syncthreads();
 #pragma unroll
@@ -132,7 +132,7 @@ __device__ __forceinline__ void pass_a_body(const Apply2DArgs& a, int64_t lb) {
     for (int e = tid; e < RT * NX2; e += BLK) {
         const int yl = e % RT, qx = e / RT;
         const int y = y0 + yl;
-        if (y < ny) st_store<CT, ST>(Tk + (size_t)qx * ny + y, s_rows[yl * LD + lds_pad<CT>(qx)]);
+        if (y < ny) st_store<CT, ST>(Tk + (size_t)qx * ny + y, s_rows[yl * LD + lds_pad<NX2, CT>(qx)]);
     }
 }
 
@@ -212,7 +212,7 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
 #pragma unroll
         for (int r = 0; r < NLE; ++r) {
             const int e = tid + r * BLK;
-            if (e < RT * NX2) s_rows[(e % RT) * LD + lds_pad<CT>(e / RT)] = st_conv<CT, ST>(pre[r]);
+            if (e < RT * NX2) s_rows[(e % RT) * LD + lds_pad<NX2, CT>(e / RT)] = st_conv<CT, ST>(pre[r]);
         }
         __syncthreads();
         if (c + 1 < c1) prefetch(c + 1);  // in flight during this coil's transforms
@@ -247,7 +247,7 @@ __device__ __forceinline__ void pass_c_body(const Apply2DArgs& a, int64_t lb) {
                     const int yl = e / NXC, xx = e - yl * NXC;
                     const int y = y0 + yl;
                     if (y < ny) {
-                        CT v = s_rows[yl * LD + lds_pad<CT>(xx)];
+                        CT v = s_rows[yl * LD + lds_pad<NX2, CT>(xx)];
                         if (cm) cfmac(acc[i], cto<CT>(cm[(size_t)y * nx + xx]), v);
                         else acc[i] = cadd(acc[i], v);
                     }

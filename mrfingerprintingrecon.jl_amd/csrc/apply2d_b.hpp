@@ -167,7 +167,7 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
             const int v = tid + e * BLK;
             if (v < NV) {
                 const int wc = v / NY2, qy = v - wc * NY2;
-                CT* col = s_buf + (size_t)wc * K * LL + lds_pad<CT>(qy);
+                CT* col = s_buf + (size_t)wc * K * LL + lds_pad<NY2, CT>(qy);
                 CT in[K], acc[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -209,7 +209,7 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
 #pragma unroll
                 for (int m = 0; m < NR / 2; ++m) {
                     const int i = t + m * TP;
-                    if (col_ok) st_store<CT, ST>(dst + i, line[lds_pad<CT>(i)]);
+                    if (col_ok) st_store<CT, ST>(dst + i, line[lds_pad<NY2, CT>(i)]);
                 }
             }
         }

@@ -44,7 +44,7 @@ pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, ST* __rest
             v = bk[row * nx + xx];
             if (cm) v = cmul(cto<CT>(cm[row * nx + xx]), v);
         }
-        s[l * LD + lds_pad<CT>(xx)] = v;
+        s[l * LD + lds_pad<NX2, CT>(xx)] = v;
     }
     __syncthreads();
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
@@ -53,7 +53,7 @@ pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, ST* __rest
     for (int e = threadIdx.x; e < F * NX2; e += BLK) {
         const int l = e / NX2, q = e - l * NX2;
         const int64_t row = r0 + l;
-        if (row < nrows) st_store<CT, ST>(out + row * NX2 + q, s[l * LD + lds_pad<CT>(q)]);
+        if (row < nrows) st_store<CT, ST>(out + row * NX2 + q, s[l * LD + lds_pad<NX2, CT>(q)]);
     }
 }
 
@@ -85,7 +85,7 @@ pass_e_3d(const ST* __restrict__ T1, const float2* __restrict__ cmaps,
         for (int e = threadIdx.x; e < F * NX2; e += BLK) {
             const int l = e / NX2, q = e - l * NX2;
             const int64_t row = r0 + l;
-            s[l * LD + lds_pad<CT>(q)] = row < nrows ? st_load<CT, ST>(in + row * NX2 + q) : cz<CT>();
+            s[l * LD + lds_pad<NX2, CT>(q)] = row < nrows ? st_load<CT, ST>(in + row * NX2 + q) : cz<CT>();
         }
         __syncthreads();
         fft_lds<NX2, +1>(s + g * LD, s_tw, t);
@@ -97,7 +97,7 @@ pass_e_3d(const ST* __restrict__ T1, const float2* __restrict__ cmaps,
                 const int l = e / NXC, xx = e - l * NXC;
                 const int64_t row = r0 + l;
                 if (row < nrows) {
-                    const CT v = s[l * LD + lds_pad<CT>(xx)];
+                    const CT v = s[l * LD + lds_pad<NX2, CT>(xx)];
                     if (cm) cfmac(acc[i], cto<CT>(cm[row * nx + xx]), v);
                     else acc[i] = cadd(acc[i], v);
                 }

@@ -105,7 +105,7 @@ pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
             const int v = tid + e * BLK;
             if (v < NV) {
                 const int wc = v / NZ2, qz = v - wc * NZ2;
-                CT* col = col_base(wc) + lds_pad<CT>(qz);
+                CT* col = col_base(wc) + lds_pad<NZ2, CT>(qz);
                 CT in[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) in[k] = col[k * kstride];
@@ -203,7 +203,7 @@ pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
             if (z < nz && qx < nx2)
                 v = st_load<CT, ST>(T2 + (((int64_t)c * K + k) * nz + z) * plane +
                                     (int64_t)qy * nx2 + qx);
-            s_buf[(wc * K + k) * LL + lds_pad<CT>(z)] = v;
+            s_buf[(wc * K + k) * LL + lds_pad<NZ2, CT>(z)] = v;
         }
         __syncthreads();
         fft_lds<NZ2, -1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
@@ -217,7 +217,7 @@ pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
             const int qx = qx0 + wc;
             if (z < nz && qx < nx2)
                 st_store<CT, ST>(T2 + (((int64_t)c * K + k) * nz + z) * plane + (int64_t)qy * nx2 + qx,
-                                 s_buf[(wc * K + k) * LL + lds_pad<CT>(z)]);
+                                 s_buf[(wc * K + k) * LL + lds_pad<NZ2, CT>(z)]);
         }
         __syncthreads();
     }

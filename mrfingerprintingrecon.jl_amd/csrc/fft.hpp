@@ -338,21 +338,37 @@ template <int DIR> struct Butterfly<5, DIR> {
 };
 
 // ----------------------------------------------------------- LDS layout
-// Element i of a line lives at lds_pad<CT>(i): one complex of padding after
-// every 8 (ComplexF64) or 32 (ComplexF32) elements, so the stride-R stores of the
-// first Stockham stages and the strided reads of the transposing passes spread
-// over the LDS banks.
-// The padding period depends on the element size, measured on MI355X at
-// 256^2 / 20 coils / K = 5: every 8 elements for 16-byte ComplexF64 (periods
-// of 2, 4, 16 and 32 took the apply from 0.306 to 0.37, 0.33, 0.310 and 0.315
-// ms), every 32 for 8-byte ComplexF32 (0.217 ms with a period of 8, 0.193 with
-// 16, 0.190 with 32).
+// Element i of an N-point line lives at lds_pad<N, CT>(i).  The LDS serves a
+// wavefront's 16-byte accesses 16 lanes at a time over 16 slots of 16 bytes (a
+// probe on MI355X: 16 lanes on 16 distinct slots take the minimum time, two
+// lanes per slot twice it), and the transforms access a line in three
+// patterns: stride R (the first stage's stores), the blocks of the middle
+// stages' stores, and unit stride (every stage's loads, the last stage's
+// stores, the kernel multiply and the passes' row copies).
+// * ComplexF64 lines of plans with at least 16 threads per line: element i
+//   at i ^ ((i >> 3) & 15), a permutation inside each aligned block of 16, so
+//   every one of those patterns of a radix-8 plan meets 16 distinct slots (one
+//   complex of padding every 8 elements left the unit-stride accesses at two
+//   lanes per slot: measured 38 against 71 probe cycles per wavefront access);
+//   modelled over every plan's stages, no worse than the padding for any plan
+//   with 16 or more threads per line and better for all of 128 points or more.
+// * other ComplexF64 lines: one complex of padding after every 8 elements;
+//   ComplexF32: after every 32 (at 256^2 / 20 coils / K = 5, a period of 8 took
+//   the ComplexF32 apply to 0.217 ms, 16 to 0.193, 32 to 0.190).
 template <class CT>
 constexpr int pad_shift() { return sizeof(CT) == 16 ? 3 : 5; }
-template <class CT>
-__host__ __device__ constexpr int lds_pad(int i) { return i + (i >> pad_shift<CT>()); }
 template <int N, class CT>
-constexpr int lds_line() { return N + (N >> pad_shift<CT>()); }
+constexpr bool lds_swizzled() { return sizeof(CT) == 16 && FFTPlan<N>::T >= 16; }
+template <int N, class CT>
+__host__ __device__ constexpr int lds_pad(int i) {
+    if constexpr (lds_swizzled<N, CT>()) return i ^ ((i >> 3) & 15);
+    else return i + (i >> pad_shift<CT>());
+}
+template <int N, class CT>
+constexpr int lds_line() {
+    if constexpr (lds_swizzled<N, CT>()) return (N + 15) & ~15;
+    else return N + (N >> pad_shift<CT>());
+}
 
 // ----------------------------------------------------------- stages
 // The T threads of a line always lie in one wavefront (every plan's T divides
@@ -393,7 +409,7 @@ __device__ __forceinline__ void fft_stage(CT* buf, const CT* tw, int t, bool act
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if constexpr (IN_REG) v[p][r] = reg[p * R + r];
-                else v[p][r] = buf[lds_pad<CT>(j + r * NB)];
+                else v[p][r] = buf[lds_pad<N, CT>(j + r * NB)];
             }
         }
     }
@@ -438,7 +454,7 @@ __device__ __forceinline__ void fft_stage(CT* buf, const CT* tw, int t, bool act
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if constexpr (OUT_REG) reg[p * R + r] = v[p][r];
-                else buf[lds_pad<CT>(base + r * NS)] = v[p][r];
+                else buf[lds_pad<N, CT>(base + r * NS)] = v[p][r];
             }
         }
     }
@@ -456,7 +472,7 @@ __device__ __forceinline__ void fft_stages(CT* buf, const CT* tw, int t, bool ac
     }
 }
 
-// In-place N-point FFT of a line (element i at buf[lds_pad<CT>(i)]) by the
+// In-place N-point FFT of a line (element i at buf[lds_pad<N, CT>(i)]) by the
 // T = FFTPlan<N>::T threads with local index t.  Requires a barrier before the
 // call (data in LDS); ends with a barrier.  All threads of the workgroup must
 // call it; threads with active = false only take part in the final barrier

@@ -57,10 +57,10 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
             const int64_t line = l0 + l;
             CT v = cz<CT>();
             if (line < p.nlines && i < p.lin) v = in[line * p.in_pitch + i];
-            s[l * LD + lds_pad<CT>(i)] = v;
+            s[l * LD + lds_pad<NF, CT>(i)] = v;
         }
     } else {
-        for (int e = threadIdx.x; e < F * NF; e += BLK) s[(e / NF) * LD + lds_pad<CT>(e % NF)] = cz<CT>();
+        for (int e = threadIdx.x; e < F * NF; e += BLK) s[(e / NF) * LD + lds_pad<NF, CT>(e % NF)] = cz<CT>();
         __syncthreads();
         for (int e = threadIdx.x; e < F * p.lin; e += BLK) {
             const int l = e / p.lin, i = e - l * p.lin;
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
                 CT v = in[line * p.in_pitch + i];
                 if (p.win64) v = cscale(v, (RT)p.win64[i]);
                 else if (p.win) v = cscale(v, (RT)p.win[i]);
-                s[l * LD + lds_pad<CT>(select_bin(i, p.lin, NF, p.isel))] = v;
+                s[l * LD + lds_pad<NF, CT>(select_bin(i, p.lin, NF, p.isel))] = v;
             }
         }
     }
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
         const int l = e / p.lout, j = e - l * p.lout;
         const int64_t line = l0 + l;
         if (line >= p.nlines) continue;
-        CT v = s[l * LD + lds_pad<CT>(select_bin(j, p.lout, NF, p.sel))];
+        CT v = s[l * LD + lds_pad<NF, CT>(select_bin(j, p.lout, NF, p.sel))];
         if (p.w64) v = cscale(v, (RT)p.w64[j]);
         else if (p.w) v = cscale(v, (RT)p.w[j]);
         out[line * p.out_pitch + j] = v;
@@ -134,10 +134,10 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
             const int64_t col = col0 + wc;
             CT v = cz<CT>();
             if (col < p.ncols && i < p.lin) v = load_in(b * p.in_bstride + i * p.in_stride + col);
-            s[wc * LD + lds_pad<CT>(i)] = v;
+            s[wc * LD + lds_pad<NF, CT>(i)] = v;
         }
     } else {
-        for (int e = threadIdx.x; e < WC * NF; e += BLK) s[(e % WC) * LD + lds_pad<CT>(e / WC)] = cz<CT>();
+        for (int e = threadIdx.x; e < WC * NF; e += BLK) s[(e % WC) * LD + lds_pad<NF, CT>(e / WC)] = cz<CT>();
         __syncthreads();
         for (int e = threadIdx.x; e < WC * p.lin; e += BLK) {
             const int wc = e % WC, i = e / WC;
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
                 CT v = load_in(b * p.in_bstride + i * p.in_stride + col);
                 if (p.win64) v = cscale(v, (RT)p.win64[i]);
                 else if (p.win) v = cscale(v, (RT)p.win[i]);
-                s[wc * LD + lds_pad<CT>(select_bin(i, p.lin, NF, p.isel))] = v;
+                s[wc * LD + lds_pad<NF, CT>(select_bin(i, p.lin, NF, p.isel))] = v;
             }
         }
     }
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
         const int wc = e % WC, j = e / WC;
         const int64_t col = col0 + wc;
         if (col >= p.ncols) continue;
-        CT v = s[wc * LD + lds_pad<CT>(select_bin(j, p.lout, NF, p.sel))];
+        CT v = s[wc * LD + lds_pad<NF, CT>(select_bin(j, p.lout, NF, p.sel))];
         if (p.w64) v = cscale(v, (RT)p.w64[j]);
         else if (p.w) v = cscale(v, (RT)p.w[j]);
         if (!p.lam) {
