@@ -81,7 +81,10 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
     using RT = real_t<CT>;
     constexpr int TP = S::TP, WC = S::WC, BLK = S::BLK, NV = S::NV, EV = S::EV, P = S::P;
     constexpr int FT = S::FT;
-    constexpr int LL = lds_line<NY2, CT>();
+    // the swizzled LDS layout up to K = 5 (above, its address arithmetic
+    // makes the pass spill)
+    constexpr bool SWZ = lds_swizzled<NY2, CT>() && K <= 5;
+    constexpr int LL = lds_line<NY2, CT, SWZ>();
     // ComplexF64 twiddles are read from the global table (L1 / L2), which frees
     // LDS for one more workgroup per CU; ComplexF32 ones from an LDS copy
     __shared__ CT s_tw_lds[sizeof(CT) == 16 ? 1 : NY2];
@@ -159,7 +162,7 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
 #pragma unroll
         for (int e = 0; e < NR; ++e) xr[e] = pre[e];
         if (PREFETCH && c + 1 < c1) prefetch(c + 1);  // in flight during this coil's transforms
-        if (fft_thread) fft_wave<NY2, -1, true, false>(line, s_tw, t, true, xr);
+        if (fft_thread) fft_wave<NY2, -1, true, false, CT, SWZ>(line, s_tw, t, true, xr);
         __syncthreads();
         // per-voxel Hermitian (or real symmetric) K x K multiply
 #pragma unroll
@@ -167,7 +170,7 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
             const int v = tid + e * BLK;
             if (v < NV) {
                 const int wc = v / NY2, qy = v - wc * NY2;
-                CT* col = s_buf + (size_t)wc * K * LL + lds_pad<NY2, CT>(qy);
+                CT* col = s_buf + (size_t)wc * K * LL + lds_pad<NY2, CT, SWZ>(qy);
                 CT in[K], acc[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -196,7 +199,7 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
                 // storage only: at 256^2 / 20 coils / K = 5 on MI355X it took the
                 // apply from 0.219 to 0.212 ms (ComplexF32) and 0.289 to 0.277 ms
                 // (compact), but ComplexF64 storage from 0.302 to 0.325 ms
-                fft_wave<NY2, +1, false, true>(line, s_tw, t, true, xr);
+                fft_wave<NY2, +1, false, true, CT, SWZ>(line, s_tw, t, true, xr);
 #pragma unroll
                 for (int p = 0; p < NBL / TP; ++p)
 #pragma unroll
@@ -205,11 +208,11 @@ __device__ __forceinline__ void pass_b_body(const Apply2DArgs& a, int64_t lb) {
             } else {
                 // the last stage writes the line to LDS; the crop stores
                 // elements i < ny, lanes on consecutive i
-                fft_wave<NY2, +1, false, false>(line, s_tw, t, true, xr);
+                fft_wave<NY2, +1, false, false, CT, SWZ>(line, s_tw, t, true, xr);
 #pragma unroll
                 for (int m = 0; m < NR / 2; ++m) {
                     const int i = t + m * TP;
-                    if (col_ok) st_store<CT, ST>(dst + i, line[lds_pad<NY2, CT>(i)]);
+                    if (col_ok) st_store<CT, ST>(dst + i, line[lds_pad<NY2, CT, SWZ>(i)]);
                 }
             }
         }

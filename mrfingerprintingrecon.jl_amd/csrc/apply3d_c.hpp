@@ -47,9 +47,17 @@ struct PassC3Shape {
     static constexpr int P = K * (K + 1) / 2;
 };
 
-// at most 128 VGPRs: four waves per SIMD, two 512-thread workgroups per CU
+// The register form (below) for exact plans of at most 256 points whose
+// threads hold at most 8 values, and K <= 4: bounded to 128 VGPRs (four waves
+// per SIMD, two 512-thread workgroups per CU), it spills with more values per
+// thread or a larger kernel table.
+template <int NZ2, int K>
+constexpr bool pc3_register_form() {
+    return fft_plan_exact<NZ2>() && NZ2 <= 256 && NZ2 / FFTPlan<NZ2>::T <= 8 && K <= 4;
+}
+
 template <int NZ2, int K, bool LREAL, class CT, class ST>
-__global__ void __launch_bounds__(512, 4)
+__global__ void __launch_bounds__(512, (pc3_register_form<NZ2, K>() ? 4 : 2))
 pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restrict__ tw,
           int nx2, int ny2, int nz, int C, int Gb, const int* __restrict__ skip) {
     if (skip && *skip) return;
@@ -85,20 +93,29 @@ pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
         const int wc = v / NZ2, qz = v - wc * NZ2;
         const int qx = qx0 + wc;
         const bool ok = (v < NV) && (qx < nx2);
-        // branch-free, so the P loads issue together: an out-of-range voxel
-        // reads column 0 and keeps zero
-        const LT* src = lam + (ok ? ((int64_t)qy * nx2 + qx) * P * NZ2 + qz : 0);
+        if constexpr (pc3_register_form<NZ2, K>()) {
+            // branch-free, so the P loads issue together: an out-of-range
+            // voxel reads column 0 and keeps zero
+            const LT* src = lam + (ok ? ((int64_t)qy * nx2 + qx) * P * NZ2 + qz : 0);
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const LT x = src[(int64_t)p * NZ2];
-            if constexpr (LREAL) lr[e][p] = ok ? x : 0.f;
-            else lr[e][p] = ok ? x : make_float2(0.f, 0.f);
+            for (int p = 0; p < P; ++p) {
+                const LT x = src[(int64_t)p * NZ2];
+                if constexpr (LREAL) lr[e][p] = ok ? x : 0.f;
+                else lr[e][p] = ok ? x : make_float2(0.f, 0.f);
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                if (ok) lr[e][p] = lam[(((int64_t)qy * nx2 + qx) * P + p) * NZ2 + qz];
+                else if constexpr (LREAL) lr[e][p] = 0.f;
+                else lr[e][p] = make_float2(0.f, 0.f);
+            }
         }
     }
-    // per-voxel Hermitian (or real symmetric) K x K multiply of the lines in
-    // LDS, line (k, wc) at col_base(wc) + k * kstride; the kernel entry is
-    // promoted on use (opaque: the registers keep the Float32 table across
-    // coils instead of a Float64 copy)
+    // register form: per-voxel Hermitian (or real symmetric) K x K multiply of
+    // the lines in LDS, line (k, wc) at col_base(wc) + k * kstride; the kernel
+    // entry is promoted on use (opaque: the registers keep the Float32 table
+    // across coils instead of a Float64 copy)
     auto multiply = [&](int kstride, auto col_base) {
 #pragma unroll
         for (int e = 0; e < EV; ++e) {
@@ -131,7 +148,7 @@ pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
         }
     };
 
-    if constexpr (fft_plan_exact<NZ2>()) {
+    if constexpr (pc3_register_form<NZ2, K>()) {
         // Register form (exact plans): line (k, wc) = g = k WC + wc, its TP
         // threads in one wavefront (adjacent columns share a wavefront, so its
         // accesses pair up like the staged form's).  A thread loads its line's
@@ -207,7 +224,33 @@ pass_c_3d(ST* __restrict__ T2, const void* __restrict__ lam_v, const CT* __restr
         }
         __syncthreads();
         fft_lds<NZ2, -1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
-        multiply(LL, [&](int wc) { return s_buf + (size_t)wc * K * LL; });
+#pragma unroll
+        for (int e = 0; e < EV; ++e) {
+            const int v = tid + e * BLK;
+            if (v < NV) {
+                const int wc = v / NZ2, qz = v - wc * NZ2;
+                CT* col = s_buf + (size_t)wc * K * LL + lds_pad<NZ2, CT>(qz);
+                CT in[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) in[k] = col[k * LL];
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    CT acc = cz<CT>();
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        if constexpr (LREAL) {
+                            const RT l = (RT)lr[e][j <= k ? packed_index(j, k) : packed_index(k, j)];
+                            acc.x = fma(l, in[k].x, acc.x);
+                            acc.y = fma(l, in[k].y, acc.y);
+                        } else {
+                            if (j <= k) cfma(acc, cto<CT>(lr[e][packed_index(j, k)]), in[k]);
+                            else cfmac(acc, cto<CT>(lr[e][packed_index(k, j)]), in[k]);
+                        }
+                    }
+                    col[j * LL] = acc;
+                }
+            }
+        }
         __syncthreads();
         fft_lds<NZ2, +1>(s_buf + (fft_thread ? g : 0) * LL, s_tw, t, fft_thread);
         for (int el = tid; el < WC * K * NZ2; el += BLK) {

@@ -345,28 +345,35 @@ template <int DIR> struct Butterfly<5, DIR> {
 // patterns: stride R (the first stage's stores), the blocks of the middle
 // stages' stores, and unit stride (every stage's loads, the last stage's
 // stores, the kernel multiply and the passes' row copies).
-// * ComplexF64 lines of plans with at least 16 threads per line: element i
-//   at i ^ ((i >> 3) & 15), a permutation inside each aligned block of 16, so
+// * ComplexF64 lines of plans with at least 16 threads per line, each thread
+//   holding at most 8 values (128, 256, 512 points): element i at
+//   i ^ ((i >> 3) & 15), a permutation inside each aligned block of 16, so
 //   every one of those patterns of a radix-8 plan meets 16 distinct slots (one
 //   complex of padding every 8 elements left the unit-stride accesses at two
-//   lanes per slot: measured 38 against 71 probe cycles per wavefront access);
-//   modelled over every plan's stages, no worse than the padding for any plan
-//   with 16 or more threads per line and better for all of 128 points or more.
+//   lanes per slot: measured 38 against 71 probe cycles per wavefront access).
+//   Modelled over every plan's stages it would also suit the other plans with
+//   16 or more threads per line, but where a thread holds more values the
+//   swizzled addresses raise register use until kernels spill (the 320- and
+//   384-point z-passes of the 3D apply), so those keep the padding.
 // * other ComplexF64 lines: one complex of padding after every 8 elements;
 //   ComplexF32: after every 32 (at 256^2 / 20 coils / K = 5, a period of 8 took
 //   the ComplexF32 apply to 0.217 ms, 16 to 0.193, 32 to 0.190).
 template <class CT>
 constexpr int pad_shift() { return sizeof(CT) == 16 ? 3 : 5; }
 template <int N, class CT>
-constexpr bool lds_swizzled() { return sizeof(CT) == 16 && FFTPlan<N>::T >= 16; }
-template <int N, class CT>
+constexpr bool lds_swizzled() {
+    return sizeof(CT) == 16 && FFTPlan<N>::T >= 16 && N / FFTPlan<N>::T <= 8;
+}
+// (SWZ: a kernel may keep the padding where the swizzle's address arithmetic
+// would make it spill; every access to a line must use the same choice)
+template <int N, class CT, bool SWZ = lds_swizzled<N, CT>()>
 __host__ __device__ constexpr int lds_pad(int i) {
-    if constexpr (lds_swizzled<N, CT>()) return i ^ ((i >> 3) & 15);
+    if constexpr (SWZ) return i ^ ((i >> 3) & 15);
     else return i + (i >> pad_shift<CT>());
 }
-template <int N, class CT>
+template <int N, class CT, bool SWZ = lds_swizzled<N, CT>()>
 constexpr int lds_line() {
-    if constexpr (lds_swizzled<N, CT>()) return (N + 15) & ~15;
+    if constexpr (SWZ) return (N + 15) & ~15;
     else return N + (N >> pad_shift<CT>());
 }
 
@@ -394,7 +401,7 @@ constexpr bool fft_plan_exact() {
 // One stage.  MODE bit 0: the inputs come from reg (reg[p R + r] = x[j + r N/R]
 // for j = t + p T) instead of LDS; bit 1: the outputs go to reg (for the last
 // stage reg[p R + r] = X[j + r N/R]) instead of LDS.
-template <int N, int R, int NS, int T, int DIR, int MODE, class CT>
+template <int N, int R, int NS, int T, int DIR, int MODE, class CT, bool SWZ>
 __device__ __forceinline__ void fft_stage(CT* buf, const CT* tw, int t, bool active, CT* reg) {
     constexpr int NB = N / R;
     constexpr int PER = (NB + T - 1) / T;
@@ -409,7 +416,7 @@ __device__ __forceinline__ void fft_stage(CT* buf, const CT* tw, int t, bool act
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if constexpr (IN_REG) v[p][r] = reg[p * R + r];
-                else v[p][r] = buf[lds_pad<N, CT>(j + r * NB)];
+                else v[p][r] = buf[lds_pad<N, CT, SWZ>(j + r * NB)];
             }
         }
     }
@@ -454,21 +461,21 @@ __device__ __forceinline__ void fft_stage(CT* buf, const CT* tw, int t, bool act
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if constexpr (OUT_REG) reg[p * R + r] = v[p][r];
-                else buf[lds_pad<N, CT>(base + r * NS)] = v[p][r];
+                else buf[lds_pad<N, CT, SWZ>(base + r * NS)] = v[p][r];
             }
         }
     }
     if constexpr (!OUT_REG) wave_lds_sync();  // visible to the next stage's reads
 }
 
-template <int N, int T, int DIR, int S, int NS, int MODE, class CT>
+template <int N, int T, int DIR, int S, int NS, int MODE, class CT, bool SWZ>
 __device__ __forceinline__ void fft_stages(CT* buf, const CT* tw, int t, bool active, CT* reg) {
     constexpr int NST = FFTPlan<N>::NST;
     if constexpr (S < NST) {
         constexpr int R = FFTPlan<N>::R[S];
         constexpr int M = (S == 0 ? (MODE & 1) : 0) | (S == NST - 1 ? (MODE & 2) : 0);
-        fft_stage<N, R, NS, T, DIR, M, CT>(buf, tw, t, active, reg);
-        fft_stages<N, T, DIR, S + 1, NS * R, MODE, CT>(buf, tw, t, active, reg);
+        fft_stage<N, R, NS, T, DIR, M, CT, SWZ>(buf, tw, t, active, reg);
+        fft_stages<N, T, DIR, S + 1, NS * R, MODE, CT, SWZ>(buf, tw, t, active, reg);
     }
 }
 
@@ -477,9 +484,9 @@ __device__ __forceinline__ void fft_stages(CT* buf, const CT* tw, int t, bool ac
 // call (data in LDS); ends with a barrier.  All threads of the workgroup must
 // call it; threads with active = false only take part in the final barrier
 // (buf is not touched), for workgroups with more threads than line work.
-template <int N, int DIR, class CT>
+template <int N, int DIR, class CT, bool SWZ = lds_swizzled<N, CT>()>
 __device__ __forceinline__ void fft_lds(CT* buf, const CT* tw, int t, bool active = true) {
-    fft_stages<N, FFTPlan<N>::T, DIR, 0, 1, 0, CT>(buf, tw, t, active, nullptr);
+    fft_stages<N, FFTPlan<N>::T, DIR, 0, 1, 0, CT, SWZ>(buf, tw, t, active, nullptr);
     __syncthreads();
 }
 
@@ -490,10 +497,10 @@ __device__ __forceinline__ void fft_lds(CT* buf, const CT* tw, int t, bool activ
 // stage's layout, reg[p RL + r] = X[t + p T + r N/RL] (otherwise in buf in
 // natural order, visible to this wavefront; other wavefronts need a barrier).
 // reg holds N / T values.
-template <int N, int DIR, bool IN_REG, bool OUT_REG, class CT>
+template <int N, int DIR, bool IN_REG, bool OUT_REG, class CT, bool SWZ = lds_swizzled<N, CT>()>
 __device__ __forceinline__ void fft_wave(CT* buf, const CT* tw, int t, bool active, CT* reg) {
     static_assert(fft_plan_exact<N>(), "fft_wave needs an exact plan");
-    fft_stages<N, FFTPlan<N>::T, DIR, 0, 1, (IN_REG ? 1 : 0) | (OUT_REG ? 2 : 0), CT>(buf, tw, t, active, reg);
+    fft_stages<N, FFTPlan<N>::T, DIR, 0, 1, (IN_REG ? 1 : 0) | (OUT_REG ? 2 : 0), CT, SWZ>(buf, tw, t, active, reg);
 }
 
 // Copy an N-entry twiddle table from global memory into LDS (no barrier).
