@@ -518,7 +518,7 @@ void spread_bp(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& src
         dim3 grd((unsigned)nwg);
         if (CG == 1) hipLaunchKernelGGL(bp_gather_kernel<1>, grd, dim3(64), 0, st, a);
         else if (CG == 8) hipLaunchKernelGGL(bp_gather_kernel<8>, grd, dim3(64), 0, st, a);
-        else if (M <= BP_NARROW_POINTS)
+        else if (!bp_wide_offsets(M))
             hipLaunchKernelGGL(bp_gather_mfma_kernel<false>, grd, dim3(64), 0, st, a);
         else
             hipLaunchKernelGGL(bp_gather_mfma_kernel<true>, grd, dim3(64), 0, st, a);

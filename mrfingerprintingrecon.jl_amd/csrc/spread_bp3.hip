@@ -527,7 +527,7 @@ void spread_bp3(const msr_traj* traj, const SpreadPlan& plan, const ValueSrc& sr
         dim3 grd((unsigned)nwg);
         if (CG == 1) hipLaunchKernelGGL(bp3_gather_kernel<1>, grd, dim3(64), 0, st, a);
         else if (CG == 8) hipLaunchKernelGGL(bp3_gather_kernel<8>, grd, dim3(64), 0, st, a);
-        else if (M <= BP_NARROW_POINTS)
+        else if (!bp_wide_offsets(M))
             hipLaunchKernelGGL(bp3_gather_mfma_kernel<false>, grd, dim3(64), 0, st, a);
         else
             hipLaunchKernelGGL(bp3_gather_mfma_kernel<true>, grd, dim3(64), 0, st, a);

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cstdlib>
 #include <numeric>
 #include <vector>
 
@@ -172,6 +173,14 @@ __host__ __device__ constexpr int mfma_channel(int r, int h) {
 constexpr int BP_PAD_PAIRS = 8 * BP_DEPTH;
 // groups of at most this many points address their values with 32-bit offsets
 constexpr int64_t BP_NARROW_POINTS = (int64_t)1 << 25;
+
+// the 64-bit value offsets: for groups above BP_NARROW_POINTS points, and on
+// request (environment MSR_BP_WIDE_OFFSETS=1) to test them at small sizes
+inline bool bp_wide_offsets(int64_t M) {
+    if (M > BP_NARROW_POINTS) return true;
+    const char* e = std::getenv("MSR_BP_WIDE_OFFSETS");
+    return e != nullptr && e[0] == '1';
+}
 
 __device__ __forceinline__ float bp_ldf(const char* __restrict__ base, uint32_t off) {
     return *reinterpret_cast<const float*>(base + (size_t)off);

@@ -180,6 +180,22 @@ def test_3d_spreading_atomic_and_slab_paths_agree(torch, msr, monkeypatch):
     assert rel_err(lam_a, lam_ref) < 3e-7 and rel_err(b_a, b_ref) < 2e-6
 
 
+def test_backprojection3d_gather_wide_offsets_same_bits(torch, msr, monkeypatch):
+    """The 3D matrix-core gather with its 64-bit value offsets (taken above 2^25
+    samples) forced at a small size: the same bits as the 32-bit offsets."""
+    N = (16, 16, 12)
+    trj, U, cm, x = kooshball_case(N, 3, 4, 6, 5)   # 12 channels: one group of 16
+    cml = [dev(torch, c) for c in cm]
+    rng = np.random.default_rng(9)
+    data = (rng.normal(size=(4,) + trj.shape[:2]) + 1j * rng.normal(size=(4,) + trj.shape[:2])).astype(np.complex64)
+    args = (dev(torch, data), dev(torch, trj), cml)
+    b1 = msr.calculate_backprojection(*args, U=dev(torch, U)).cpu().numpy()
+    monkeypatch.setenv("MSR_BP_WIDE_OFFSETS", "1")
+    b2 = msr.calculate_backprojection(*args, U=dev(torch, U)).cpu().numpy()
+    assert np.array_equal(b1, b2)
+    assert np.linalg.norm(b1) > 0
+
+
 def test_recon3d_compact_solve(torch, msr):
     """The f64c precision in 3D: FP64 arithmetic with the apply's T1 / T2
     intermediates stored in 8 bytes (four storage roundings of 2^-27 per

@@ -289,6 +289,22 @@ def test_backprojection_and_kernel_are_bit_reproducible(torch, msr):
     assert np.array_equal(l1, l2)
 
 
+def test_backprojection_gather_wide_offsets_same_bits(torch, msr, monkeypatch):
+    """The matrix-core gather's 64-bit value offsets (taken above 2^25 samples)
+    forced at a small size: the same bits as the 32-bit offsets."""
+    N = (48, 40)
+    trj, U, cm, x = radial_case(N, 3, 4, 10, 3)   # 12 channels: one group of 16
+    rng = np.random.default_rng(6)
+    data = (rng.normal(size=(4,) + trj.shape[:2]) + 1j * rng.normal(size=(4,) + trj.shape[:2])).astype(np.complex64)
+    cml = [dev(torch, c) for c in cm]
+    args = (dev(torch, data), dev(torch, trj), cml)
+    b1 = msr.calculate_backprojection(*args, U=dev(torch, U)).cpu().numpy()
+    monkeypatch.setenv("MSR_BP_WIDE_OFFSETS", "1")
+    b2 = msr.calculate_backprojection(*args, U=dev(torch, U)).cpu().numpy()
+    assert np.array_equal(b1, b2)
+    assert np.linalg.norm(b1) > 0
+
+
 @pytest.mark.parametrize("masked,C", [(False, 3), (True, 3), (False, 6), (True, 9), (False, 0)])
 def test_forward_model_matches_oracle_and_is_adjoint(torch, msr, masked, C):
     """Forward (type-2) model vs the FP64 oracle, and <A x, y> = <x, A^H y> with
