@@ -422,7 +422,7 @@ size_t op_footprint(const msr_normalop* op, int prec) {
     size_t cg = vs * n * 2 + es * n;                                   // r, u, c
     if (v64) cg += sizeof(double2) * n;                                // x in FP64
     if (prec == MSR_PREC_MIXED) cg += sizeof(float2) * n;
-    cg += sizeof(CgScalars) + sizeof(double) * 2 * RED_BLOCKS;
+    cg += sizeof(CgScalars) + sizeof(double) * CG_PART_DOUBLES;
     const size_t maps = op->identity_cmaps ? 0 : sizeof(float2) * (size_t)op->C * op->V;
     return maps + std::max(scratch, (size_t)op->T.bytes + op->T2.bytes + op->partial.bytes) + cg;
 }
@@ -441,7 +441,7 @@ void op_cg_buffers(msr_normalop* op, int prec) {
     if (v64) op->cg_x64.alloc(sizeof(double2) * n);
     if (prec == MSR_PREC_MIXED) op->cg_u32.alloc(sizeof(float2) * n);
     op->cg_scal.alloc(sizeof(CgScalars));
-    op->cg_part.alloc(sizeof(double) * 2 * RED_BLOCKS);
+    op->cg_part.alloc(sizeof(double) * CG_PART_DOUBLES);
     op_ensure_scratch(op, a64);
 }
 
@@ -485,8 +485,8 @@ static void cg_iteration(msr_normalop* op, const CgVecs<VT, AT>& v, CgScalars* S
     const int64_t n = op->V * op->K;
     launch_cg_u(v, n, S, st);
     op_passes(op, v.ua, am, st, &S->done);
-    launch_cg_reduce_dot(op->partial.as<AT>(), op->G, n, v, op_scale(op), part, S, st);
-    launch_cg_xr(v, n, part, S, st);
+    launch_cg_reduce_dot(op->partial.as<AT>(), op->G, n, v, op_scale(op), part, S, st, false);
+    launch_cg_xr_step(v, n, part, S, st);
 }
 
 // c = A x0 in the apply precision
@@ -675,7 +675,7 @@ size_t cg_noalloc_bytes(int64_t n, int prec) {
     size_t b = 2 * align256(vs * n) + align256(es * n);        // r, u, c
     if (v64) b += align256(sizeof(double2) * n);               // x in FP64
     if (prec == MSR_PREC_MIXED) b += align256(sizeof(float2) * n);
-    return b + align256(sizeof(CgScalars)) + align256(sizeof(double) * 2 * RED_BLOCKS);
+    return b + align256(sizeof(CgScalars)) + align256(sizeof(double) * CG_PART_DOUBLES);
 }
 
 template <class VT, class AT>
@@ -698,7 +698,7 @@ static void cg_noalloc_run(msr_normalop* op, unsigned char* base, float2* x, con
     if constexpr (std::is_same<VT, AT>::value) v.ua = v.u;
     else v.ua = reinterpret_cast<AT*>(take(sizeof(AT) * n));
     auto* S = reinterpret_cast<CgScalars*>(take(sizeof(CgScalars)));
-    auto* part = reinterpret_cast<double*>(take(sizeof(double) * 2 * RED_BLOCKS));
+    auto* part = reinterpret_cast<double*>(take(sizeof(double) * CG_PART_DOUBLES));
     // every buffer above is written by the start kernels before it is read
     // (r, u, x and the scalars by cg_start; c, part by each iteration), so
     // nothing carried over from an earlier call is used

@@ -215,15 +215,45 @@ __device__ __forceinline__ void cg_alpha_body(const double* __restrict__ part, i
     S->alpha_im = -r2 * di / den;
 }
 
-// x += alpha u; r -= alpha c; partial ||r||^2
+// x += alpha u; r -= alpha c; partial ||r||^2.  With dpart, alpha is formed
+// here from the nb dot partials (as cg_alpha_body does: one wavefront sums them
+// in a fixed order, so every block gets the same value) and block 0 stores it
+// and the dot in Sw.
 template <class VT, class AT>
 __device__ __forceinline__ void cg_xr_body(const CgVecs<VT, AT>& v, int64_t n,
-                                           double* __restrict__ part,
-                                           const CgScalars* __restrict__ S, int blk, int nblk) {
+                                           double* __restrict__ part, const CgScalars* S,
+                                           int blk, int nblk, const double* dpart = nullptr,
+                                           int dnb = 0, CgScalars* Sw = nullptr) {
     if (S->done) return;
     using T = real_t<VT>;
     __shared__ double s[RED_THREADS / 64];
-    const T ar = (T)S->alpha_re, ai = (T)S->alpha_im;
+    double alr, ali;
+    if (dpart) {
+        __shared__ double s_alpha[2];
+        if (threadIdx.x < 64) {
+            const double dr = wave_sum_partials(dpart, dnb, 2, 0);
+            const double di = wave_sum_partials(dpart, dnb, 2, 1);
+            if (threadIdx.x == 0) {
+                const double r2 = S->res * S->res;
+                const double den = dr * dr + di * di;
+                s_alpha[0] = r2 * dr / den;
+                s_alpha[1] = -r2 * di / den;
+                if (blk == 0) {
+                    Sw->dot_re = dr;
+                    Sw->dot_im = di;
+                    Sw->alpha_re = s_alpha[0];
+                    Sw->alpha_im = s_alpha[1];
+                }
+            }
+        }
+        __syncthreads();
+        alr = s_alpha[0];
+        ali = s_alpha[1];
+    } else {
+        alr = S->alpha_re;
+        ali = S->alpha_im;
+    }
+    const T ar = (T)alr, ai = (T)ali;
     double acc = 0.0;
     for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)nblk * blockDim.x) {
@@ -311,11 +341,13 @@ __global__ void cg_alpha_kernel(const double* __restrict__ part, int nb, CgScala
 
 template <class VT, class AT>
 void launch_cg_reduce_dot(const AT* partial, int G, int64_t n, const CgVecs<VT, AT>& v,
-                          double scale, double* part, CgScalars* S, hipStream_t st) {
+                          double scale, double* part, CgScalars* S, hipStream_t st,
+                          bool step_kernel) {
     const int g = red_grid(n);
     hipLaunchKernelGGL((cg_reduce_dot_kernel<VT, AT>), dim3(g), dim3(RED_THREADS), 0, st, partial,
                        G, n, v, scale, part, S);
     MSR_CHECK_LAUNCH();
+    if (!step_kernel) return;
     hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(64), 0, st, part, g, S);
     MSR_CHECK_LAUNCH();
 }
@@ -329,6 +361,25 @@ cg_xr_kernel(CgVecs<VT, AT> v, int64_t n, double* __restrict__ part,
 
 __global__ void cg_iter_finish(const double* __restrict__ part, int nb, CgScalars* S) {
     cg_iter_finish_body(part, nb, S);
+}
+
+template <class VT, class AT>
+__global__ void __launch_bounds__(RED_THREADS)
+cg_xr_step_kernel(CgVecs<VT, AT> v, int64_t n, const double* __restrict__ dpart, int dnb,
+                  double* __restrict__ part, CgScalars* S) {
+    cg_xr_body(v, n, part, S, blockIdx.x, gridDim.x, dpart, dnb, S);
+}
+
+template <class VT, class AT>
+void launch_cg_xr_step(const CgVecs<VT, AT>& v, int64_t n, double* part, CgScalars* S,
+                       hipStream_t st) {
+    const int g = red_grid(n);
+    double* rpart = part + 2 * RED_BLOCKS;
+    hipLaunchKernelGGL((cg_xr_step_kernel<VT, AT>), dim3(g), dim3(RED_THREADS), 0, st, v, n, part,
+                       g, rpart, S);
+    MSR_CHECK_LAUNCH();
+    hipLaunchKernelGGL(cg_iter_finish, dim3(1), dim3(64), 0, st, rpart, g, S);
+    MSR_CHECK_LAUNCH();
 }
 
 template <class VT, class AT>
@@ -474,9 +525,11 @@ void launch_to_f32_batch(const CgJob<double2, AT>* jobs, const int* pre, int nj,
                                           int, bool, hipStream_t);                               \
     template void launch_cg_u<VT, AT>(const CgVecs<VT, AT>&, int64_t, CgScalars*, hipStream_t);  \
     template void launch_cg_reduce_dot<VT, AT>(const AT*, int, int64_t, const CgVecs<VT, AT>&,   \
-                                               double, double*, CgScalars*, hipStream_t);        \
+                                               double, double*, CgScalars*, hipStream_t, bool);  \
     template void launch_cg_xr<VT, AT>(const CgVecs<VT, AT>&, int64_t, double*, CgScalars*,      \
                                        hipStream_t);                                             \
+    template void launch_cg_xr_step<VT, AT>(const CgVecs<VT, AT>&, int64_t, double*, CgScalars*, \
+                                            hipStream_t);                                        \
     template void launch_cg_start_batch<VT, AT>(const CgJob<VT, AT>*, const int*, int, int,      \
                                                 bool, bool, double, double, int, hipStream_t);   \
     template void launch_cg_u_batch<VT, AT>(const CgJob<VT, AT>*, const int*, int, int,          \

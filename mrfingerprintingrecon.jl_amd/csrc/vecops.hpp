@@ -15,6 +15,10 @@ namespace msr {
 
 constexpr int RED_BLOCKS = 512;   // blocks of the two-level FP64 reductions
 constexpr int RED_THREADS = 256;
+// reduction partials of a single-operator CG: the dot partials (two per block)
+// in [0, 2 RED_BLOCKS), the residual partials of the fused step in
+// [2 RED_BLOCKS, 3 RED_BLOCKS)
+constexpr int CG_PART_DOUBLES = 3 * RED_BLOCKS;
 
 inline int red_grid(int64_t n) {
     int64_t g = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
@@ -54,11 +58,20 @@ void launch_cg_start(const float2* b, const float2* x_in, const CgVecs<VT, AT>& 
 template <class VT, class AT>
 void launch_cg_u(const CgVecs<VT, AT>& v, int64_t n, CgScalars* S, hipStream_t st);
 // c = scale*sum_g partial[g]; dot(u, c) in FP64; alpha = res^2 / dot
+// (step_kernel = false: no alpha kernel; launch_cg_xr_step forms alpha)
 template <class VT, class AT>
 void launch_cg_reduce_dot(const AT* partial, int G, int64_t n, const CgVecs<VT, AT>& v,
-                          double scale, double* part, CgScalars* S, hipStream_t st);
+                          double scale, double* part, CgScalars* S, hipStream_t st,
+                          bool step_kernel = true);
 template <class VT, class AT>
 void launch_cg_xr(const CgVecs<VT, AT>& v, int64_t n, double* part, CgScalars* S, hipStream_t st);
+// x += alpha u; r -= alpha c with alpha = res^2 / dot formed by every block
+// from the dot partials in part (the same value in each: one wavefront sums
+// them in a fixed order), so no alpha kernel runs; the residual partials go to
+// part + 2 RED_BLOCKS (part holds CG_PART_DOUBLES)
+template <class VT, class AT>
+void launch_cg_xr_step(const CgVecs<VT, AT>& v, int64_t n, double* part, CgScalars* S,
+                       hipStream_t st);
 // x32 = (float) x64
 void launch_to_f32(const double2* x64, float2* x32, int64_t n, hipStream_t st);
 
