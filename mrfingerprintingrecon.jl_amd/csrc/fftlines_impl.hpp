@@ -126,8 +126,16 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
         out[idx] = v;
     };
     if constexpr (!(sizeof(CT) == 16)) load_twiddles<NF>(s_tw_lds, tw);
-    const int64_t col0 = (int64_t)blockIdx.x * WC;
-    const int64_t b = blockIdx.y;
+    // Workgroups go to the 8 XCDs round-robin by linear id, so neighbouring
+    // column groups (the two halves of a 128-byte line when WC values are 64
+    // bytes) would land on different L2s. Tile 2u+1 is therefore taken by the
+    // workgroup 8 ids after tile 2u's, on the same XCD and dispatched next to
+    // it (a bijection on each run of 16 ids; a ragged tail keeps its own).
+    const unsigned nbx = gridDim.x, total = nbx * gridDim.y;
+    unsigned L = blockIdx.x + nbx * blockIdx.y;
+    if (L < total - total % 16u) L = (((L >> 4) << 3) + (L & 7u)) * 2u + ((L >> 3) & 1u);
+    const int64_t col0 = (int64_t)(L % nbx) * WC;
+    const int64_t b = L / nbx;
     if (p.isel == SEL_FIRST && !p.win && !p.win64) {
         for (int e = threadIdx.x; e < WC * NF; e += BLK) {
             const int wc = e % WC, i = e / WC;
