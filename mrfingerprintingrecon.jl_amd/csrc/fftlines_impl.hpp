@@ -136,7 +136,40 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
     if (L < total - total % 16u) L = (((L >> 4) << 3) + (L & 7u)) * 2u + ((L >> 3) & 1u);
     const int64_t col0 = (int64_t)(L % nbx) * WC;
     const int64_t b = L / nbx;
-    if (p.isel == SEL_FIRST && !p.win && !p.win64) {
+    // Compact (8-byte) columns in pairs: one 16-byte load or store covers two
+    // adjacent columns, half the memory instructions of single values, with
+    // the same decode / encode per value.  Needs WC even, even strides and a
+    // 16-byte aligned base, so that every pair (col0 is a multiple of WC) is
+    // aligned; a last odd column goes alone.
+    constexpr bool PAIRED = sizeof(CT) == 16 && WC % 2 == 0;
+    bool pair_in = false, pair_out = false;
+    if constexpr (PAIRED) {
+        pair_in = p.in_compact && ((p.in_stride | p.in_bstride) & 1) == 0 &&
+                  (reinterpret_cast<uintptr_t>(p.in) & 15u) == 0;
+        pair_out = p.out_compact && !p.lam && ((p.out_stride | p.out_bstride) & 1) == 0 &&
+                   (reinterpret_cast<uintptr_t>(p.out) & 15u) == 0;
+    }
+    if (p.isel == SEL_FIRST && !p.win && !p.win64 && pair_in) {
+        if constexpr (PAIRED) {
+            for (int e = threadIdx.x; e < (WC / 2) * NF; e += BLK) {
+                const int wc = 2 * (e % (WC / 2)), i = e / (WC / 2);
+                const int64_t col = col0 + wc;
+                CT v0 = cz<CT>(), v1 = cz<CT>();
+                if (i < p.lin) {
+                    const cx64* q = static_cast<const cx64*>(p.in) + (b * p.in_bstride + (int64_t)i * p.in_stride + col);
+                    if (col + 1 < p.ncols) {
+                        const uint4 w = *reinterpret_cast<const uint4*>(q);
+                        v0 = cx64_decode(cx64{w.x, w.y});
+                        v1 = cx64_decode(cx64{w.z, w.w});
+                    } else if (col < p.ncols) {
+                        v0 = cx64_decode(*q);
+                    }
+                }
+                s[wc * LD + lds_pad<NF, CT>(i)] = v0;
+                s[(wc + 1) * LD + lds_pad<NF, CT>(i)] = v1;
+            }
+        }
+    } else if (p.isel == SEL_FIRST && !p.win && !p.win64) {
         for (int e = threadIdx.x; e < WC * NF; e += BLK) {
             const int wc = e % WC, i = e / WC;
             const int64_t col = col0 + wc;
@@ -162,6 +195,33 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
 #pragma unroll
     for (int r0 = 0; r0 < WC; r0 += F) fft_lds<NF, DIR>(s + (r0 + g) * LD, s_tw, t);
+    if constexpr (PAIRED) {
+        if (pair_out) {
+            for (int e = threadIdx.x; e < (WC / 2) * p.lout; e += BLK) {
+                const int wc = 2 * (e % (WC / 2)), j = e / (WC / 2);
+                const int64_t col = col0 + wc;
+                if (col >= p.ncols) continue;
+                const int bin = lds_pad<NF, CT>(select_bin(j, p.lout, NF, p.sel));
+                CT v0 = s[wc * LD + bin], v1 = s[(wc + 1) * LD + bin];
+                if (p.w64) {
+                    v0 = cscale(v0, (RT)p.w64[j]);
+                    v1 = cscale(v1, (RT)p.w64[j]);
+                } else if (p.w) {
+                    v0 = cscale(v0, (RT)p.w[j]);
+                    v1 = cscale(v1, (RT)p.w[j]);
+                }
+                cx64* q = static_cast<cx64*>(p.out) + (b * p.out_bstride + (int64_t)j * p.out_stride + col);
+                const cx64 c0 = cx64_encode(v0.x, v0.y);
+                if (col + 1 < p.ncols) {
+                    const cx64 c1 = cx64_encode(v1.x, v1.y);
+                    *reinterpret_cast<uint4*>(q) = make_uint4(c0.w0, c0.w1, c1.w0, c1.w1);
+                } else {
+                    *q = c0;
+                }
+            }
+            return;
+        }
+    }
     for (int e = threadIdx.x; e < WC * p.lout; e += BLK) {
         const int wc = e % WC, j = e / WC;
         const int64_t col = col0 + wc;
