@@ -5,6 +5,8 @@
 #include "fft.hpp"
 #include "fftlines.hpp"
 
+#include <cstdlib>
+
 namespace msr {
 
 // LDS budget of a line-pass workgroup (gfx950: 160 KiB per CU), with room for
@@ -144,9 +146,9 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
     constexpr bool PAIRED = sizeof(CT) == 16 && WC % 2 == 0;
     bool pair_in = false, pair_out = false;
     if constexpr (PAIRED) {
-        pair_in = p.in_compact && ((p.in_stride | p.in_bstride) & 1) == 0 &&
+        pair_in = !p.single && p.in_compact && ((p.in_stride | p.in_bstride) & 1) == 0 &&
                   (reinterpret_cast<uintptr_t>(p.in) & 15u) == 0;
-        pair_out = p.out_compact && !p.lam && ((p.out_stride | p.out_bstride) & 1) == 0 &&
+        pair_out = !p.single && p.out_compact && !p.lam && ((p.out_stride | p.out_bstride) & 1) == 0 &&
                    (reinterpret_cast<uintptr_t>(p.out) & 15u) == 0;
     }
     if (p.isel == SEL_FIRST && !p.win && !p.win64 && pair_in) {
@@ -259,8 +261,17 @@ void launch_rows_n(const LinePass& p, const CT* tw, hipStream_t st) {
     else hipLaunchKernelGGL((rows_kernel<N, 1, CT>), grid, dim3(BLK), 0, st, p, tw);
 }
 
+// MSR_COLS_SINGLE=1 in the environment: compact column values one per access
+// (to compare with the paired accesses)
+inline bool cols_single_forced() {
+    const char* e = std::getenv("MSR_COLS_SINGLE");
+    return e != nullptr && e[0] == '1';
+}
+
 template <int N, class CT>
-void launch_cols_n(const LinePass& p, const CT* tw, hipStream_t st) {
+void launch_cols_n(const LinePass& p0, const CT* tw, hipStream_t st) {
+    LinePass p = p0;
+    p.single = cols_single_forced() ? 1 : 0;
     constexpr int WC = cols_wc<N, CT>();
     constexpr int BLK = (WC * FFTPlan<N>::T) < 256 ? WC * FFTPlan<N>::T : 256;
     dim3 grid((unsigned)ceil_div(p.ncols, WC), (unsigned)p.nb);

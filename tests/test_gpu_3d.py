@@ -220,3 +220,32 @@ def test_recon3d_compact_solve(torch, msr):
         print(f"3d solve f64c ({mode}) rel err {e:.3e}", flush=True)
         assert e < 2e-5
 
+
+
+@pytest.mark.parametrize("N,K,C", [((16, 16, 16), 3, 3), ((24, 20, 12), 2, 2)])
+def test_compact_column_pairs_same_bits(torch, msr, monkeypatch, N, K, C):
+    """The 3D apply's y-column passes over 8-byte (compact) values load and
+    store two adjacent columns per 16-byte access; MSR_COLS_SINGLE=1 takes one
+    value per access.  The decode / encode of each value is the same, so the
+    compact apply and a compact device CG give the same bits either way."""
+    from test_gpu_apply import make_case
+    print(f"compact column pairs N={N} K={K} C={C}", flush=True)
+    rng = np.random.default_rng(21)
+    lam, km, cm, x = make_case(rng, N, K, C)
+    A = msr.NFFTNormalOp(N, dev(torch, lam), dev(torch, km + 1), cmaps=[dev(torch, c) for c in cm])
+    xd64 = dev(torch, x.astype(np.complex128) + 1e-3j * rng.normal(size=x.shape)).reshape(-1)
+    xd = dev(torch, x).reshape(-1)
+
+    def run():
+        y = torch.empty_like(xd64)
+        A.A.mul_f64_(y, xd64, compact=True)
+        s = msr.cg(A, xd, maxiter=5, reltol=0.0, mode="device", precision="f64c")
+        return y.cpu().numpy(), s.cpu().numpy()
+
+    paired = run()
+    monkeypatch.setenv("MSR_COLS_SINGLE", "1")
+    single = run()
+    for name, a, b in zip(("apply", "cg"), paired, single):
+        print(f"  {name}: max |diff| {np.max(np.abs(a - b)):.3e}", flush=True)
+        assert np.array_equal(a, b), name
+    assert np.linalg.norm(paired[0]) > 0
