@@ -138,19 +138,40 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
     if (L < total - total % 16u) L = (((L >> 4) << 3) + (L & 7u)) * 2u + ((L >> 3) & 1u);
     const int64_t col0 = (int64_t)(L % nbx) * WC;
     const int64_t b = L / nbx;
-    // Compact (8-byte) columns in pairs: one 16-byte load or store covers two
-    // adjacent columns, half the memory instructions of single values, with
-    // the same decode / encode per value.  Needs WC even, even strides and a
-    // 16-byte aligned base, so that every pair (col0 is a multiple of WC) is
-    // aligned; a last odd column goes alone.
-    constexpr bool PAIRED = sizeof(CT) == 16 && WC % 2 == 0;
+    // 8-byte values (compact ComplexF64, or ComplexF32) in pairs: one 16-byte
+    // load or store covers two adjacent columns, half the memory instructions
+    // of single values, with the same decode / encode per value.  Needs WC
+    // even, even strides and a 16-byte aligned base, so that every pair (col0
+    // is a multiple of WC) is aligned; a last odd column goes alone.
+    constexpr bool PAIRED = WC % 2 == 0;
     bool pair_in = false, pair_out = false;
     if constexpr (PAIRED) {
-        pair_in = !p.single && p.in_compact && ((p.in_stride | p.in_bstride) & 1) == 0 &&
+        const bool in8 = sizeof(CT) == 8 || p.in_compact, out8 = sizeof(CT) == 8 || p.out_compact;
+        pair_in = !p.single && in8 && ((p.in_stride | p.in_bstride) & 1) == 0 &&
                   (reinterpret_cast<uintptr_t>(p.in) & 15u) == 0;
-        pair_out = !p.single && p.out_compact && !p.lam && ((p.out_stride | p.out_bstride) & 1) == 0 &&
+        pair_out = !p.single && out8 && !p.lam && ((p.out_stride | p.out_bstride) & 1) == 0 &&
                    (reinterpret_cast<uintptr_t>(p.out) & 15u) == 0;
     }
+    // the pair of columns at element idx, idx + 1 (pair_in / pair_out only)
+    auto load_pair = [&](int64_t idx, CT& v0, CT& v1) {
+        if constexpr (sizeof(CT) == 16) {
+            const uint4 w = *reinterpret_cast<const uint4*>(static_cast<const cx64*>(p.in) + idx);
+            v0 = cx64_decode(cx64{w.x, w.y});
+            v1 = cx64_decode(cx64{w.z, w.w});
+        } else {
+            const float4 w = *reinterpret_cast<const float4*>(in + idx);
+            v0 = make_float2(w.x, w.y);
+            v1 = make_float2(w.z, w.w);
+        }
+    };
+    auto store_pair = [&](int64_t idx, CT v0, CT v1) {
+        if constexpr (sizeof(CT) == 16) {
+            const cx64 c0 = cx64_encode(v0.x, v0.y), c1 = cx64_encode(v1.x, v1.y);
+            *reinterpret_cast<uint4*>(static_cast<cx64*>(p.out) + idx) = make_uint4(c0.w0, c0.w1, c1.w0, c1.w1);
+        } else {
+            *reinterpret_cast<float4*>(out + idx) = make_float4(v0.x, v0.y, v1.x, v1.y);
+        }
+    };
     if (p.isel == SEL_FIRST && !p.win && !p.win64 && pair_in) {
         if constexpr (PAIRED) {
             for (int e = threadIdx.x; e < (WC / 2) * NF; e += BLK) {
@@ -158,14 +179,9 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
                 const int64_t col = col0 + wc;
                 CT v0 = cz<CT>(), v1 = cz<CT>();
                 if (i < p.lin) {
-                    const cx64* q = static_cast<const cx64*>(p.in) + (b * p.in_bstride + (int64_t)i * p.in_stride + col);
-                    if (col + 1 < p.ncols) {
-                        const uint4 w = *reinterpret_cast<const uint4*>(q);
-                        v0 = cx64_decode(cx64{w.x, w.y});
-                        v1 = cx64_decode(cx64{w.z, w.w});
-                    } else if (col < p.ncols) {
-                        v0 = cx64_decode(*q);
-                    }
+                    const int64_t idx = b * p.in_bstride + (int64_t)i * p.in_stride + col;
+                    if (col + 1 < p.ncols) load_pair(idx, v0, v1);
+                    else if (col < p.ncols) v0 = load_in(idx);
                 }
                 s[wc * LD + lds_pad<NF, CT>(i)] = v0;
                 s[(wc + 1) * LD + lds_pad<NF, CT>(i)] = v1;
@@ -212,14 +228,9 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
                     v0 = cscale(v0, (RT)p.w[j]);
                     v1 = cscale(v1, (RT)p.w[j]);
                 }
-                cx64* q = static_cast<cx64*>(p.out) + (b * p.out_bstride + (int64_t)j * p.out_stride + col);
-                const cx64 c0 = cx64_encode(v0.x, v0.y);
-                if (col + 1 < p.ncols) {
-                    const cx64 c1 = cx64_encode(v1.x, v1.y);
-                    *reinterpret_cast<uint4*>(q) = make_uint4(c0.w0, c0.w1, c1.w0, c1.w1);
-                } else {
-                    *q = c0;
-                }
+                const int64_t idx = b * p.out_bstride + (int64_t)j * p.out_stride + col;
+                if (col + 1 < p.ncols) store_pair(idx, v0, v1);
+                else store_out(idx, v0);
             }
             return;
         }
@@ -261,7 +272,7 @@ void launch_rows_n(const LinePass& p, const CT* tw, hipStream_t st) {
     else hipLaunchKernelGGL((rows_kernel<N, 1, CT>), grid, dim3(BLK), 0, st, p, tw);
 }
 
-// MSR_COLS_SINGLE=1 in the environment: compact column values one per access
+// MSR_COLS_SINGLE=1 in the environment: 8-byte column values one per access
 // (to compare with the paired accesses)
 inline bool cols_single_forced() {
     const char* e = std::getenv("MSR_COLS_SINGLE");

@@ -224,10 +224,11 @@ def test_recon3d_compact_solve(torch, msr):
 
 @pytest.mark.parametrize("N,K,C", [((16, 16, 16), 3, 3), ((24, 20, 12), 2, 2)])
 def test_compact_column_pairs_same_bits(torch, msr, monkeypatch, N, K, C):
-    """The 3D apply's y-column passes over 8-byte (compact) values load and
-    store two adjacent columns per 16-byte access; MSR_COLS_SINGLE=1 takes one
-    value per access.  The decode / encode of each value is the same, so the
-    compact apply and a compact device CG give the same bits either way."""
+    """The 3D apply's y-column passes over 8-byte values (compact ComplexF64,
+    ComplexF32) load and store two adjacent columns per 16-byte access;
+    MSR_COLS_SINGLE=1 takes one value per access.  The decode / encode of each
+    value is the same, so the compact apply, a compact device CG and the
+    ComplexF32 apply give the same bits either way."""
     from test_gpu_apply import make_case
     print(f"compact column pairs N={N} K={K} C={C}", flush=True)
     rng = np.random.default_rng(21)
@@ -240,12 +241,37 @@ def test_compact_column_pairs_same_bits(torch, msr, monkeypatch, N, K, C):
         y = torch.empty_like(xd64)
         A.A.mul_f64_(y, xd64, compact=True)
         s = msr.cg(A, xd, maxiter=5, reltol=0.0, mode="device", precision="f64c")
-        return y.cpu().numpy(), s.cpu().numpy()
+        return y.cpu().numpy(), s.cpu().numpy(), (A @ xd).cpu().numpy()
 
     paired = run()
     monkeypatch.setenv("MSR_COLS_SINGLE", "1")
     single = run()
-    for name, a, b in zip(("apply", "cg"), paired, single):
+    for name, a, b in zip(("apply", "cg", "f32 apply"), paired, single):
         print(f"  {name}: max |diff| {np.max(np.abs(a - b)):.3e}", flush=True)
         assert np.array_equal(a, b), name
     assert np.linalg.norm(paired[0]) > 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_backprojection_column_pairs_same_bits(torch, msr, monkeypatch, dim):
+    """The backprojection's ComplexF32 column passes with two adjacent columns
+    per 16-byte access and with one value per access (MSR_COLS_SINGLE=1): the
+    same bits, in 2D and 3D."""
+    if dim == 3:
+        N = (16, 16, 12)
+        trj, U, cm, _ = kooshball_case(N, 3, 4, 6, 5)
+    else:
+        N = (40, 36)
+        trj = msr.synth.traj_2d_radial_goldenratio(80, 3, 10).astype(np.float32)
+        U = msr.synth.t2_basis(10, 3)
+        cm = msr.synth.gaussian_coil_maps(N, 4)
+    cml = [dev(torch, c) for c in cm]
+    rng = np.random.default_rng(13)
+    data = (rng.normal(size=(4,) + trj.shape[:2]) + 1j * rng.normal(size=(4,) + trj.shape[:2])).astype(np.complex64)
+    args = (dev(torch, data), dev(torch, trj), cml)
+    b1 = msr.calculate_backprojection(*args, U=dev(torch, U)).cpu().numpy()
+    monkeypatch.setenv("MSR_COLS_SINGLE", "1")
+    b2 = msr.calculate_backprojection(*args, U=dev(torch, U)).cpu().numpy()
+    print(f"bp column pairs {dim}D: max |diff| {np.max(np.abs(b1 - b2)):.3e}", flush=True)
+    assert np.array_equal(b1, b2)
+    assert np.linalg.norm(b1) > 0
