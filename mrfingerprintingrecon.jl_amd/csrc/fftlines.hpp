@@ -51,9 +51,9 @@ struct LinePass {
     // cols passes with f64: input / output stored as cx64 (8 bytes per value,
     // common.hpp) instead of ComplexF64 (the 3D apply's compact intermediates)
     int in_compact = 0, out_compact = 0;
-    // cols passes over 8-byte values (cx64, ComplexF32): one value per access
-    // instead of two adjacent columns per 16-byte access (set by the launcher
-    // from MSR_COLS_SINGLE=1)
+    // passes over 8-byte values (cx64, ComplexF32): one value per access
+    // instead of two adjacent columns (or two consecutive line elements) per
+    // 16-byte access (set by the launchers from MSR_COLS_SINGLE=1)
     int single = 0;
 };
 

@@ -53,7 +53,35 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
     CT* out = static_cast<CT*>(p.out);
     if constexpr (!(sizeof(CT) == 16)) load_twiddles<NF>(s_tw_lds, tw);
     const int64_t l0 = (int64_t)blockIdx.x * F;
-    if (p.isel == SEL_FIRST && !p.win && !p.win64) {
+    // ComplexF32 lines in pairs of consecutive elements, one 16-byte access
+    // each (even pitches and a 16-byte aligned base; a last odd element goes
+    // alone), as the column passes pair adjacent columns
+    constexpr bool PAIRED = sizeof(CT) == 8 && NF % 2 == 0;
+    const bool pair_in = PAIRED && !p.single && (p.in_pitch & 1) == 0 &&
+                         (reinterpret_cast<uintptr_t>(p.in) & 15u) == 0;
+    const bool pair_out = PAIRED && !p.single && (p.out_pitch & 1) == 0 &&
+                          (reinterpret_cast<uintptr_t>(p.out) & 15u) == 0;
+    if (p.isel == SEL_FIRST && !p.win && !p.win64 && pair_in) {
+        if constexpr (PAIRED) {
+            for (int e = threadIdx.x; e < F * (NF / 2); e += BLK) {
+                const int l = e / (NF / 2), i = 2 * (e - l * (NF / 2));
+                const int64_t line = l0 + l;
+                CT v0 = cz<CT>(), v1 = cz<CT>();
+                if (line < p.nlines) {
+                    const int64_t idx = line * p.in_pitch + i;
+                    if (i + 1 < p.lin) {
+                        const float4 w = *reinterpret_cast<const float4*>(in + idx);
+                        v0 = make_float2(w.x, w.y);
+                        v1 = make_float2(w.z, w.w);
+                    } else if (i < p.lin) {
+                        v0 = in[idx];
+                    }
+                }
+                s[l * LD + lds_pad<NF, CT>(i)] = v0;
+                s[l * LD + lds_pad<NF, CT>(i + 1)] = v1;
+            }
+        }
+    } else if (p.isel == SEL_FIRST && !p.win && !p.win64) {
         for (int e = threadIdx.x; e < F * NF; e += BLK) {
             const int l = e / NF, i = e - l * NF;
             const int64_t line = l0 + l;
@@ -78,6 +106,29 @@ __global__ void __launch_bounds__(256) rows_kernel(LinePass p, const CT* __restr
     __syncthreads();
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
     fft_lds<NF, DIR>(s + g * LD, s_tw, t);
+    if constexpr (PAIRED) {
+        if (pair_out) {
+            const int LP = (p.lout + 1) / 2;
+            for (int e = threadIdx.x; e < F * LP; e += BLK) {
+                const int l = e / LP, j = 2 * (e - l * LP);
+                const int64_t line = l0 + l;
+                if (line >= p.nlines) continue;
+                CT v0 = s[l * LD + lds_pad<NF, CT>(select_bin(j, p.lout, NF, p.sel))];
+                if (p.w64) v0 = cscale(v0, (RT)p.w64[j]);
+                else if (p.w) v0 = cscale(v0, (RT)p.w[j]);
+                const int64_t idx = line * p.out_pitch + j;
+                if (j + 1 < p.lout) {
+                    CT v1 = s[l * LD + lds_pad<NF, CT>(select_bin(j + 1, p.lout, NF, p.sel))];
+                    if (p.w64) v1 = cscale(v1, (RT)p.w64[j + 1]);
+                    else if (p.w) v1 = cscale(v1, (RT)p.w[j + 1]);
+                    *reinterpret_cast<float4*>(out + idx) = make_float4(v0.x, v0.y, v1.x, v1.y);
+                } else {
+                    out[idx] = v0;
+                }
+            }
+            return;
+        }
+    }
     for (int e = threadIdx.x; e < F * p.lout; e += BLK) {
         const int l = e / p.lout, j = e - l * p.lout;
         const int64_t line = l0 + l;
@@ -263,20 +314,22 @@ __global__ void __launch_bounds__(256) cols_kernel(LinePass p, const CT* __restr
     }
 }
 
+// MSR_COLS_SINGLE=1 in the environment: 8-byte line and column values one
+// per access (to compare with the paired accesses)
+inline bool cols_single_forced() {
+    const char* e = std::getenv("MSR_COLS_SINGLE");
+    return e != nullptr && e[0] == '1';
+}
+
 template <int N, class CT>
 void launch_rows_n(const LinePass& p, const CT* tw, hipStream_t st) {
     constexpr int BLK = rows_block<N, CT>();
     constexpr int F = BLK / FFTPlan<N>::T;
     dim3 grid((unsigned)ceil_div(p.nlines, F));
-    if (p.dir < 0) hipLaunchKernelGGL((rows_kernel<N, -1, CT>), grid, dim3(BLK), 0, st, p, tw);
-    else hipLaunchKernelGGL((rows_kernel<N, 1, CT>), grid, dim3(BLK), 0, st, p, tw);
-}
-
-// MSR_COLS_SINGLE=1 in the environment: 8-byte column values one per access
-// (to compare with the paired accesses)
-inline bool cols_single_forced() {
-    const char* e = std::getenv("MSR_COLS_SINGLE");
-    return e != nullptr && e[0] == '1';
+    LinePass q = p;
+    q.single = cols_single_forced() ? 1 : 0;
+    if (q.dir < 0) hipLaunchKernelGGL((rows_kernel<N, -1, CT>), grid, dim3(BLK), 0, st, q, tw);
+    else hipLaunchKernelGGL((rows_kernel<N, 1, CT>), grid, dim3(BLK), 0, st, q, tw);
 }
 
 template <int N, class CT>

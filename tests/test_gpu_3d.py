@@ -254,9 +254,10 @@ def test_compact_column_pairs_same_bits(torch, msr, monkeypatch, N, K, C):
 
 @pytest.mark.parametrize("dim", [2, 3])
 def test_backprojection_column_pairs_same_bits(torch, msr, monkeypatch, dim):
-    """The backprojection's ComplexF32 column passes with two adjacent columns
-    per 16-byte access and with one value per access (MSR_COLS_SINGLE=1): the
-    same bits, in 2D and 3D."""
+    """The backprojection's ComplexF32 line passes with two adjacent columns
+    (column passes) or two consecutive elements (row passes) per 16-byte
+    access and with one value per access (MSR_COLS_SINGLE=1): the same bits,
+    in 2D and 3D."""
     if dim == 3:
         N = (16, 16, 12)
         trj, U, cm, _ = kooshball_case(N, 3, 4, 6, 5)
