@@ -11,6 +11,9 @@
 #include "apply3d_c.hpp"
 #include "fftlines.hpp"
 
+#include <cstdlib>
+#include <type_traits>
+
 namespace msr {
 
 template <int NX2, class CT>
@@ -22,7 +25,8 @@ constexpr int rows3_block() {
 template <int NX2, class CT, class ST>
 __global__ void __launch_bounds__(256)
 pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, ST* __restrict__ T1,
-          const CT* __restrict__ tw, int nx, int64_t nrows, int K, const int* __restrict__ skip) {
+          const CT* __restrict__ tw, int nx, int64_t nrows, int K, const int* __restrict__ skip,
+          int pair) {
     if (skip && *skip) return;
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rows3_block<NX2, CT>();
@@ -50,6 +54,21 @@ pass_a_3d(const CT* __restrict__ b, const float2* __restrict__ cmaps, ST* __rest
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
     fft_lds<NX2, -1>(s + g * LD, s_tw, t);
     ST* out = T1 + ((int64_t)c * K + k) * nrows * NX2;
+    if constexpr (std::is_same<ST, cx64>::value) {
+        if (pair) {
+            // two consecutive cx64 values per 16-byte store (NX2 even)
+            for (int e = threadIdx.x; e < F * (NX2 / 2); e += BLK) {
+                const int l = e / (NX2 / 2), q = 2 * (e - l * (NX2 / 2));
+                const int64_t row = r0 + l;
+                if (row < nrows) {
+                    const CT v0 = s[l * LD + lds_pad<NX2, CT>(q)], v1 = s[l * LD + lds_pad<NX2, CT>(q + 1)];
+                    const cx64 a0 = cx64_encode(v0.x, v0.y), a1 = cx64_encode(v1.x, v1.y);
+                    *reinterpret_cast<uint4*>(out + row * NX2 + q) = make_uint4(a0.w0, a0.w1, a1.w0, a1.w1);
+                }
+            }
+            return;
+        }
+    }
     for (int e = threadIdx.x; e < F * NX2; e += BLK) {
         const int l = e / NX2, q = e - l * NX2;
         const int64_t row = r0 + l;
@@ -61,7 +80,7 @@ template <int NX2, class CT, class ST>
 __global__ void __launch_bounds__(256)
 pass_e_3d(const ST* __restrict__ T1, const float2* __restrict__ cmaps,
           CT* __restrict__ partial, const CT* __restrict__ tw, int nx, int64_t nrows,
-          int K, int C, int G, const int* __restrict__ skip) {
+          int K, int C, int G, const int* __restrict__ skip, int pair) {
     if (skip && *skip) return;
     constexpr int TP = FFTPlan<NX2>::T;
     constexpr int BLK = rows3_block<NX2, CT>();
@@ -82,10 +101,31 @@ pass_e_3d(const ST* __restrict__ T1, const float2* __restrict__ cmaps,
     const int g = threadIdx.x / TP, t = threadIdx.x - g * TP;
     for (int c = c0; c < c1; ++c) {
         const ST* in = T1 + ((int64_t)c * K + k) * nrows * NX2;
-        for (int e = threadIdx.x; e < F * NX2; e += BLK) {
-            const int l = e / NX2, q = e - l * NX2;
-            const int64_t row = r0 + l;
-            s[l * LD + lds_pad<NX2, CT>(q)] = row < nrows ? st_load<CT, ST>(in + row * NX2 + q) : cz<CT>();
+        bool paired = false;
+        if constexpr (std::is_same<ST, cx64>::value) {
+            if (pair) {
+                // two consecutive cx64 values per 16-byte load (NX2 even)
+                for (int e = threadIdx.x; e < F * (NX2 / 2); e += BLK) {
+                    const int l = e / (NX2 / 2), q = 2 * (e - l * (NX2 / 2));
+                    const int64_t row = r0 + l;
+                    CT v0 = cz<CT>(), v1 = cz<CT>();
+                    if (row < nrows) {
+                        const uint4 w = *reinterpret_cast<const uint4*>(in + row * NX2 + q);
+                        v0 = cx64_decode(cx64{w.x, w.y});
+                        v1 = cx64_decode(cx64{w.z, w.w});
+                    }
+                    s[l * LD + lds_pad<NX2, CT>(q)] = v0;
+                    s[l * LD + lds_pad<NX2, CT>(q + 1)] = v1;
+                }
+                paired = true;
+            }
+        }
+        if (!paired) {
+            for (int e = threadIdx.x; e < F * NX2; e += BLK) {
+                const int l = e / NX2, q = e - l * NX2;
+                const int64_t row = r0 + l;
+                s[l * LD + lds_pad<NX2, CT>(q)] = row < nrows ? st_load<CT, ST>(in + row * NX2 + q) : cz<CT>();
+            }
         }
         __syncthreads();
         fft_lds<NX2, +1>(s + g * LD, s_tw, t);
@@ -132,6 +172,14 @@ static void launch_pass_c_3d(const Apply3DArgs& a, hipStream_t st, const int* sk
     MSR_CHECK_LAUNCH();
 }
 
+// 16-byte accesses of two consecutive cx64 row values in passes A and E: T1
+// 16-byte aligned (each row of 2Nx values then is), and MSR_COLS_SINGLE=1 not
+// set (one value per access, to compare)
+static int rows3_pair(const void* T1) {
+    const char* e = std::getenv("MSR_COLS_SINGLE");
+    return (e == nullptr || e[0] != '1') && (reinterpret_cast<uintptr_t>(T1) & 15u) == 0 ? 1 : 0;
+}
+
 template <class CT, class ST>
 static void run_row_passes_a(const Apply3DArgs& a, const void* b, hipStream_t st, const int* skip) {
     const int NX2 = 2 * a.nx;
@@ -142,7 +190,7 @@ static void run_row_passes_a(const Apply3DArgs& a, const void* b, hipStream_t st
             dim3 grid((unsigned)ceil_div(nrows, F), a.K, a.C);
             hipLaunchKernelGGL((pass_a_3d<N, CT, ST>), grid, dim3(rows3_block<N, CT>()), 0, st,
                                static_cast<const CT*>(b), a.cmaps, static_cast<ST*>(a.T1),
-                               static_cast<const CT*>(a.tw_x), a.nx, nrows, a.K, skip);
+                               static_cast<const CT*>(a.tw_x), a.nx, nrows, a.K, skip, rows3_pair(a.T1));
         } else {
             fail(MSR_ERR_UNSUPPORTED, "Toeplitz grid 2Nx = %d above 512", N);
         }
@@ -160,7 +208,8 @@ static void run_row_passes_e(const Apply3DArgs& a, hipStream_t st, const int* sk
             dim3 grid((unsigned)ceil_div(nrows, F), a.K, a.G);
             hipLaunchKernelGGL((pass_e_3d<N, CT, ST>), grid, dim3(rows3_block<N, CT>()), 0, st,
                                static_cast<const ST*>(a.T1), a.cmaps, static_cast<CT*>(a.partial),
-                               static_cast<const CT*>(a.tw_x), a.nx, nrows, a.K, a.C, a.G, skip);
+                               static_cast<const CT*>(a.tw_x), a.nx, nrows, a.K, a.C, a.G, skip,
+                               rows3_pair(a.T1));
         }
     });
     MSR_CHECK_LAUNCH();

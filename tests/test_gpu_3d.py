@@ -225,7 +225,8 @@ def test_recon3d_compact_solve(torch, msr):
 @pytest.mark.parametrize("N,K,C", [((16, 16, 16), 3, 3), ((24, 20, 12), 2, 2)])
 def test_compact_column_pairs_same_bits(torch, msr, monkeypatch, N, K, C):
     """The 3D apply's y-column passes over 8-byte values (compact ComplexF64,
-    ComplexF32) load and store two adjacent columns per 16-byte access;
+    ComplexF32) load and store two adjacent columns per 16-byte access, and its
+    x-row passes A and E two consecutive compact values;
     MSR_COLS_SINGLE=1 takes one value per access.  The decode / encode of each
     value is the same, so the compact apply, a compact device CG and the
     ComplexF32 apply give the same bits either way."""
